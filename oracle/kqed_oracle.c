@@ -1,0 +1,1495 @@
+/*
+ * kqed_oracle.c -- CPU oracle (test infrastructure) for KQED's QED kernel L0..L3 / M-L2.
+ *
+ * A plain-C restatement of the reference's FP64 CPU path as built with AVX+FMA (the build the
+ * reference README recommends, README.md:7-13).  The AVX build differs from the scalar build in
+ * three observable ways and this oracle follows the AVX one:
+ *   - extractff2 takes the x derivative (F[1]) as a lerp of the d/dx table, never a Hermite
+ *     step (getff-new.c:274-277 with interpol5 at getff-new.c:69-90);
+ *   - the swap-regime second y-derivative finite difference of chnr_dV.c:454-465 evaluates
+ *     both ends with the same stencil weights (accessv6 uses Inv.INVy, not Inv1.y), so it is
+ *     exactly zero unless one end sits on the top y edge;
+ *   - Clenshaw sums run in the FMA form of simd.c:79-118 with ratio look-up tables.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this file's
+ * library.  Compile with -ffp-contract=off.
+ */
+#include "kqed_oracle.h"
+
+#include <math.h>
+#include <stdbool.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+#undef I /* the name I is used for the invariants below */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ flop accounting */
+static __thread double g_flops = 0.0;
+#define FL(n) (g_flops += (double)(n))
+double oq_flops_get(void) { return g_flops; }
+void oq_flops_reset(void) { g_flops = 0.0; }
+
+/* ------------------------------------------------------------------ CRC-32C (crc32c.c:160-172) */
+static uint32_t crc_tab[256];
+static int crc_ready = 0;
+static void crc_init(void)
+{
+  /* reflected Castagnoli polynomial 0x1EDC6F41 -> 0x82F63B78 */
+  for (uint32_t i = 0; i < 256; i++) {
+    uint32_t c = i;
+    for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ 0x82F63B78u : (c >> 1);
+    crc_tab[i] = c;
+  }
+  crc_ready = 1;
+}
+uint32_t oq_crc32c(uint32_t crc, const void *buf, size_t len)
+{
+  if (!crc_ready) crc_init();
+  const unsigned char *p = (const unsigned char *)buf;
+  /* the reference's SSE4.2 path consumes whole 32-bit words only (crc32c.c:62-72) */
+  len &= ~(size_t)3;
+  while (len--) crc = crc_tab[(crc ^ *p++) & 0xFFu] ^ (crc >> 8);
+  return crc;
+}
+static inline uint32_t rotl32(uint32_t w, uint32_t r) { return r ? (w << r) | (w >> (32 - r)) : w; }
+void oq_checksum_accum(uint32_t *a, uint32_t *b, uint32_t rank, const void *buf, size_t len)
+{
+  const uint32_t w = oq_crc32c(0, buf, len);
+  *a ^= rotl32(w, rank % 29);
+  *b ^= rotl32(w, rank % 31);
+}
+
+/* ------------------------------------------------------------------ table I/O (io.c:42-288) */
+#define MAGIC 816968
+static int rd(void *p, size_t sz, size_t n, FILE *f) { return fread(p, sz, n, f) != n; }
+
+static int load_ff(const char *dir, oq_tables *T)
+{
+  char path[4096];
+  snprintf(path, sizeof path, "%s/FFxy_single_cksum.bin", dir);
+  FILE *f = fopen(path, "rb");
+  if (!f) { fprintf(stderr, "oracle: cannot open %s\n", path); return 1; }
+  int magic, n;
+  uint32_t ck[2], a, b;
+  if (rd(&magic, 4, 1, f) || magic != MAGIC) goto bad;
+  if (rd(&T->nstpx, 4, 1, f) || T->nstpx < 2 || T->nstpx > 100) goto bad;
+  T->XX = malloc(sizeof(double) * T->nstpx);
+  if (rd(T->XX, 8, T->nstpx, f) || rd(ck, 4, 2, f)) goto bad;
+  a = b = 0; oq_checksum_accum(&a, &b, 0, T->XX, 8 * (size_t)T->nstpx);
+  if (a != ck[0] || b != ck[1]) goto bad;
+  if (rd(&T->nstpy, 4, 1, f) || T->nstpy < 2) goto bad;
+  T->YY = malloc(sizeof(double) * T->nstpy);
+  if (rd(T->YY, 8, T->nstpy, f) || rd(ck, 4, 2, f)) goto bad;
+  a = b = 0; oq_checksum_accum(&a, &b, 0, T->YY, 8 * (size_t)T->nstpy);
+  if (a != ck[0] || b != ck[1]) goto bad;
+  T->xstp = T->XX[1] - T->XX[0];
+  T->ystp = T->YY[1] - T->YY[0];
+  if (rd(&T->Nffa, 4, 1, f) || T->Nffa != 14) goto bad;
+  /* first pass over the payload to learn nfx: rows are length-prefixed */
+  long start = ftell(f);
+  T->nfx = calloc(T->nstpx, sizeof(int));
+  T->nfmax = 0;
+  for (int a_ = 0; a_ < T->Nffa; a_++) {
+    if (rd(&n, 4, 1, f) || n != T->nstpx) goto bad;
+    for (int ix = 0; ix < T->nstpx; ix++) {
+      if (rd(&n, 4, 1, f) || n != T->nstpy) goto bad;
+      for (int iy = 0; iy < T->nstpy; iy++) {
+        int nx;
+        if (rd(&nx, 4, 1, f) || nx < 1 || nx > 4096) goto bad;
+        if (a_ == 0 && iy == 0) T->nfx[ix] = nx;
+        else if (T->nfx[ix] != nx) goto bad; /* one length per x node */
+        if (nx > T->nfmax) T->nfmax = nx;
+        if (fseek(f, 8L * nx, SEEK_CUR)) goto bad;
+      }
+    }
+  }
+  fseek(f, start, SEEK_SET);
+  size_t tot = (size_t)T->Nffa * T->nstpx * T->nstpy * T->nfmax;
+  T->Ffm = calloc(tot, sizeof(float));
+  T->Ffp = calloc(tot, sizeof(float));
+  uint32_t ma = 0, mb = 0, pa = 0, pb = 0, rank = 0;
+  for (int a_ = 0; a_ < T->Nffa; a_++) {
+    rd(&n, 4, 1, f);
+    for (int ix = 0; ix < T->nstpx; ix++) {
+      rd(&n, 4, 1, f);
+      for (int iy = 0; iy < T->nstpy; iy++) {
+        int nx;
+        rd(&nx, 4, 1, f);
+        size_t o = (((size_t)a_ * T->nstpx + ix) * T->nstpy + iy) * T->nfmax;
+        if (rd(T->Ffm + o, 4, nx, f)) goto bad;
+        oq_checksum_accum(&ma, &mb, rank, T->Ffm + o, 4 * (size_t)nx);
+        if (rd(T->Ffp + o, 4, nx, f)) goto bad;
+        oq_checksum_accum(&pa, &pb, rank, T->Ffp + o, 4 * (size_t)nx);
+        rank++;
+      }
+    }
+  }
+  uint32_t ck4[4];
+  if (rd(ck4, 4, 4, f)) goto bad;
+  if (ck4[0] != ma || ck4[1] != mb || ck4[2] != pa || ck4[3] != pb) goto bad;
+  fclose(f);
+  return 0;
+bad:
+  fprintf(stderr, "oracle: bad form-factor file %s\n", path);
+  fclose(f);
+  return 1;
+}
+
+static int load_tay(const char *dir, const char *name, int nrow, int ncol, double *dst)
+{
+  char path[4096];
+  snprintf(path, sizeof path, "%s/%s", dir, name);
+  FILE *f = fopen(path, "rb");
+  if (!f) { fprintf(stderr, "oracle: cannot open %s\n", path); return 1; }
+  int magic, n;
+  uint32_t a = 0, b = 0, ck[2];
+  if (rd(&magic, 4, 1, f) || magic != MAGIC) goto bad;
+  if (rd(&n, 4, 1, f) || n != nrow) goto bad;
+  for (int i = 0; i < nrow; i++) {
+    if (rd(&n, 4, 1, f) || n != ncol) goto bad;
+    if (rd(dst + (size_t)i * ncol, 8, ncol, f)) goto bad;
+    oq_checksum_accum(&a, &b, (uint32_t)i, dst + (size_t)i * ncol, 8 * (size_t)ncol);
+  }
+  if (rd(ck, 4, 2, f) || ck[0] != a || ck[1] != b) goto bad;
+  fclose(f);
+  return 0;
+bad:
+  fprintf(stderr, "oracle: bad Taylor file %s\n", path);
+  fclose(f);
+  return 1;
+}
+
+int oq_load(const char *dir, oq_tables *T)
+{
+  memset(T, 0, sizeof *T);
+  if (load_ff(dir, T)) return 1;
+  if (load_tay(dir, "taylorx_cksum.bin", 23, 810, &T->TX[0][0])) return 1;
+  if (load_tay(dir, "taylory_cksum.bin", 14, 100, &T->TY[0][0])) return 1;
+  T->NY_tay = 810;
+  T->NX_tay = 100;
+  /* ratio LUTs (simd.c:716-784); sized past the longest Clenshaw so no read runs off the end */
+  T->nLUT = T->nfmax + 4;
+  for (int l = 1; l <= 6; l++) T->LUT[l] = calloc(T->nLUT, sizeof(double));
+  for (int i = 1; i < T->nLUT; i++) {
+    T->LUT[1][i] = (i + 1) / (double)i;
+    T->LUT[2][i] = (i + 3) / (double)(i + 1);
+  }
+  for (int i = 2; i < T->nLUT; i++) {
+    T->LUT[3][i] = (i + 1) / (double)(i - 1);
+    T->LUT[4][i] = (i + 4) / (double)i;
+  }
+  for (int i = 3; i < T->nLUT; i++) {
+    T->LUT[5][i] = (i + 1) / (double)(i - 2);
+    T->LUT[6][i] = (i + 5) / (double)(i - 1);
+  }
+  return 0;
+}
+
+void oq_free(oq_tables *T)
+{
+  free(T->XX); free(T->YY); free(T->nfx); free(T->Ffm); free(T->Ffp);
+  for (int l = 1; l <= 6; l++) free(T->LUT[l]);
+  memset(T, 0, sizeof *T);
+}
+
+/* ------------------------------------------------------------------ enums (KQED.h:45-60) */
+enum { QG0, QG1, QG2, QG3, QL4, QL2, dxQG0, dxQG1, dxQG2, dxQG3, dxQL4, dxQL2, d2xQG2, d2xQG3 };
+enum { YYr = 0, G0dy, G0dx, Gl2, Gl2dy, Gl21, Gl21dy, Gl3, Gl3dy, G21, G21dy, G22A, G22B, G22Ady,
+       G22Bdy, G3A, G3B, G3Ady, G3Bdy, G31A, G31B, G31Ady, G31Bdy };
+enum { alpha0dx_0p = 0, alpha0_1p, alpha3_0p, beta2_1p, alpha3_1p, alpha1_0p, beta4_1p, alpha1_1p,
+       alpha1dx_1p, alpha3dx_0p, alpha3dxdx_0p, beta2dx_1p, alpha3dx_1p, alpha1dx_0p };
+
+static inline const float *FFm(const oq_tables *T, int nm, int ix, int iy)
+{ return T->Ffm + (((size_t)nm * T->nstpx + ix) * T->nstpy + iy) * T->nfmax; }
+static inline const float *FFp(const oq_tables *T, int nm, int ix, int iy)
+{ return T->Ffp + (((size_t)nm * T->nstpx + ix) * T->nstpy + iy) * T->nfmax; }
+
+static inline double lerp(double a, double t1, double t2) { return a * t1 + (1.0 - a) * t2; }
+static inline double scal(const double a[4], const double b[4])
+{ return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+
+/* power-case table of getff2 (getff-new.c:116-123): yp selector and mshm */
+static void ffcase(int nm, int *ypk, int *mshm)
+{
+  switch (nm) {
+  case QG0: case dxQG0: case QG1: case dxQG1: case QG3: case dxQG3: case d2xQG3:
+    *ypk = 0; *mshm = 2; break;               /* yp = 1 */
+  case QG2: case dxQG2: case d2xQG2: *ypk = 1; *mshm = 3; break; /* yp = 1/y */
+  case QL4: case dxQL4: *ypk = 2; *mshm = 1; break;              /* yp = y */
+  default: *ypk = 3; *mshm = 4; break;                            /* QL2: yp = 1/y^2 */
+  }
+}
+
+/* ------------------------------------------------------------------ scalar Clenshaw (cheby.c) */
+double oq_chebU(int order, int nk, double x, const double *co)
+{
+  double ya = 0.0, yb = 0.0, ytmp;
+  const double twox = 2.0 * x;
+  int j;
+  switch (order) {
+  case 0: /* cheby.c:13-28 */
+    for (j = nk - 1; j > 0; j--) { ytmp = yb; yb = twox * yb - ya + co[j]; ya = ytmp; }
+    FL(3.0 * (nk - 1) + 3);
+    return -ya + twox * yb + co[0];
+  case 1: /* cheby.c:34-49 */
+    for (j = nk - 1; j > 0; j--) {
+      ytmp = yb; yb = twox * (j + 1) * yb / j - (j + 3) * ya / (j + 1) + co[j]; ya = ytmp;
+    }
+    FL(7.0 * (nk - 1) + 1);
+    return 2.0 * yb;
+  case 2: /* cheby.c:54-70 */
+    for (j = nk - 1; j > 1; j--) {
+      ytmp = yb; yb = twox * (j + 1) * yb / (j - 1); yb -= (j + 4) * ya / j; yb += co[j]; ya = ytmp;
+    }
+    FL(7.0 * (nk - 2) + 1);
+    return 8 * yb;
+  default: /* cheby.c:76-91 */
+    for (j = nk - 1; j > 2; j--) {
+      ytmp = yb; yb = twox * (j + 1) * yb / (j - 2) - (j + 5) * ya / (j - 1) + co[j]; ya = ytmp;
+    }
+    FL(7.0 * (nk - 3) + 1);
+    return 48 * yb;
+  }
+}
+
+/* FMA-form Clenshaw of simd.c:79-118 / 182-336 for one lane of one array. */
+double oq_chebU_avx(int order, int nk, double x, const double *co, const oq_tables *T)
+{
+  const double twox = 2. * x;
+  double ya = 0., yb = 0., ytmp;
+  int j = nk - 1;
+  if (order == 0) {
+    for (int i = 0; i < nk; i++, j--) {
+      ytmp = yb; ya = ya - co[j]; yb = fma(twox, yb, -ya); ya = ytmp;
+    }
+    FL(3.0 * nk);
+    return yb;
+  }
+  const double *PA = T->LUT[2 * order - 1], *PB = T->LUT[2 * order];
+  for (int i = 0; i < nk - order; i++, j--) {
+    ytmp = yb;
+    ya = fma(PB[j], ya, -co[j]);
+    yb = yb * twox;
+    yb = fma(PA[j], yb, -ya);
+    ya = ytmp;
+  }
+  FL(5.0 * (nk - order) + 1);
+  static const double fin[4] = { 1., 2., 8., 48. };
+  return yb * fin[order];
+}
+
+/* ------------------------------------------------------------------ search + Hermite weights */
+int oq_bsrch(const double *arr, double target, int lo, int hi)
+{
+  /* getff-new.c:199-210, iterative form of the same bisection */
+  while (hi - lo >= 2) {
+    const int mid = (hi + lo) / 2;
+    if (arr[mid] > target) hi = mid; else lo = mid;
+  }
+  return lo;
+}
+
+typedef struct { double A, B, C1, C2, D, lA; double y1, y2; int idx; } oq_ip;
+typedef struct {
+  double x, cb, y, xsq, ysq, xdoty, cborig, yorig, xmysq, xmy;
+  bool flag2;
+  oq_ip INVx, INVy;
+} oq_inv;
+
+/* precompute_INVx / precompute_INV (getff-new.c:30-56, simd.c:10-37) */
+static void precomp(oq_ip *P, double y, double y1, double y2, int idx)
+{
+  P->idx = idx;
+  const double dy = y1 - y2, ymy1 = y - y1, ymy2 = y - y2;
+  const double ym2sq = ymy2 * ymy2, ym1sq = ymy1 * ymy1;
+  P->A = -(ym2sq) * (2 * y - 3 * y1 + y2);
+  P->B = (ym1sq) * (2 * y + y1 - 3 * y2);
+  P->C1 = (ymy1) * (ym2sq) * dy;
+  P->C2 = (ym1sq) * (ymy2) * dy;
+  P->D = 1. / (dy * dy * dy);
+  P->lA = (y2 - y) / (y2 - y1);
+  P->y1 = y1; P->y2 = y2;
+  FL(30);
+}
+
+/* set_invariants (getff-new.c:316-370) */
+static oq_inv set_inv(const oq_tables *T, const double xv[4], const double yv[4])
+{
+  oq_inv I;
+  const double EPSIN = 1E-7;
+  I.xsq = scal(xv, xv); I.xsq = I.xsq < EPSIN ? EPSIN : I.xsq; I.x = sqrt(fabs(I.xsq));
+  I.ysq = scal(yv, yv); I.ysq = I.ysq < EPSIN ? EPSIN : I.ysq; I.y = sqrt(fabs(I.ysq));
+  I.xdoty = scal(xv, yv);
+  I.cb = I.xdoty / (I.x * I.y);
+  I.cborig = I.cb; I.yorig = I.y;
+  I.xmysq = (I.xsq + I.ysq) * (1.00000000000001) - 2.0 * I.xdoty;
+  I.xmy = sqrt(fabs(I.xmysq));
+  I.flag2 = false;
+  const double rx = (I.x > I.xmy ? I.xmy / I.x : I.x / I.xmy);
+  const double rxy = (I.x > I.y ? I.y / I.x : I.x / I.y);
+  if (rx < rxy && I.xmy < T->YY[T->nstpy - 1] && fabs(I.xmysq) > 1E-28) {
+    I.cb = (I.x - I.y * I.cb) / I.xmy;
+    I.y = I.xmy;
+    I.flag2 = true;
+  }
+  const int ix1 = oq_bsrch(T->XX, I.x, 0, T->nstpx);
+  int ix2 = ix1 + 1; if (ix2 >= T->nstpx) ix2 = ix1;
+  precomp(&I.INVx, I.x, T->XX[ix1], T->XX[ix2], ix1);
+  const int iy1 = oq_bsrch(T->YY, I.y, 0, T->nstpy);
+  int iy2 = iy1 + 1; if (iy2 >= T->nstpy) iy2 = iy1;
+  precomp(&I.INVy, I.y, T->YY[iy1], T->YY[iy2], iy1);
+  FL(40);
+  return I;
+}
+
+/* ------------------------------------------------------------------ scalar interpolation path */
+/* getff2 (getff-new.c:104-156) */
+static void getff2(double res[2], int nf, int nm, bool ndy, int ndcb, double y, double x,
+                   const float *fm, const float *fp)
+{
+  const double y1 = 1.0 / y;
+  int ypk, mshm;
+  ffcase(nm, &ypk, &mshm);
+  const double yp = ypk == 0 ? 1.0 : ypk == 1 ? y1 : ypk == 2 ? y : y1 * y1;
+  int mshp = 2 - mshm;
+  const int mm = abs(mshp);
+  double fval[4200], fvalD[4200];
+  memset(fval, 0, (nf + mm) * sizeof(double));
+  memset(fvalD, 0, (nf + mm) * sizeof(double));
+  double facm = y1 * y1 * yp, facp = yp;
+  for (int j = 0; j < mm; j++) { facm *= y1; facp *= y; }
+  mshm += mm; mshp += mm;
+  for (int j = 0; j < nf; j++) {
+    fvalD[mm + j] = y1 * mshp * facp * fp[j] - y1 * mshm * facm * fm[j];
+    fval[mm + j] = facm * fm[j] + facp * fp[j];
+    facm *= y1; facp *= y; mshm++; mshp++;
+  }
+  FL(12.0 * nf);
+  res[1] = oq_chebU(mm + ndcb, nf + mm, x, fvalD);
+  res[0] = ndy ? res[1] : oq_chebU(mm + ndcb, nf + mm, x, fval);
+}
+
+static inline double interpol3(double y, double y1, double y2, double dy, double f1, double f2,
+                               double g1, double g2)
+{
+  FL(30);
+  return ((-f1 * (y - y2) * (y - y2) * (2 * y - 3 * y1 + y2) +
+           (y - y1) * (f2 * (y - y1) * (2 * y + y1 - 3 * y2)
+                       + (y - y2) * dy * (g1 * y + g2 * y - g2 * y1 - g1 * y2))) / (dy * dy * dy));
+}
+
+/* accessv (getff-new.c:161-194) */
+static double accessv(const oq_tables *T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm,
+                      bool ndy, int ndcb, double cb, double y)
+{
+  const int nx = T->nfx[ix];
+  const double y1 = T->YY[iy];
+  double res1[2] = { 0., 0. };
+  getff2(res1, nx, nm, ndy, ndcb, y1, cb, FFm(T, nm, ix, iy), FFp(T, nm, ix, iy));
+  if (!flag_hy) {
+    const int iy2 = iy + 1;
+    const double y2 = T->YY[iy2];
+    double res2[2] = { 0., 0. };
+    getff2(res2, nx, nm, ndy, ndcb, y2, cb, FFm(T, nm, ix, iy2), FFp(T, nm, ix, iy2));
+    if (use_y_derivs) return interpol3(y, y1, y2, y1 - y2, res1[0], res2[0], res1[1], res2[1]);
+    return lerp((y2 - y) / (y2 - y1), res1[0], res2[0]);
+  }
+  return res1[0];
+}
+
+static inline double interpol4(const oq_ip *P, double f1, double f2, double g1, double g2)
+{
+  FL(8);
+  return (f1 * P->A + f2 * P->B + g1 * P->C1 + g2 * P->C2) * P->D;
+}
+
+/* ------------------------------------------------------------------ AVX interpolation path */
+/* Per-pair stencil: the reference packs 14 tables x 4 corners into per-thread scratch
+ * (QED_kernel.c:447-502).  Lanes: (ix1,iy1), (ix1,iy2), (ix2,iy1), (ix2,iy2). */
+typedef struct {
+  int nmax[14];
+  double *Fm[14], *Fp[14];     /* [nmax][4] */
+  double buf[2 * 14 * 4 * 4096];
+} oq_pc;
+
+static void pack_pc(const oq_tables *T, const oq_inv *I, oq_pc *P)
+{
+  const int ix1 = I->INVx.idx;
+  int ix2 = ix1 + 1; if (ix2 >= T->nstpx) ix2 = ix1;
+  const int iy1 = I->INVy.idx;
+  int iy2 = iy1 + 1; if (iy2 >= T->nstpy) iy2 = iy1;
+  const int nx1 = T->nfx[ix1], nx2 = T->nfx[ix2];
+  const int nmax = nx1 < nx2 ? nx2 : nx1;
+  double *b = P->buf;
+  for (int i = 0; i < 14; i++) {
+    P->nmax[i] = nmax;
+    P->Fm[i] = b; b += 4 * (size_t)nmax;
+    P->Fp[i] = b; b += 4 * (size_t)nmax;
+    const float *m[4] = { FFm(T, i, ix1, iy1), FFm(T, i, ix1, iy2), FFm(T, i, ix2, iy1), FFm(T, i, ix2, iy2) };
+    const float *p[4] = { FFp(T, i, ix1, iy1), FFp(T, i, ix1, iy2), FFp(T, i, ix2, iy1), FFp(T, i, ix2, iy2) };
+    for (int j = 0; j < nmax; j++) {
+      for (int l = 0; l < 4; l++) {
+        const int len = l < 2 ? nx1 : nx2;
+        P->Fm[i][4 * j + l] = j < len ? (double)m[l][j] : 0.0;
+        P->Fp[i][4 * j + l] = j < len ? (double)p[l][j] : 0.0;
+      }
+    }
+  }
+}
+
+/* Clenshaw of one order over two 4-lane arrays (Func_usm5, simd.c:339-340) */
+static void usm5(const oq_tables *T, int order, int nk, double x, const double *co1, const double *co2,
+                 double out[8])
+{
+  double c1[4200], c2[4200];
+  for (int l = 0; l < 4; l++) {
+    for (int j = 0; j < nk; j++) { c1[j] = co1[4 * j + l]; c2[j] = co2[4 * j + l]; }
+    out[l] = oq_chebU_avx(order, nk, x, c1, T);
+    out[4 + l] = oq_chebU_avx(order, nk, x, c2, T);
+  }
+}
+
+/* getff6 / getff7 array build (simd.c:343-485 and 487-613, FMA variants) */
+static int build_arrays(const oq_tables *T, const oq_pc *PC, int nm, int ix, const double y12[4],
+                        bool calc, double *fval, double *fvald, double *fvaldx, double *fvaldxd,
+                        int *mm_out, int *nk_out)
+{
+  const int nf_max = T->nfx[ix] > T->nfx[ix + 1] ? T->nfx[ix] : T->nfx[ix + 1];
+  int ypk, mshm;
+  ffcase(nm, &ypk, &mshm);
+  const int offset = nm < dxQG0 ? dxQG0 : QL4;
+  const int dm = offset + nm > d2xQG3 ? 0 : offset + nm;
+  int mshp = 2 - mshm;
+  const int mm = abs(mshp);
+  double y1[4], yp[4], facm[4], facp[4], MSHM, MSHP;
+  for (int l = 0; l < 4; l++) {
+    y1[l] = 1.0 / y12[l];
+    yp[l] = ypk == 0 ? 1.0 : ypk == 1 ? y1[l] : ypk == 2 ? y12[l] : y1[l] * y1[l];
+    facm[l] = y1[l] * yp[l];
+    facp[l] = yp[l];
+    facm[l] = facm[l] * y1[l];
+    for (int j = 0; j < mm; j++) { facm[l] = facm[l] * y1[l]; facp[l] = facp[l] * y12[l]; }
+  }
+  mshm += mm; mshp += mm;
+  MSHM = mshm; MSHP = mshp;
+  const double *pFm1 = PC->Fm[nm], *pFm2 = PC->Fm[dm], *pFp1 = PC->Fp[nm], *pFp2 = PC->Fp[dm];
+  if (!calc) for (int l = 0; l < 4; l++) { facm[l] = facm[l] * y1[l]; facp[l] = facp[l] * y1[l]; }
+  for (int j = 0; j < nf_max; j++) {
+    for (int l = 0; l < 4; l++) {
+      const int o = 4 * (mm + j) + l;
+      const double vm = facm[l] * pFm1[4 * j + l], vp = facp[l] * pFp1[4 * j + l];
+      const double vmdx = facm[l] * pFm2[4 * j + l], vpdx = facp[l] * pFp2[4 * j + l];
+      if (calc) {
+        double A = MSHM * vm, B = MSHM * vmdx;
+        fval[o] = vm + vp;
+        fvaldx[o] = vmdx + vpdx;
+        A = fma(MSHP, vp, -A);
+        B = fma(MSHP, vpdx, -B);
+        fvald[o] = y1[l] * A;
+        fvaldxd[o] = y1[l] * B;
+      } else {
+        const double A = MSHM * vm, B = MSHM * vmdx;
+        fvald[o] = fma(MSHP, vp, -A);
+        fvaldxd[o] = fma(MSHP, vpdx, -B);
+      }
+      facm[l] = facm[l] * y1[l];
+      facp[l] = facp[l] * y12[l];
+    }
+    MSHM = MSHM + 1.0;
+    MSHP = MSHP + 1.0;
+  }
+  FL(4.0 * nf_max * 14);
+  for (int k = 0; k < 4 * mm; k++) { fval[k] = fvald[k] = fvaldx[k] = fvaldxd[k] = 0.0; }
+  *mm_out = mm;
+  *nk_out = nf_max + mm;
+  return dm;
+}
+
+/* INBLOCK (simd.c:54-63): y-Hermite of lane pairs (0,1),(2,3) of A (values) and B (y-derivs) */
+static void inblock(const oq_ip *P, const double A[8], const double B[8], double F[4])
+{
+  for (int k = 0; k < 4; k++) {
+    double t = A[2 * k] * P->A;
+    t = fma(A[2 * k + 1], P->B, t);
+    t = fma(B[2 * k], P->C1, t);
+    t = fma(B[2 * k + 1], P->C2, t);
+    F[k] = t * P->D;
+  }
+  FL(28);
+}
+static void lerpblock(const oq_ip *P, const double B[8], double F[4])
+{
+  for (int k = 0; k < 4; k++) F[k] = fma(B[2 * k] - B[2 * k + 1], P->lA, B[2 * k + 1]);
+  FL(12);
+}
+
+/* accessv6 (simd.c:616-650) */
+static void accessv6(const oq_tables *T, const oq_pc *PC, int ix, int nm, bool ndy, int ndcb,
+                     double cb, const oq_ip *INVy, double f[4])
+{
+  static __thread double fv[4 * 4200], fd[4 * 4200], fx[4 * 4200], fxd[4 * 4200];
+  const double y12[4] = { INVy->y1, INVy->y2, INVy->y1, INVy->y2 };
+  int mm, nk;
+  build_arrays(T, PC, nm, ix, y12, !ndy, fv, fd, fx, fxd, &mm, &nk);
+  double f12[8] = { 0 }, g12[8] = { 0 };
+  if (!ndy) usm5(T, mm + ndcb, nk, cb, fv, fx, f12);
+  usm5(T, mm + ndcb, nk, cb, fd, fxd, g12);
+  if (ndy) lerpblock(INVy, g12, f); else inblock(INVy, f12, g12, f);
+}
+
+/* accessv7 (simd.c:653-687) */
+static void accessv7(const oq_tables *T, const oq_pc *PC, int ix, int nm, int ndcb, double cb,
+                     const oq_ip *INVy, double f[12])
+{
+  static __thread double fv[4 * 4200], fd[4 * 4200], fx[4 * 4200], fxd[4 * 4200];
+  const double y12[4] = { INVy->y1, INVy->y2, INVy->y1, INVy->y2 };
+  int mm, nk;
+  build_arrays(T, PC, nm, ix, y12, true, fv, fd, fx, fxd, &mm, &nk);
+  double f12[8] = { 0 }, g12[8] = { 0 }, f212[8] = { 0 }, g212[8] = { 0 };
+  usm5(T, mm + ndcb, nk, cb, fv, fx, f12);
+  usm5(T, mm + ndcb, nk, cb, fd, fxd, g12);
+  if (mm + ndcb + 1 < 4) {
+    usm5(T, mm + ndcb + 1, nk, cb, fv, fx, f212);
+    usm5(T, mm + ndcb + 1, nk, cb, fd, fxd, g212);
+  }
+  inblock(INVy, f12, g12, f);
+  lerpblock(INVy, g12, f + 4);
+  inblock(INVy, f212, g212, f + 8);
+}
+
+/* interpol5 (getff-new.c:69-90) */
+static void interpol5(double F[4], const oq_ip *P, const double f[12])
+{
+  const double Y0[4] = { f[8], f[3], f[4], f[0] };
+  const double Y1[4] = { f[9], f[2], f[5], f[1] };
+  const double Y2[4] = { f[10], f[3], f[6], f[2] };
+  const double Y3[4] = { f[11], 0.0, f[7], f[3] };
+  const double a[4] = { P->A, 1.0 / P->D, P->A, P->A };
+  const double b[4] = { P->B, P->lA / P->D, P->B, P->B };
+  const double c1[4] = { P->C1, -P->lA / P->D, P->C1, P->C1 };
+  const double c2[4] = { P->C2, 0.0, P->C2, P->C2 };
+  for (int k = 0; k < 4; k++) {
+    double t = Y0[k] * a[k];
+    t = fma(Y1[k], b[k], t);
+    t = fma(Y2[k], c1[k], t);
+    t = fma(Y3[k], c2[k], t);
+    F[k] = t * P->D;
+  }
+  FL(36);
+}
+
+/* extractff (getff-new.c:214-258), AVX build */
+static double extractff(const oq_tables *T, const oq_pc *PC, int nm, bool ndy, int ndcb, const oq_inv *I)
+{
+  const bool flag_hx = (I->x >= T->XX[T->nstpx - 1]);
+  const bool flag_hy = (I->y >= T->YY[T->nstpy - 1]);
+  const bool use_x_derivs = (nm < dxQG0 || nm == dxQG2 || nm == dxQG3);
+  if (!flag_hx && !flag_hy) {
+    double f[4];
+    accessv6(T, PC, I->INVx.idx, nm, ndy, ndcb, I->cb, &I->INVy, f);
+    if (use_x_derivs) return interpol4(&I->INVx, f[0], f[1], f[2], f[3]);
+    FL(3);
+    return lerp(I->INVx.lA, f[0], f[1]);
+  }
+  const int ix = I->INVx.idx, iy = I->INVy.idx;
+  const bool use_y_derivs = (ndy == false);
+  const double f1iy = accessv(T, flag_hy, use_y_derivs, ix, iy, nm, ndy, ndcb, I->cb, I->y);
+  if (!flag_hx) {
+    const double f2iy = accessv(T, flag_hy, use_y_derivs, ix + 1, iy, nm, ndy, ndcb, I->cb, I->y);
+    if (use_x_derivs) {
+      const int offset = nm < dxQG0 ? dxQG0 : QL4;
+      const double g1iy = accessv(T, flag_hy, use_y_derivs, ix, iy, nm + offset, ndy, ndcb, I->cb, I->y);
+      const double g2iy = accessv(T, flag_hy, use_y_derivs, ix + 1, iy, nm + offset, ndy, ndcb, I->cb, I->y);
+      return interpol4(&I->INVx, f1iy, f2iy, g1iy, g2iy);
+    }
+    return lerp(I->INVx.lA, f1iy, f2iy);
+  }
+  return f1iy;
+}
+
+/* extractff2 (getff-new.c:262-312), AVX build */
+static void extractff2(const oq_tables *T, const oq_pc *PC, int nm, int ndcb, const oq_inv *I, double F[4])
+{
+  const bool flag_hx = (I->x >= T->XX[T->nstpx - 1]);
+  const bool flag_hy = (I->y >= T->YY[T->nstpy - 1]);
+  if (!flag_hx && !flag_hy) {
+    double f[16] = { 0 };
+    accessv7(T, PC, I->INVx.idx, nm, ndcb, I->cb, &I->INVy, f);
+    interpol5(F, &I->INVx, f);
+    return;
+  }
+  const int ndcb2 = ndcb + 1 < 5 ? ndcb + 1 : ndcb;
+  static const int dxmap[14] = { dxQG0, dxQG1, dxQG2, dxQG3, dxQL4, dxQL2, dxQG0, dxQG1, d2xQG2, d2xQG3,
+                                 dxQL4, dxQL2, d2xQG2, d2xQG3 };
+  F[0] = extractff(T, PC, nm, false, ndcb2, I);
+  F[1] = extractff(T, PC, dxmap[nm], false, ndcb, I);
+  F[2] = extractff(T, PC, nm, true, ndcb, I);
+  F[3] = extractff(T, PC, nm, false, ndcb, I);
+}
+
+/* ------------------------------------------------------------------ chnr_dS (chnr_dS.c:10-95) */
+static int dg_taylorx(const oq_tables *T, double *dg0dy, double *dg0dx, double *dg0dcb, const oq_inv *I)
+{
+  const int iy_tay = oq_bsrch(T->TX[YYr], I->y, 0, T->NY_tay);
+  const int iy = oq_bsrch(T->YY, I->y, 0, T->nstpy);
+  if (iy_tay == T->NY_tay - 1 || iy == T->nstpy - 1) return 1;
+  const int iy2_tay = iy_tay + 1, ix = 0;
+  const double ay = (T->TX[YYr][iy2_tay] - I->y) / (T->TX[YYr][iy2_tay] - T->TX[YYr][iy_tay]);
+  const double ax = (T->XX[0] - I->x) / (T->XX[0]);
+  double f1 = I->cb * lerp(ay, T->TX[G0dx][iy_tay], T->TX[G0dx][iy2_tay]);
+  double f2 = accessv(T, false, true, ix, iy, dxQG0, false, 0, I->cb, I->y);
+  *dg0dx = lerp(ax, f1, f2);
+  f1 = lerp(ay, T->TX[G0dy][iy_tay], T->TX[G0dy][iy2_tay]);
+  f2 = accessv(T, false, false, ix, iy, QG0, true, 0, I->cb, I->y);
+  *dg0dy = lerp(ax, f1, f2);
+  f2 = accessv(T, false, true, ix, iy, QG0, false, 1, I->cb, I->y);
+  *dg0dcb = lerp(ax, 0.0, f2);
+  FL(30);
+  return 0;
+}
+
+static int chnr_dS(const oq_tables *T, const oq_pc *PC, const double xv[4], const double yv[4],
+                   const oq_inv *I, double dxv[4], double dyv[4], double *sc)
+{
+  double dg0dy, dg0dx, dg0dcb;
+  if (I->x < T->XX[0]) {
+    if (dg_taylorx(T, &dg0dy, &dg0dx, &dg0dcb, I)) return 1;
+  } else if (I->x > T->XX[T->nstpx - 1]) {
+    return 1;
+  } else {
+    double f[4];
+    extractff2(T, PC, QG0, 0, I, f);
+    if (sc) memcpy(sc, f, sizeof f);
+    dg0dcb = f[0]; dg0dx = f[1]; dg0dy = f[2];
+  }
+  if (I->flag2) {
+    const double dgs0dx = dg0dx, dgs0dcb = dg0dcb, dgs0dy = dg0dy;
+    const double ca = I->cb;
+    const double cd = (I->yorig - I->x * I->cborig) / I->xmy;
+    dg0dx = dgs0dx + (1.0 - ca * ca) / I->xmy * dgs0dcb + ca * dgs0dy;
+    dg0dcb = -(I->ysq * cd * dgs0dcb + I->x * I->yorig * I->xmy * dgs0dy) / I->xmysq;
+    dg0dy = -(I->cborig + ca * cd) / I->xmy * dgs0dcb + cd * dgs0dy;
+    FL(30);
+  }
+  for (int bet = 0; bet < 4; bet++) {
+    const double yhat = yv[bet] / I->yorig;
+    const double xhat = xv[bet] / I->x;
+    const double c2 = (xhat - I->cborig * yhat) / I->yorig;
+    const double c4 = (yhat - I->cborig * xhat) / I->x;
+    dyv[bet] = yhat * dg0dy + c2 * dg0dcb;
+    dxv[bet] = xhat * dg0dx + c4 * dg0dcb;
+  }
+  FL(48);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ chnr_dT (chnr_dT.c:10-293) */
+typedef struct {
+  double ell1, ell2, ell3, v1, ell4, dv1dx, dv1dy, dv1dcb, dell2dx, dell2dy, dell2dcb;
+  double dell1dx, dell1dy, dell1dcb, dell3dx, dell3dy, dell3dcb, dell4dy, dell4dx, dell4dcb;
+} ttmps;
+
+static int taylorx_T(const oq_tables *T_, ttmps *T, const oq_inv *I)
+{
+  const double ysq = I->flag2 ? I->xmysq : I->ysq;
+  const int iy_tay = oq_bsrch(T_->TX[YYr], I->y, 0, T_->NY_tay);
+  const int iy = oq_bsrch(T_->YY, I->y, 0, T_->nstpy);
+  if (iy_tay == T_->NY_tay - 1 || iy == T_->nstpy - 1) return 1;
+  const int iy2_tay = iy_tay + 1, ix = 0;
+  const double (*TX)[810] = T_->TX;
+  const double ay = (TX[YYr][iy2_tay] - I->y) / (TX[YYr][iy2_tay] - TX[YYr][iy_tay]);
+  const double ax = (T_->XX[0] - I->x) / (T_->XX[0]);
+  const double xb = T_->XX[0], xbsq = xb * xb;
+  const double cb = I->cb, y = I->y;
+  const double ell2a = lerp(ay, TX[Gl2][iy_tay], TX[Gl2][iy2_tay]);
+  const double ell2b = accessv(T_, false, true, ix, iy, QL2, false, 0, cb, y);
+  T->ell2 = lerp(ax, ell2a, ell2b);
+  const double dell2adx = lerp(ay, TX[Gl21][iy_tay], TX[Gl21][iy2_tay]) * cb;
+  const double dell2bdx = accessv(T_, false, true, ix, iy, dxQL2, false, 0, cb, y);
+  T->dell2dx = lerp(ax, dell2adx, dell2bdx);
+  const double dell2ady = lerp(ay, TX[Gl2dy][iy_tay], TX[Gl2dy][iy2_tay]);
+  const double dell2bdy = accessv(T_, false, false, ix, iy, QL2, true, 0, cb, y);
+  T->dell2dy = lerp(ax, dell2ady, dell2bdy);
+  const double dell2adcb = 0.0;
+  const double dell2bdcb = accessv(T_, false, true, ix, iy, QL2, false, 1, cb, y);
+  T->dell2dcb = lerp(ax, dell2adcb, dell2bdcb);
+  const double v1b = accessv(T_, false, true, ix, iy, QG1, false, 0, cb, y);
+  const double dv1bdx = accessv(T_, false, true, ix, iy, dxQG1, false, 0, cb, y);
+  const double dv1bdy = accessv(T_, false, false, ix, iy, QG1, true, 0, cb, y);
+  const double dv1bdcb = accessv(T_, false, true, ix, iy, QG1, false, 1, cb, y);
+  const double ell4b = accessv(T_, false, true, ix, iy, QL4, false, 0, cb, y);
+  const double dell4bdx = accessv(T_, false, true, ix, iy, dxQL4, false, 0, cb, y);
+  const double dell4bdy = accessv(T_, false, false, ix, iy, QL4, true, 0, cb, y);
+  const double dell4bdcb = accessv(T_, false, true, ix, iy, QL4, false, 1, cb, y);
+  const double ell3a = lerp(ay, TX[Gl3][iy_tay], TX[Gl3][iy2_tay]);
+  const double ell3b = ell4b / (2.0 * xbsq * ysq) - y * cb * ell2b / xb;
+  T->ell3 = lerp(ax, ell3a, ell3b);
+  const double ell1b = 4.0 / (3.0 * xbsq * xbsq) * (v1b - xbsq * ysq * (cb * cb - 0.25) * ell2b
+                                                   - 1.5 * xbsq * xb * y * cb * ell3b);
+  T->ell1 = ell1b;
+  const double dell3ady = lerp(ay, TX[Gl3dy][iy_tay], TX[Gl3dy][iy2_tay]);
+  const double dell3bdy = (-ell4b / (ysq * y) + dell4bdy / (2.0 * ysq) - cb * xb * ell2b
+                           - y * xb * cb * dell2bdy) / (xbsq);
+  T->dell3dy = lerp(ax, dell3ady, dell3bdy);
+  const double dell3adcb = 0.0;
+  const double dell3bdcb = (dell4bdcb / (2.0 * xb * ysq) - y * ell2b - y * cb * dell2bdcb) / xb;
+  T->dell3dcb = lerp(ax, dell3adcb, dell3bdcb);
+  const double dell3bdx = (-ell4b / (xb * ysq) + dell4bdx / (2.0 * ysq) + y * cb * ell2b
+                           - y * xb * cb * dell2bdx) / xbsq;
+  T->dell3dx = dell3bdx;
+  T->dell1dy = 4.0 / (3.0 * xbsq * xbsq) * (dv1bdy - 2.0 * xbsq * y * (cb * cb - 0.25) * ell2b
+                                            - xbsq * ysq * (cb * cb - 0.25) * dell2bdy
+                                            - 1.5 * xbsq * xb * cb * ell3b
+                                            - 1.5 * xbsq * xb * y * cb * dell3bdy);
+  T->dell1dcb = 4.0 / (3.0 * xbsq * xbsq)
+    * (dv1bdcb - 2.0 * xbsq * ysq * cb * ell2b - xbsq * ysq * (cb * cb - 0.25) * dell2bdcb
+       - 1.5 * xbsq * xb * y * ell3b - 1.5 * xbsq * xb * y * cb * dell3bdcb);
+  T->dell1dx = -4.0 * ell1b / xb + 4.0 / (3.0 * xbsq * xbsq)
+    * (dv1bdx - 2.0 * xb * ysq * (cb * cb - 0.25) * ell2b - xbsq * ysq * (cb * cb - 0.25) * dell2bdx
+       - 4.5 * xbsq * y * cb * ell3b - 1.5 * xbsq * xb * y * cb * dell3bdx);
+  FL(200);
+  return 0;
+}
+
+static void xmyswap_T(ttmps *T, const oq_inv *I)
+{
+  const double ell2s = T->ell2, dells2dx = T->dell2dx, dells2dcb = T->dell2dcb, dells2dy = T->dell2dy;
+  const double cd = (I->yorig - I->x * I->cborig) / I->xmy;
+  const double cb = I->cb;
+  T->ell2 = ell2s;
+  T->dell2dx = dells2dx + (1.0 - cb * cb) / I->xmy * dells2dcb + cb * dells2dy;
+  T->dell2dcb = -(I->ysq * cd * dells2dcb + I->x * I->yorig * I->xmy * dells2dy) / I->xmysq;
+  T->dell2dy = -(I->cborig + cb * cd) / I->xmy * dells2dcb + cd * dells2dy;
+  const double ell3s = -T->ell3, dells3dx = -T->dell3dx, dells3dcb = -T->dell3dcb, dells3dy = -T->dell3dy;
+  T->ell3 = ell3s - T->ell2;
+  T->dell3dx = dells3dx + (1.0 - cb * cb) / I->xmy * dells3dcb + cb * dells3dy - T->dell2dx;
+  T->dell3dcb = -(I->ysq * cd * dells3dcb + I->x * I->yorig * I->xmy * dells3dy) / I->xmysq - T->dell2dcb;
+  T->dell3dy = -(I->cborig + cb * cd) / I->xmy * dells3dcb + cd * dells3dy - T->dell2dy;
+  const double ell1s = T->ell1, dells1dx = T->dell1dx, dells1dcb = T->dell1dcb, dells1dy = T->dell1dy;
+  T->ell1 = ell1s - T->ell2 - 2.0 * T->ell3;
+  T->dell1dx = dells1dx + (1.0 - cb * cb) / I->xmy * dells1dcb + cb * dells1dy - T->dell2dx - 2.0 * T->dell3dx;
+  T->dell1dcb = -(I->ysq * cd * dells1dcb + I->x * I->yorig * I->xmy * dells1dy) / I->xmysq
+    - T->dell2dcb - 2.0 * T->dell3dcb;
+  T->dell1dy = -(I->cborig + cb * cd) / I->xmy * dells1dcb + cd * dells1dy - T->dell2dy - 2.0 * T->dell3dy;
+  FL(110);
+}
+
+static int chnr_dT(const oq_tables *T_, const oq_pc *PC, const double xv[4], const double yv[4],
+                   const oq_inv *I, double dxv[64], double dyv[64], double *sc)
+{
+  ttmps T;
+  memset(&T, 0, sizeof T);
+  const double ysq = I->flag2 ? I->xmysq : I->ysq;
+  if (I->x < T_->XX[0]) {
+    if (taylorx_T(T_, &T, I)) return 1;
+  } else if (I->x > T_->XX[T_->nstpx - 1]) {
+    return 1;
+  } else {
+    double f[4];
+    const double x = I->x, y = I->y, cb = I->cb, xsq = I->xsq;
+    extractff2(T_, PC, QL2, 0, I, f);
+    if (sc) memcpy(sc, f, sizeof f);
+    T.dell2dx = f[1]; T.dell2dy = f[2]; T.ell2 = f[3]; T.dell2dcb = f[0];
+    extractff2(T_, PC, QL4, 0, I, f);
+    if (sc) memcpy(sc + 4, f, sizeof f);
+    T.dell4dx = f[1]; T.dell4dy = f[2]; T.ell4 = f[3]; T.dell4dcb = f[0];
+    extractff2(T_, PC, QG1, 0, I, f);
+    if (sc) memcpy(sc + 8, f, sizeof f);
+    T.dv1dx = f[1]; T.dv1dy = f[2]; T.v1 = f[3]; T.dv1dcb = f[0];
+    T.ell3 = T.ell4 / (2.0 * xsq * ysq) - y * cb * T.ell2 / x;
+    T.ell1 = 4.0 / (3.0 * xsq * xsq) * (T.v1 - xsq * ysq * (cb * cb - 0.25) * T.ell2
+                                        - 1.5 * xsq * x * y * cb * T.ell3);
+    T.dell3dy = (-T.ell4 / (ysq * y) + T.dell4dy / (2.0 * ysq) - cb * x * T.ell2 - y * x * cb * T.dell2dy) / (xsq);
+    T.dell3dcb = (T.dell4dcb / (2.0 * x * ysq) - y * T.ell2 - y * cb * T.dell2dcb) / x;
+    T.dell3dx = (-T.ell4 / (x * ysq) + T.dell4dx / (2.0 * ysq) + y * cb * T.ell2 - y * x * cb * T.dell2dx) / xsq;
+    T.dell1dy = 4.0 / (3.0 * xsq * xsq) * (T.dv1dy - 2.0 * xsq * y * (cb * cb - 0.25) * T.ell2
+                                           - xsq * ysq * (cb * cb - 0.25) * T.dell2dy
+                                           - 1.5 * xsq * x * cb * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dy);
+    T.dell1dcb = 4.0 / (3.0 * xsq * xsq) * (T.dv1dcb - 2.0 * xsq * ysq * cb * T.ell2
+                                            - xsq * ysq * (cb * cb - 0.25) * T.dell2dcb
+                                            - 1.5 * xsq * x * y * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dcb);
+    T.dell1dx = -4.0 * T.ell1 / x
+      + 4.0 / (3.0 * xsq * xsq) * (T.dv1dx - 2.0 * x * ysq * (cb * cb - 0.25) * T.ell2
+                                   - xsq * ysq * (cb * cb - 0.25) * T.dell2dx
+                                   - 4.5 * xsq * y * cb * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dx);
+    FL(170);
+  }
+  if (I->flag2) xmyswap_T(&T, I);
+  for (int bet = 0; bet < 4; bet++) {
+    const double c1 = yv[bet] / I->yorig;
+    const double c3 = xv[bet] / I->x;
+    const double c2 = (c3 - I->cborig * c1) / I->yorig;
+    const double c4 = (c1 - I->cborig * c3) / I->x;
+    for (int alf = 0; alf < 4; alf++) {
+      const int d_ab = (alf == bet);
+      for (int dta = 0; dta < 4; dta++) {
+        const int d_bd = (bet == dta), d_ad = (alf == dta);
+        const double t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
+        const double t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
+        const double t3 = xv[alf] * xv[dta] - d_ad * I->xsq / 4;
+        const double t4 = yv[alf] * yv[dta] - d_ad * I->ysq / 4;
+        const double t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * I->xdoty * d_ad;
+        dyv[16 * alf + 4 * bet + dta] = +t1 * T.ell2 + t2 * T.ell3 + t3 * (c1 * T.dell1dy + c2 * T.dell1dcb)
+          + t4 * (c1 * T.dell2dy + c2 * T.dell2dcb) + t5 * (c1 * T.dell3dy + c2 * T.dell3dcb);
+        dxv[16 * alf + 4 * bet + dta] = +t2 * T.ell1 + t1 * T.ell3 + t3 * (c3 * T.dell1dx + c4 * T.dell1dcb)
+          + t4 * (c3 * T.dell2dx + c4 * T.dell2dcb) + t5 * (c3 * T.dell3dx + c4 * T.dell3dcb);
+      }
+    }
+  }
+  FL(64 * 60);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ chnr_dV (chnr_dV.c:10-479) */
+typedef struct {
+  double g2, dg1dx, dg1dy, dg1dcb, dg2dx, dg2dy, dg2dcb, dg3dx, dg3dy, dg3dcb;
+  double ddg2dxdx, ddg2dxdy, ddg2dxdcb, ddg2dydcb, ddg2dcbdcb;
+  double ddg3dxdx, ddg3dxdy, ddg3dxdcb, ddg3dydcb, ddg3dcbdcb;
+  double ddg1dxdx, ddg1dxdy, ddg1dxdcb, ddg1dydcb, ddg1dcbdcb;
+  double ddg1dydy, ddg2dydy;
+} vtmp;
+
+static int taylorx_V(const oq_tables *T, vtmp *D, double x, double y, double xsq, bool flag2, double cb)
+{
+  const int iy_tay = oq_bsrch(T->TX[YYr], y, 0, T->NY_tay);
+  const int iy = oq_bsrch(T->YY, y, 0, T->nstpy);
+  if (iy_tay == T->NY_tay - 1 || iy == T->nstpy - 1) return 1;
+  const int iy2_tay = iy_tay + 1, ix = 0;
+  const double (*TX)[810] = T->TX;
+  const double ay = (TX[YYr][iy2_tay] - y) / (TX[YYr][iy2_tay] - TX[YYr][iy_tay]);
+  const double ax = (T->XX[0] - x) / (T->XX[0]);
+  const double xb = T->XX[0], xbsq = xb * xb;
+  const double g2a = 0.0;
+  const double g2b = accessv(T, false, true, ix, iy, QG2, false, 0, cb, y);
+  D->g2 = lerp(ax, g2a, g2b);
+  const double ddg2adxdcb = lerp(ay, TX[G21][iy_tay], TX[G21][iy2_tay]);
+  const double ddg2bdxdcb = accessv(T, false, true, ix, iy, dxQG2, false, 1, cb, y);
+  D->ddg2dxdcb = lerp(ax, ddg2adxdcb, ddg2bdxdcb);
+  const double dg2bdx = accessv(T, false, true, ix, iy, dxQG2, false, 0, cb, y);
+  D->dg2dx = D->ddg2dxdcb * cb;
+  const double dg2bdcb = accessv(T, false, true, ix, iy, QG2, false, 1, cb, y);
+  D->dg2dcb = D->ddg2dxdcb * x;
+  const double dg2bdy = accessv(T, false, false, ix, iy, QG2, true, 0, cb, y);
+  D->dg2dy = lerp(ax, 0.0, dg2bdy);
+  const double ddg2adxdy = cb * lerp(ay, TX[G21dy][iy_tay], TX[G21dy][iy2_tay]);
+  const double ddg2bdxdy = accessv(T, false, false, ix, iy, dxQG2, true, 0, cb, y);
+  D->ddg2dxdy = lerp(ax, ddg2adxdy, ddg2bdxdy);
+  const double ddg2adxdx = 2.0 * (lerp(ay, TX[G22A][iy_tay], TX[G22A][iy2_tay])
+                                  + lerp(ay, TX[G22B][iy_tay], TX[G22B][iy2_tay]) * (0.4 + 0.6 * (2.0 * cb * cb - 1)));
+  const double ddg2bdxdx = accessv(T, false, true, ix, iy, d2xQG2, false, 0, cb, y);
+  D->ddg2dxdx = lerp(ax, ddg2adxdx, ddg2bdxdx);
+  const double ddg2bdcbdcb = accessv(T, false, true, ix, iy, QG2, false, 2, cb, y);
+  D->ddg2dcbdcb = ddg2bdcbdcb * xsq / xbsq;
+  const double ddg2bdydcb = accessv(T, false, false, ix, iy, QG2, true, 1, cb, y);
+  D->ddg2dydcb = lerp(ax, 0.0, ddg2bdydcb);
+  const double dg3bdy = accessv(T, false, false, ix, iy, QG3, true, 0, cb, y);
+  const double ddg3bdxdy = accessv(T, false, false, ix, iy, dxQG3, true, 0, cb, y);
+  const double ddg3bdxdcb = accessv(T, false, true, ix, iy, dxQG3, false, 1, cb, y);
+  const double ddg3bdxdx = accessv(T, false, true, ix, iy, d2xQG3, false, 0, cb, y);
+  const double ddg3bdcbdcb = accessv(T, false, true, ix, iy, QG3, false, 2, cb, y);
+  const double ddg3bdydcb = accessv(T, false, false, ix, iy, QG3, true, 1, cb, y);
+  const double dg1bdy = dg3bdy - cb / xb * g2b - (y / xb) * cb * dg2bdy;
+  const double ddg1bdxdx = ddg3bdxdx - 2.0 * y / (xbsq * xb) * cb * g2b + y / xbsq * cb * dg2bdx
+    + y / xbsq * cb * dg2bdx - y / xb * cb * ddg2bdxdx;
+  const double ddg1bdxdy = ddg3bdxdy + cb / xbsq * g2b + y * cb / xbsq * dg2bdy - cb / xb * dg2bdx
+    - y / xb * cb * ddg2bdxdy;
+  const double ddg1bdxdcb = ddg3bdxdcb + y / xbsq * g2b + y / xbsq * cb * dg2bdcb - y / xb * dg2bdx
+    - y / xb * cb * ddg2bdxdcb;
+  const double ddg1bdydcb = ddg3bdydcb - g2b / xb - cb / xb * dg2bdcb - y / xb * dg2bdy - y / xb * cb * ddg2bdydcb;
+  const double ddg1bdcbdcb = ddg3bdcbdcb - 2.0 * y / xb * dg2bdcb - y / xb * cb * ddg2bdcbdcb;
+  const double dg1ady = lerp(ay, TX[G3Ady][iy_tay], TX[G3Ady][iy2_tay]) - lerp(ay, TX[G3Bdy][iy_tay], TX[G3Bdy][iy2_tay]);
+  const double ddg1adxdy = cb * (lerp(ay, TX[G31Ady][iy_tay], TX[G31Ady][iy2_tay])
+                                 - (2 / 3.) * lerp(ay, TX[G31Bdy][iy_tay], TX[G31Bdy][iy2_tay])
+                                 - lerp(ay, TX[G22A][iy_tay], TX[G22A][iy2_tay])
+                                 - y * lerp(ay, TX[G22Ady][iy_tay], TX[G22Ady][iy2_tay]));
+  const double ddg1adxdcb = lerp(ay, TX[G31A][iy_tay], TX[G31A][iy2_tay])
+    - (2 / 3.) * lerp(ay, TX[G31B][iy_tay], TX[G31B][iy2_tay]) - y * lerp(ay, TX[G22A][iy_tay], TX[G22A][iy2_tay]);
+  const double ddg1adxdx = ddg1bdxdx;
+  D->ddg1dxdcb = lerp(ax, ddg1adxdcb, ddg1bdxdcb);
+  D->dg1dx = D->ddg1dxdcb * cb;
+  D->dg1dcb = D->ddg1dxdcb * x;
+  D->dg1dy = lerp(ax, dg1ady, dg1bdy);
+  D->ddg1dxdy = lerp(ax, ddg1adxdy, ddg1bdxdy);
+  D->ddg1dxdx = lerp(ax, ddg1adxdx, ddg1bdxdx);
+  D->ddg1dcbdcb = ddg1bdcbdcb * xsq / xbsq;
+  D->ddg1dydcb = lerp(ax, 0.0, ddg1bdydcb);
+  if (flag2) {
+    const double ya = T->YY[iy], yb = T->YY[iy + 1];
+    const double ddg2adydy = 0.0;
+    const double dg2bdy_ya = accessv(T, false, false, ix, iy, QG2, true, 0, cb, ya);
+    const double dg2bdy_yb = accessv(T, false, false, ix, iy, QG2, true, 0, cb, yb);
+    const double ddg2bdydy = (dg2bdy_yb - dg2bdy_ya) / T->ystp;
+    D->ddg2dydy = ax * ddg2adydy + (1.0 - ax) * ddg2bdydy;
+    const double ddg1adydy = (TX[G3Ady][iy2_tay] - TX[G3Bdy][iy2_tay] - (TX[G3Ady][iy_tay] - TX[G3Bdy][iy_tay]))
+      / (TX[YYr][iy2_tay] - TX[YYr][iy_tay]);
+    const double dg3bdy_ya = accessv(T, false, false, ix, iy, QG3, true, 0, cb, ya);
+    const double dg3bdy_yb = accessv(T, false, false, ix, iy, QG3, true, 0, cb, yb);
+    const double ddg3bdydy = (dg3bdy_yb - dg3bdy_ya) / T->ystp;
+    const double ddg1bdydy = ddg3bdydy - (y * ddg2bdydy + 2.0 * dg2bdy) * cb / x;
+    D->ddg1dydy = ax * ddg1adydy + (1.0 - ax) * ddg1bdydy;
+  }
+  FL(250);
+  return 0;
+}
+
+static void xmyswap_V(vtmp *D, double x, double xmy, double y, double xsq, double xmysq, double ysq,
+                      double ca, double cborig)
+{
+  vtmp F = *D;
+  F.g2 = -D->g2; F.dg2dx = -D->dg2dx; F.dg2dy = -D->dg2dy; F.dg2dcb = -D->dg2dcb;
+  F.ddg2dxdx = -D->ddg2dxdx; F.ddg2dxdy = -D->ddg2dxdy; F.ddg2dydy = -D->ddg2dydy;
+  F.ddg2dxdcb = -D->ddg2dxdcb; F.ddg2dydcb = -D->ddg2dydcb; F.ddg2dcbdcb = -D->ddg2dcbdcb;
+  const double cd = (y - x * cborig) / xmy;
+  D->g2 = F.g2;
+  D->dg2dx = F.dg2dx + (1.0 - ca * ca) / xmy * F.dg2dcb + ca * F.dg2dy;
+  D->dg2dcb = -(ysq * cd * F.dg2dcb + x * y * xmy * F.dg2dy) / xmysq;
+  D->dg2dy = -(cborig + ca * cd) / xmy * F.dg2dcb + cd * F.dg2dy;
+  D->dg1dx = F.dg1dx + (1.0 - ca * ca) / xmy * F.dg1dcb + ca * F.dg1dy - D->dg2dx;
+  D->dg1dcb = -(ysq * cd * F.dg1dcb + x * y * xmy * F.dg1dy) / xmysq - D->dg2dcb;
+  D->dg1dy = -(cborig + ca * cd) / xmy * F.dg1dcb + cd * F.dg1dy - D->dg2dy;
+  D->ddg2dxdx = (F.ddg2dxdx + (1.0 - ca * ca) / xmy * F.ddg2dxdcb + ca * F.ddg2dxdy)
+    - 3.0 * ca * (1.0 - ca * ca) / xmysq * F.dg2dcb
+    + (1.0 - ca * ca) / xmy * (F.ddg2dxdcb + (1.0 - ca * ca) / xmy * F.ddg2dcbdcb + ca * F.ddg2dydcb)
+    + (1.0 - ca * ca) / xmy * F.dg2dy + ca * (F.ddg2dxdy + (1.0 - ca * ca) / xmy * F.ddg2dydcb + ca * F.ddg2dydy);
+  D->ddg2dxdcb = ysq / (xmysq * xmy) * (cborig + 3.0 * ca * cd) * F.dg2dcb
+    - ysq / xmysq * cd * (F.ddg2dxdcb + (1.0 - ca * ca) / xmy * F.ddg2dcbdcb + ca * F.ddg2dydcb)
+    + y / xmysq * (x * ca - xmy) * F.dg2dy - x * y / xmy * (F.ddg2dxdy + (1.0 - ca * ca) / xmy * F.ddg2dydcb
+                                                          + ca * F.ddg2dydy);
+  D->ddg2dxdy = (-(cborig + ca * cd) / xmy * F.ddg2dxdcb + cd * F.ddg2dxdy)
+    + (2.0 * ca * cborig + cd * (3.0 * ca * ca - 1.0)) / xmysq * F.dg2dcb
+    + (1.0 - ca * ca) / xmy * (-(cborig + ca * cd) / xmy * F.ddg2dcbdcb + cd * F.ddg2dydcb)
+    - (cborig + ca * cd) / xmy * F.dg2dy + ca * (-(cborig + ca * cd) / xmy * F.ddg2dydcb + cd * F.ddg2dydy);
+  D->ddg2dydcb = y / (xmysq * xmy) * ((3.0 * cd * cd - 1.0) * y - 2 * xmy * cd) * F.dg2dcb
+    - ysq / (xmysq) * cd * (-(cborig + ca * cd) / xmy * F.ddg2dcbdcb + cd * F.ddg2dydcb)
+    + x / xmysq * (y * cd - xmy) * F.dg2dy
+    - x * y / xmy * (-(cborig + ca * cd) / xmy * F.ddg2dydcb + cd * F.ddg2dydy);
+  D->ddg2dcbdcb = x * ysq / (xmysq * xmysq) * (xmy - 3.0 * y * cd) * F.dg2dcb
+    - ysq / (xmysq) * cd * (-(ysq * cd * F.ddg2dcbdcb + x * y * xmy * F.ddg2dydcb) / xmysq)
+    - xsq * ysq / (xmysq * xmy) * F.dg2dy
+    - x * y / xmy * (-(ysq * cd * F.ddg2dydcb + x * y * xmy * F.ddg2dydy) / xmysq);
+  D->ddg1dxdx = (F.ddg1dxdx + (1.0 - ca * ca) / xmy * F.ddg1dxdcb + ca * F.ddg1dxdy)
+    - 3.0 * ca * (1.0 - ca * ca) / xmysq * F.dg1dcb
+    + (1.0 - ca * ca) / xmy * (F.ddg1dxdcb + (1.0 - ca * ca) / xmy * F.ddg1dcbdcb + ca * F.ddg1dydcb)
+    + (1.0 - ca * ca) / xmy * F.dg1dy + ca * (F.ddg1dxdy + (1.0 - ca * ca) / xmy * F.ddg1dydcb + ca * F.ddg1dydy)
+    - D->ddg2dxdx;
+  D->ddg1dxdcb = ysq / (xmysq * xmy) * (cborig + 3.0 * ca * cd) * F.dg1dcb
+    - ysq / xmysq * cd * (F.ddg1dxdcb + (1.0 - ca * ca) / xmy * F.ddg1dcbdcb + ca * F.ddg1dydcb)
+    + y / xmysq * (x * ca - xmy) * F.dg1dy
+    - x * y / xmy * (F.ddg1dxdy + (1.0 - ca * ca) / xmy * F.ddg1dydcb + ca * F.ddg1dydy)
+    - D->ddg2dxdcb;
+  D->ddg1dxdy = (-(cborig + ca * cd) / xmy * F.ddg1dxdcb + cd * F.ddg1dxdy)
+    + (2.0 * ca * cborig + cd * (3.0 * ca * ca - 1.0)) / xmysq * F.dg1dcb
+    + (1.0 - ca * ca) / xmy * (-(cborig + ca * cd) / xmy * F.ddg1dcbdcb + cd * F.ddg1dydcb)
+    - (cborig + ca * cd) / xmy * F.dg1dy + ca * (-(cborig + ca * cd) / xmy * F.ddg1dydcb + cd * F.ddg1dydy)
+    - D->ddg2dxdy;
+  D->ddg1dydcb = y / (xmysq * xmy) * ((3.0 * cd * cd - 1.0) * y - 2 * xmy * cd) * F.dg1dcb
+    - ysq / (xmysq) * cd * (-(cborig + ca * cd) / xmy * F.ddg1dcbdcb + cd * F.ddg1dydcb)
+    + x / xmysq * (y * cd - xmy) * F.dg1dy - x * y / xmy * (-(cborig + ca * cd) / xmy * F.ddg1dydcb + cd * F.ddg1dydy)
+    - D->ddg2dydcb;
+  D->ddg1dcbdcb = x * ysq / (xmysq * xmysq) * (xmy - 3.0 * y * cd) * F.dg1dcb
+    - ysq / (xmysq) * cd * (-(ysq * cd * F.ddg1dcbdcb + x * y * xmy * F.ddg1dydcb) / xmysq)
+    - xsq * ysq / (xmysq * xmy) * F.dg1dy
+    - x * y / xmy * (-(ysq * cd * F.ddg1dydcb + x * y * xmy * F.ddg1dydy) / xmysq)
+    - D->ddg2dcbdcb;
+  FL(500);
+}
+
+static void get_dv(double x, const double xv[4], double y, const double yv[4], double cb, const vtmp *D,
+                   double dv[64])
+{
+  double xhat[4], yhat[4], c2v[4], c4v[4], D1[4], D2[4], D3[4], D4[4], D5[4], D6[4], D7[4], D8[4];
+  for (int i = 0; i < 4; i++) { xhat[i] = xv[i] / x; yhat[i] = yv[i] / y; }
+  for (int i = 0; i < 4; i++) {
+    c2v[i] = (xhat[i] - cb * yhat[i]) / y;
+    c4v[i] = (yhat[i] - cb * xhat[i]) / x;
+    D1[i] = xv[i] * D->ddg1dxdx + yv[i] * D->ddg2dxdx;
+    D2[i] = xv[i] * D->ddg1dxdy + yv[i] * D->ddg2dxdy;
+    D3[i] = xv[i] * D->ddg1dxdcb + yv[i] * D->ddg2dxdcb;
+    D4[i] = xv[i] * D->ddg1dydcb + yv[i] * D->ddg2dydcb;
+    D5[i] = xv[i] * D->ddg1dxdcb + yv[i] * D->ddg2dxdcb;
+    D6[i] = xv[i] * D->ddg1dcbdcb + yv[i] * D->ddg2dcbdcb;
+    D7[i] = (xv[i] * D->dg1dx + yv[i] * D->dg2dx) / x;
+    D8[i] = (xv[i] * D->dg1dcb + yv[i] * D->dg2dcb) / x;
+  }
+  for (int alf = 0; alf < 4; alf++) {
+    const double astuff = (D->dg1dx + D->dg2dx) * xhat[alf] + (D->dg1dcb + D->dg2dcb) * c4v[alf];
+    for (int bet = 0; bet < 4; bet++) {
+      const double bstuff = (D->dg1dx * xhat[bet] + D->dg1dy * yhat[bet]
+                             + D->dg1dcb * (xhat[bet] / y + yhat[bet] / x - cb * (xhat[bet] / x + yhat[bet] / y)));
+      const double c2c4 = c2v[bet] + c4v[bet];
+      const int d_ab = (alf == bet);
+      const double xaxb = xhat[alf] * xhat[bet], yayb = yhat[alf] * yhat[bet];
+      const double xayb = xhat[alf] * yhat[bet], yaxb = yhat[alf] * xhat[bet];
+      const double c1 = ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) / x + (d_ab + xayb * cb - xaxb - yayb) / y);
+      for (int dta = 0; dta < 4; dta++) {
+        const int d_bd = (bet == dta), d_ad = (alf == dta);
+        double f = d_bd * astuff;
+        f += d_ad * bstuff;
+        f += D1[dta] * xaxb + D2[dta] * xayb;
+        f += (D3[dta] * xhat[bet] + D4[dta] * yhat[bet]) * c4v[alf];
+        f += (D5[dta] * xhat[alf] + D6[dta] * c4v[alf]) * c2c4;
+        f += D7[dta] * (d_ab - xaxb);
+        dv[16 * alf + 4 * bet + dta] = f + D8[dta] * c1;
+      }
+    }
+  }
+  FL(64 * 30 + 200);
+}
+
+static int chnr_dV(const oq_tables *T, const oq_pc *PC, const double xv[4], const double yv[4],
+                   const oq_inv *I, double dv[64], double *sc)
+{
+  vtmp D;
+  memset(&D, 0, sizeof D);
+  if (I->x < T->XX[0]) {
+    if (taylorx_V(T, &D, I->x, I->y, I->xsq, I->flag2, I->cb)) return 1;
+  } else if (I->x > T->XX[T->nstpx - 1]) {
+    return 1;
+  } else {
+    const double x = I->x, y = I->y, cb = I->cb, xsq = I->xsq;
+    double f[4];
+    D.dg3dy = extractff(T, PC, QG3, true, 0, I);
+    if (sc) sc[0] = D.dg3dy;
+    extractff2(T, PC, QG2, 0, I, f);
+    if (sc) memcpy(sc + 1, f, sizeof f);
+    D.g2 = f[3]; D.dg2dy = f[2];
+    extractff2(T, PC, dxQG2, 0, I, f);
+    if (sc) memcpy(sc + 5, f, sizeof f);
+    D.ddg2dxdx = f[1]; D.dg2dx = f[3]; D.ddg2dxdy = f[2]; D.ddg2dxdcb = f[0];
+    extractff2(T, PC, dxQG3, 0, I, f);
+    if (sc) memcpy(sc + 9, f, sizeof f);
+    D.ddg3dxdx = f[1]; D.dg3dx = f[3]; D.ddg3dxdy = f[2]; D.ddg3dxdcb = f[0];
+    extractff2(T, PC, QG2, 1, I, f);
+    if (sc) memcpy(sc + 13, f, sizeof f);
+    D.dg2dcb = f[3]; D.ddg2dydcb = f[2]; D.ddg2dcbdcb = f[0];
+    extractff2(T, PC, QG3, 1, I, f);
+    if (sc) memcpy(sc + 17, f, sizeof f);
+    D.dg3dcb = f[3]; D.ddg3dydcb = f[2]; D.ddg3dcbdcb = f[0];
+    D.dg1dx = D.dg3dx + y / xsq * cb * D.g2 - y / x * cb * D.dg2dx;
+    D.dg1dy = D.dg3dy - cb / x * D.g2 - (y / x) * cb * D.dg2dy;
+    D.dg1dcb = D.dg3dcb - y / x * D.g2 - y / x * cb * D.dg2dcb;
+    D.ddg1dxdx = D.ddg3dxdx - 2.0 * y / (xsq * x) * cb * D.g2 + y / xsq * cb * D.dg2dx
+      + y / xsq * cb * D.dg2dx - y / x * cb * D.ddg2dxdx;
+    D.ddg1dxdy = D.ddg3dxdy + cb / xsq * D.g2 + y * cb / xsq * D.dg2dy - cb / x * D.dg2dx - y / x * cb * D.ddg2dxdy;
+    D.ddg1dxdcb = D.ddg3dxdcb + y / xsq * D.g2 + y / xsq * cb * D.dg2dcb - y / x * D.dg2dx - y / x * cb * D.ddg2dxdcb;
+    D.ddg1dydcb = D.ddg3dydcb - D.g2 / x - cb / x * D.dg2dcb - y / x * D.dg2dy - y / x * cb * D.ddg2dydcb;
+    D.ddg1dcbdcb = D.ddg3dcbdcb - 2.0 * y / x * D.dg2dcb - y / x * cb * D.ddg2dcbdcb;
+    FL(120);
+    if (I->flag2) {
+      const int iy1 = oq_bsrch(T->YY, y, 0, T->nstpy);
+      if (iy1 == T->nstpy - 1) return 1;
+      oq_inv I1 = *I, I2 = *I;
+      I1.y = T->YY[iy1];
+      I2.y = T->YY[iy1 + 1];
+      double fq1 = extractff(T, PC, QG2, true, 0, &I1);
+      double fq2 = extractff(T, PC, QG2, true, 0, &I2);
+      if (sc) { sc[21] = fq1; sc[22] = fq2; }
+      D.ddg2dydy = (fq2 - fq1) / T->ystp;
+      fq1 = extractff(T, PC, QG3, true, 0, &I1);
+      fq2 = extractff(T, PC, QG3, true, 0, &I2);
+      if (sc) { sc[23] = fq1; sc[24] = fq2; }
+      const double ddg3dydy = (fq2 - fq1) / T->ystp;
+      D.ddg1dydy = ddg3dydy - (y * D.ddg2dydy + 2.0 * D.dg2dy) * cb / x;
+      FL(12);
+    }
+  }
+  if (I->flag2) xmyswap_V(&D, I->x, I->xmy, I->yorig, I->xsq, I->xmysq, I->ysq, I->cb, I->cborig);
+  get_dv(I->x, xv, I->yorig, yv, I->cborig, &D, dv);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ Tabd (Tabd.c:23-316) */
+static const double dlta[4][4] = { { 1, 0, 0, 0 }, { 0, 1, 0, 0 }, { 0, 0, 1, 0 }, { 0, 0, 0, 1 } };
+
+int oq_tabd_xeq0(const oq_tables *T, const double yv[4], double t[192])
+{
+  double *tI = t, *tII = t + 64, *tIII = t + 128;
+  const double ysq = scal(yv, yv), y = sqrt(fabs(ysq));
+  const int iy_tay = oq_bsrch(T->TX[YYr], y, 0, T->NY_tay);
+  if (iy_tay == T->NY_tay - 1) return 1;
+  const int iy2 = iy_tay + 1;
+  const double (*TX)[810] = T->TX;
+  const double ay = (TX[YYr][iy2] - y) / (TX[YYr][iy2] - TX[YYr][iy_tay]);
+  const double fx = 0.5 * lerp(ay, TX[G0dx][iy_tay], TX[G0dx][iy2]);
+  const double fy = 0.5 * lerp(ay, TX[G0dy][iy_tay], TX[G0dy][iy2]);
+  double yhat[4], dg0dy[4], dg0dx[4];
+  for (int i = 0; i < 4; i++) { yhat[i] = yv[i] / y; }
+  for (int i = 0; i < 4; i++) { dg0dy[i] = yhat[i] * fy / 2; dg0dx[i] = yhat[i] * fx / 2; }
+  const double ell2 = lerp(ay, TX[Gl2][iy_tay], TX[Gl2][iy2]);
+  const double dell2adx = lerp(ay, TX[Gl21][iy_tay], TX[Gl21][iy2]);
+  const double dell2ady = lerp(ay, TX[Gl2dy][iy_tay], TX[Gl2dy][iy2]);
+  const double ell3a = lerp(ay, TX[Gl3][iy_tay], TX[Gl3][iy2]);
+  const double dg2dx = lerp(ay, TX[G21][iy_tay], TX[G21][iy2]);
+  const double ddg2dxdy = lerp(ay, TX[G21dy][iy_tay], TX[G21dy][iy2]);
+  const double dg1dx = (lerp(ay, TX[G31A][iy_tay], TX[G31A][iy2]) - (2. / 3) * (lerp(ay, TX[G31B][iy_tay], TX[G31B][iy2]))
+                        - y * (lerp(ay, TX[G22A][iy_tay], TX[G22A][iy2])));
+  const double dg1dy = (lerp(ay, TX[G3Ady][iy_tay], TX[G3Ady][iy2]) - lerp(ay, TX[G3Bdy][iy_tay], TX[G3Bdy][iy2]));
+  const double phi1 = (lerp(ay, TX[G22A][iy_tay], TX[G22A][iy2]) - lerp(ay, TX[G22B][iy_tay], TX[G22B][iy2]) / 5.);
+  const double phi2 = (lerp(ay, TX[G22B][iy_tay], TX[G22B][iy2])) * 1.2;
+  const double ysq_4 = ysq / 4.;
+  const double E = (ell3a + ell2), D2 = (dell2adx + dell2ady), D1 = (dg1dy + dg1dx), D12 = (dg1dx + dg2dx),
+               D3 = (y * ddg2dxdy - dg2dx);
+  double yvE[4], yvell3a[4], yhatD1[4], yhatD2[4], yhatD3[4], yhatdg2dx[4];
+  for (int i = 0; i < 4; i++) {
+    yvE[i] = yv[i] * E; yvell3a[i] = yv[i] * ell3a; yhatD1[i] = yhat[i] * D1;
+    yhatD2[i] = yhat[i] * D2; yhatD3[i] = yhat[i] * D3; yhatdg2dx[i] = yhat[i] * dg2dx;
+  }
+  int n = 0;
+  for (int alf = 0; alf < 4; alf++) {
+    const double yalf2dx = yhat[alf] * (dell2adx);
+    const double tIIsum = dg0dx[alf] - 0.5 * yvell3a[alf] - ysq_4 * yalf2dx;
+    const double yD12 = yhat[alf] * D12;
+    for (int bet = 0; bet < 4; bet++) {
+      const double dgSUM = (dg0dx[bet] + dg0dy[bet]);
+      const double tIsum = 2 * (dlta[alf][bet] * phi1 + yhat[alf] * yhat[bet] * phi2);
+      const double tIIIsum = dgSUM - 0.5 * yvE[bet] - ysq_4 * yhatD2[bet];
+      for (int dta = 0; dta < 4; dta++, n++) {
+        tIII[n] = (yv[alf] * yv[dta] * yhatD2[bet] + dlta[alf][bet] * yvE[dta] + dlta[bet][dta] * yvE[alf]
+                   + dlta[alf][dta] * tIIIsum);
+        tII[n] = (yv[bet] * yv[dta] * yalf2dx + dlta[bet][dta] * (tIIsum)
+                  + (dlta[bet][alf] * yvell3a[dta] + dlta[alf][dta] * yvell3a[bet]));
+        tI[n] = (dlta[bet][dta] * yD12 + yhatdg2dx[dta] * dlta[alf][bet] + yhatD3[bet] * yhat[alf] * yhat[dta]
+                 + yhatD1[bet] * dlta[alf][dta] + yv[dta] * tIsum);
+      }
+    }
+  }
+  FL(64 * 40 + 150);
+  return 0;
+}
+
+int oq_tabd_yeq0(const oq_tables *T, const double xv[4], double t[192])
+{
+  double *tI = t, *tII = t + 64, *tIII = t + 128;
+  const double xsq = scal(xv, xv), x = sqrt(fabs(xsq));
+  int ix_tay = oq_bsrch(T->XX, x, 0, T->nstpx);
+  if (x > T->XX[T->nstpx - 1]) return 1;
+  /* x equal to the last node: the reference reads one node past the grid; use the last cell */
+  if (ix_tay > T->nstpx - 2) ix_tay = T->nstpx - 2;
+  const int ix2 = ix_tay + 1;
+  const double (*TY)[100] = T->TY;
+  const double ax = (T->XX[ix2] - x) / (T->XX[ix2] - T->XX[ix_tay]);
+  const double xa = T->XX[ix_tay], xb = T->XX[ix2];
+  const double fx = 0.5 * lerp(ax, TY[alpha0dx_0p][ix_tay], TY[alpha0dx_0p][ix2]);
+  const double fy = (lerp(ax, TY[alpha0_1p][ix_tay], TY[alpha0_1p][ix2]));
+  double xhat[4], dg0dy[4], dg0dx[4];
+  for (int i = 0; i < 4; i++) xhat[i] = xv[i] / x;
+  for (int i = 0; i < 4; i++) { dg0dy[i] = xhat[i] * fy / 2; dg0dx[i] = xhat[i] * fx / 2; }
+  const double xa2 = xa * xa, xa3 = xa2 * xa, xa4 = xa3 * xa, xb2 = xb * xb, xb3 = xb2 * xb, xb4 = xb3 * xb;
+  const double ell1 = 4 * (lerp(ax, TY[alpha1_0p][ix_tay] / (xa4), TY[alpha1_0p][ix2] / (xb4))) / 3.;
+  const double ell3 = lerp(ax, TY[beta4_1p][ix_tay] / (xa2), TY[beta4_1p][ix2] / (xb2));
+  const double dell1dx = 4 * lerp(ax, (TY[alpha1dx_0p][ix_tay] - 4. * TY[alpha1_0p][ix_tay] / xa) / xa4,
+                                  (TY[alpha1dx_0p][ix2] - 4. * TY[alpha1_0p][ix2] / xb) / xb4) / 3.;
+  const double dell1dycb = 2 * lerp(ax, (4. * TY[alpha1_1p][ix_tay] / (3 * xa) - TY[beta4_1p][ix_tay]) / xa3,
+                                    (4. * TY[alpha1_1p][ix2] / (3 * xb) - TY[beta4_1p][ix2]) / xb3);
+  const double dg1dx = lerp(ax, TY[alpha3dx_0p][ix_tay], TY[alpha3dx_0p][ix2]);
+  const double ddg1dxdx = lerp(ax, TY[alpha3dxdx_0p][ix_tay], TY[alpha3dxdx_0p][ix2]);
+  const double dg2dx = 2 * lerp(ax, TY[beta2dx_1p][ix_tay], TY[beta2dx_1p][ix2]);
+  const double dg1dycb = 2 * lerp(ax, TY[alpha3_1p][ix_tay] - TY[beta2_1p][ix_tay] / xa,
+                                  TY[alpha3_1p][ix2] - TY[beta2_1p][ix2] / xb);
+  const double ddg1dxdycb = 2 * lerp(ax, TY[beta2_1p][ix_tay] / (xa2) - TY[beta2dx_1p][ix_tay] / xa + TY[alpha3dx_1p][ix_tay],
+                                     TY[beta2_1p][ix2] / (xb2) - TY[beta2dx_1p][ix2] / xb + TY[alpha3dx_1p][ix2]);
+  const double xsq_4 = xsq / 4., E = ell1 + ell3, D1 = dell1dx + dell1dycb, D2 = (ddg1dxdx + ddg1dxdycb),
+               D3 = (dg1dycb - dg2dx), D4 = (dg1dx + dg1dycb);
+  double xvE[4], xvell1[4];
+  for (int i = 0; i < 4; i++) { xvE[i] = xv[i] * E; xvell1[i] = xv[i] * ell1; }
+  /* loop form of Tabd.c:203-217 (the reference's default build takes the hand-unrolled
+   * UNROLL_BETDTA branch, Tabd.c:219-312, whose terms are the same with the zero deltas dropped) */
+  int n = 0;
+  for (int alf = 0; alf < 4; alf++) {
+    const double xhd1dx = xhat[alf] * (dell1dx);
+    const double tIIsum = (dg0dx[alf] - 0.5 * xvell1[alf]);
+    for (int bet = 0; bet < 4; bet++) {
+      const double Dg0SUM = (dg0dx[bet] + dg0dy[bet] - 0.5 * xvE[bet]);
+      for (int dta = 0; dta < 4; dta++, n++) {
+        const double xad = (xv[alf] * xv[dta] - xsq_4 * dlta[alf][dta]);
+        tIII[n] = ((xad * xhat[bet] * D1 + dlta[alf][bet] * xvE[dta] + dlta[bet][dta] * xvE[alf] + Dg0SUM * dlta[alf][dta]));
+        tII[n] = ((xv[bet] * xv[dta] - xsq_4 * dlta[bet][dta]) * xhd1dx + dlta[bet][alf] * xvell1[dta]
+                  + dlta[alf][dta] * xvell1[bet] + dlta[bet][dta] * (tIIsum));
+        tI[n] = ((dlta[alf][dta] * xhat[bet] + dlta[bet][dta] * xhat[alf] + dlta[alf][bet] * xhat[dta]
+                  - xhat[alf] * xhat[bet] * xhat[dta]) * D4 + xhat[alf] * (xhat[bet] * xv[dta] * D2 - dlta[bet][dta] * D3));
+      }
+    }
+  }
+  FL(64 * 40 + 150);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ assembly from Dirac traces */
+/* The reference hand-unrolls L = sum_A G^A_{delta[rho sigma] mu nu lambda alpha beta} T^(A)_{alpha beta delta}
+ * (CONSTRUCT_FULL_KERNEL, QED_kernel.c:16-423; CONSTRUCT_KERNEL_FULL_EQZERO, QED_kernel_xy0.c:8-399).
+ * The coefficients are Euclidean Dirac traces with Gam = 1/2 {g_delta, [g_rho, g_sigma]}:
+ *   G^I   =  1/8 Tr{Gam g_mu g_alpha g_nu g_beta g_lambda}
+ *   G^II  = -1/4 delta_{beta lambda} Tr{Gam g_mu g_alpha g_nu}
+ *   G^III = -1/4 delta_{alpha mu}    Tr{Gam g_nu g_beta g_lambda}
+ * Here they are evaluated with explicit 4x4 gamma matrices in a chiral basis. */
+typedef double complex cplx;
+static void mm4(const cplx *a, const cplx *b, cplx *c)
+{
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      cplx s = 0;
+      for (int k = 0; k < 4; k++) s += a[4 * i + k] * b[4 * k + j];
+      c[4 * i + j] = s;
+    }
+}
+static double trprod(const cplx *G, int n, const int *idx, const cplx gam[4][16])
+{
+  cplx P[16], Q[16];
+  memcpy(P, G, sizeof P);
+  for (int k = 0; k < n; k++) { mm4(P, gam[idx[k]], Q); memcpy(P, Q, sizeof P); }
+  return creal(P[0] + P[5] + P[10] + P[15]);
+}
+
+void oq_assembly_maps(double *Mstd, double *Meq0)
+{
+  cplx gam[4][16];
+  memset(gam, 0, sizeof gam);
+  /* gamma_k = [[0, -i s_k], [i s_k, 0]] (k = 1..3), gamma_4 = [[0, 1], [1, 0]] */
+  const cplx s[3][4] = { { 0, 1, 1, 0 }, { 0, -_Complex_I, _Complex_I, 0 }, { 1, 0, 0, -1 } };
+  for (int k = 0; k < 3; k++)
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        gam[k][4 * i + (j + 2)] = -_Complex_I * s[k][2 * i + j];
+        gam[k][4 * (i + 2) + j] = _Complex_I * s[k][2 * i + j];
+      }
+  for (int i = 0; i < 2; i++) { gam[3][4 * i + i + 2] = 1; gam[3][4 * (i + 2) + i] = 1; }
+  static const int RS[6][2] = { { 0, 1 }, { 0, 2 }, { 0, 3 }, { 1, 2 }, { 1, 3 }, { 2, 3 } };
+  double *GI = calloc(384 * 64, sizeof(double)), *GII = calloc(384 * 64, sizeof(double)),
+         *GIII = calloc(384 * 64, sizeof(double));
+  for (int rs = 0; rs < 6; rs++) {
+    const int r = RS[rs][0], sg = RS[rs][1];
+    for (int de = 0; de < 4; de++) {
+      cplx comm[16], t1[16], t2[16], Gm[16];
+      mm4(gam[r], gam[sg], t1); mm4(gam[sg], gam[r], t2);
+      for (int i = 0; i < 16; i++) comm[i] = t1[i] - t2[i];
+      mm4(gam[de], comm, t1); mm4(comm, gam[de], t2);
+      for (int i = 0; i < 16; i++) Gm[i] = 0.5 * (t1[i] + t2[i]);
+      for (int mu = 0; mu < 4; mu++)
+        for (int nu = 0; nu < 4; nu++)
+          for (int la = 0; la < 4; la++)
+            for (int al = 0; al < 4; al++)
+              for (int be = 0; be < 4; be++) {
+                const int row = 64 * rs + 16 * mu + 4 * nu + la, col = 16 * al + 4 * be + de;
+                const int i5[5] = { mu, al, nu, be, la };
+                GI[64 * row + col] = trprod(Gm, 5, i5, gam) / 8;
+                if (be == la) { const int i3[3] = { mu, al, nu }; GII[64 * row + col] = -trprod(Gm, 3, i3, gam) / 4; }
+                if (al == mu) { const int i3[3] = { nu, be, la }; GIII[64 * row + col] = -trprod(Gm, 3, i3, gam) / 4; }
+              }
+    }
+  }
+  if (Meq0) {
+    for (int row = 0; row < 384; row++)
+      for (int c = 0; c < 64; c++) {
+        Meq0[192 * row + c] = GI[64 * row + c];
+        Meq0[192 * row + 64 + c] = GII[64 * row + c];
+        Meq0[192 * row + 128 + c] = GIII[64 * row + c];
+      }
+  }
+  if (Mstd) {
+    /* standard inputs: Vv, Txv[a][b][d] = d^x_b T_ad, Tyv = d^y_b T_ad, Sxv, Syv;
+     * T^I = Vv, T^II_{abd} = Txv[b][a][d] + 1/4 d_bd Sxv[a],
+     * T^III_{abd} = Txv[a][b][d] + Tyv[a][b][d] + 1/4 d_ad (Sxv + Syv)[b] */
+    memset(Mstd, 0, sizeof(double) * 384 * 200);
+    for (int row = 0; row < 384; row++)
+      for (int al = 0; al < 4; al++)
+        for (int be = 0; be < 4; be++)
+          for (int de = 0; de < 4; de++) {
+            const int c = 16 * al + 4 * be + de;
+            double *m = Mstd + 200 * row;
+            m[c] += GI[64 * row + c];
+            m[64 + 16 * be + 4 * al + de] += GII[64 * row + c];
+            if (be == de) m[192 + al] += 0.25 * GII[64 * row + c];
+            m[64 + c] += GIII[64 * row + c];
+            m[128 + c] += GIII[64 * row + c];
+            if (al == de) { m[192 + be] += 0.25 * GIII[64 * row + c]; m[196 + be] += 0.25 * GIII[64 * row + c]; }
+          }
+  }
+  free(GI); free(GII); free(GIII);
+}
+
+static double g_Mstd[384 * 200], g_Meq0[384 * 192];
+static int g_maps_ready = 0;
+static void maps_init(void)
+{
+  if (g_maps_ready) return;
+#pragma omp critical(oq_maps)
+  {
+    if (!g_maps_ready) { oq_assembly_maps(g_Mstd, g_Meq0); g_maps_ready = 1; }
+  }
+}
+static void apply_map(const double *M, int ncol, const double *in, double *out)
+{
+  int nnz = 0;
+  for (int r = 0; r < 384; r++) {
+    double s = 0.0;
+    const double *m = M + (size_t)ncol * r;
+    for (int c = 0; c < ncol; c++) if (m[c] != 0.0) { s += m[c] * in[c]; nnz++; }
+    out[r] = s;
+  }
+  FL(2.0 * nnz);
+}
+
+/* ------------------------------------------------------------------ STV + kernels */
+static int stv_full(const oq_tables *T, const double xv[4], const double yv[4], double stv[200], double *sc)
+{
+  const oq_inv I = set_inv(T, xv, yv);
+  memset(stv, 0, 200 * sizeof(double));
+  static __thread oq_pc PC;
+  const bool avx = !(I.x >= T->XX[T->nstpx - 1]) && !(I.y >= T->YY[T->nstpy - 1]) && !(I.x < T->XX[0]);
+  if (avx || I.x >= T->XX[0]) pack_pc(T, &I, &PC);
+  double s45[45];
+  if (sc) memset(s45, 0, sizeof s45);
+  if (chnr_dS(T, &PC, xv, yv, &I, stv + 192, stv + 196, sc ? s45 : NULL)) return 1;
+  if (chnr_dT(T, &PC, xv, yv, &I, stv + 64, stv + 128, sc ? s45 + 4 : NULL)) return 1;
+  if (chnr_dV(T, &PC, xv, yv, &I, stv, sc ? s45 + 16 : NULL)) return 1;
+  if (sc) memcpy(sc, s45, sizeof s45);
+  return 0;
+}
+
+int oq_stv(const oq_tables *T, const double x[4], const double y[4], double stv[200])
+{
+  return stv_full(T, x, y, stv, NULL);
+}
+
+int oq_interp_scalars(const oq_tables *T, const double x[4], const double y[4], double s[45])
+{
+  double stv[200];
+  memset(s, 0, 45 * sizeof(double));
+  stv_full(T, x, y, stv, s);
+  return oq_regime(T, x, y);
+}
+
+static inline bool is_zero(const double x[4]) { return (x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3]) < 1E-28; }
+static inline bool x_is_y(const double x[4], const double y[4])
+{
+  const double d[4] = { x[0] - y[0], x[1] - y[1], x[2] - y[2], x[3] - y[3] };
+  return is_zero(d);
+}
+
+int oq_regime(const oq_tables *T, const double x[4], const double y[4])
+{
+  const bool xz = is_zero(x), yz = is_zero(y);
+  if (xz && yz) return 0;
+  if (xz) return 1;
+  if (yz) return 2;
+  if (x_is_y(x, y)) return 3;
+  const oq_inv I = set_inv(T, x, y);
+  const int sw = I.flag2 ? 8 : 0;
+  if (I.x < T->XX[0]) return 4 + sw;
+  if (I.x > T->XX[T->nstpx - 1]) return 6 + sw;
+  return 5 + sw;
+}
+
+/* i_to_mulam / kernel_mulam_minus (kernels.c:30-48) */
+static void mulam_minus(double *k)
+{
+  double t[384];
+  memcpy(t, k, sizeof t);
+  for (int i = 0; i < 384; i++) {
+    const int l0 = i / 64, l1 = (i / 16) & 3, l2 = (i / 4) & 3, l3 = i & 3;
+    k[i] = -t[l1 + 4 * (l2 + 4 * (l3 + 4 * l0))];
+  }
+}
+
+void oq_L0(const oq_tables *T, const double xv[4], const double yv[4], double k[384])
+{
+  maps_init();
+  const bool xz = is_zero(xv), yz = is_zero(yv);
+  memset(k, 0, 384 * sizeof(double));
+  if (xz && yz) return;
+  double t[200];
+  if (xz) { if (!oq_tabd_xeq0(T, yv, t)) apply_map(g_Meq0, 192, t, k); return; }
+  if (yz) { if (!oq_tabd_yeq0(T, xv, t)) apply_map(g_Meq0, 192, t, k); return; }
+  if (x_is_y(xv, yv)) {
+    if (!oq_tabd_yeq0(T, xv, t)) apply_map(g_Meq0, 192, t, k);
+    mulam_minus(k);
+    return;
+  }
+  if (!stv_full(T, xv, yv, t, NULL)) apply_map(g_Mstd, 200, t, k);
+}
+
+/* atomic_kernel_Saxpy (kernels.c:52-78), FMA build */
+static void saxpy(double *k, double S, const double *kt)
+{
+  for (int i = 0; i < 384; i++) k[i] = fma(kt[i], S, k[i]);
+  FL(768);
+}
+
+void oq_L1(const oq_tables *T, const double x[4], const double y[4], double k[384])
+{
+  double kt[384];
+  oq_L0(T, x, y, k);
+  oq_L0(T, x, x, kt); saxpy(k, -0.5, kt);
+  oq_L0(T, y, y, kt); saxpy(k, -0.5, kt);
+}
+void oq_L2(const oq_tables *T, const double x[4], const double y[4], double k[384])
+{
+  double kt[384];
+  const double z[4] = { 0, 0, 0, 0 };
+  oq_L0(T, x, y, k);
+  oq_L0(T, z, y, kt); saxpy(k, -1, kt);
+  oq_L0(T, x, z, kt); saxpy(k, -1, kt);
+}
+void oq_L3(const oq_tables *T, const double x[4], const double y[4], double k[384])
+{
+  double kt[384];
+  const double z[4] = { 0, 0, 0, 0 };
+  oq_L0(T, x, y, k);
+  oq_L0(T, x, x, kt); saxpy(k, -1, kt);
+  oq_L0(T, z, x, kt); saxpy(k, +1, kt);
+  oq_L0(T, z, y, kt); saxpy(k, -1, kt);
+}
+/* QED_Mkernel_L2 (kernels.c:181-231) */
+void oq_ML2(const oq_tables *T, double M, const double xv[4], const double yv[4], double k[384])
+{
+  double kt[384];
+  const double z[4] = { 0, 0, 0, 0 };
+  oq_L0(T, xv, yv, k);
+  const double gaussX = exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.);
+  oq_L0(T, z, yv, kt);
+  for (int rs = 0; rs < 6; rs++)
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = 0; nu < 4; nu++)
+        for (int la = 0; la < 4; la++) {
+          const double *K = kt + 64 * rs;
+          k[64 * rs + 16 * mu + 4 * nu + la] -= gaussX * (K[16 * mu + 4 * nu + la] -
+            M * xv[mu] * (xv[0] * K[0 + 4 * nu + la] + xv[1] * K[16 + 4 * nu + la] + xv[2] * K[32 + 4 * nu + la]
+                          + xv[3] * K[48 + 4 * nu + la]));
+        }
+  const double gaussY = exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.);
+  oq_L0(T, xv, z, kt);
+  for (int rs = 0; rs < 6; rs++)
+    for (int mu = 0; mu < 4; mu++)
+      for (int nu = 0; nu < 4; nu++)
+        for (int la = 0; la < 4; la++) {
+          const double *K = kt + 64 * rs + 16 * mu;
+          k[64 * rs + 16 * mu + 4 * nu + la] -= gaussY * (K[4 * nu + la] -
+            M * yv[nu] * (yv[0] * K[0 + la] + yv[1] * K[4 + la] + yv[2] * K[8 + la] + yv[3] * K[12 + la]));
+        }
+  FL(384 * 24);
+}
+
+void oq_batch(const oq_tables *T, int which, double M, size_t n, const double *x, const double *y,
+              double *out, int nthreads)
+{
+  maps_init();
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+  for (size_t i = 0; i < n; i++) {
+    const double *xi = x + 4 * i, *yi = y + 4 * i;
+    double *o = out + 384 * i;
+    switch (which) {
+    case 0: oq_L0(T, xi, yi, o); break;
+    case 1: oq_L1(T, xi, yi, o); break;
+    case 2: oq_L2(T, xi, yi, o); break;
+    case 3: oq_L3(T, xi, yi, o); break;
+    default: oq_ML2(T, M, xi, yi, o); break;
+    }
+  }
+  (void)nthreads;
+}

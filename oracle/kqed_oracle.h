@@ -1,0 +1,85 @@
+/*
+ * kqed_oracle.h -- CPU oracle for the position-space QED kernel L_[rho,sigma];mu,nu,lambda(x,y).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference library's FP64 CPU
+ * evaluation (the AVX+FMA build of the src/LIBFILES sources), used as the checker for the HIP path.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the product
+ * library (kqed_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Floating-point evaluation follows
+ * the reference's expression order; it is compiled with -ffp-contract=off so that every fused
+ * multiply-add is one the reference writes explicitly (_mm256_fmadd_pd/_mm256_fmsub_pd).
+ */
+#ifndef KQED_ORACLE_H
+#define KQED_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Table container: the reference's struct Grid_coeffs (src/HEADERS/KQED.h:69-87), flattened. */
+typedef struct {
+  int nstpx, nstpy, Nffa, nfmax;
+  double *XX, *YY;
+  double xstp, ystp;
+  int *nfx;                 /* [nstpx] */
+  float *Ffm, *Ffp;         /* [Nffa][nstpx][nstpy][nfmax], zero padded beyond nfx[ix] */
+  double TX[23][810];       /* TAYLORX rows, KQED.h:57 */
+  double TY[14][100];       /* TAYLORY rows, KQED.h:60 */
+  int NY_tay, NX_tay;
+  double *LUT[7];           /* ratio look-up tables, simd.c:716-784; index 1..6 */
+  int nLUT;
+} oq_tables;
+
+/* CRC-32C and the DML rotate-xor accumulation (src/UTILS/crc32c.c:160-172). */
+uint32_t oq_crc32c(uint32_t crc, const void *buf, size_t len);
+void oq_checksum_accum(uint32_t *a, uint32_t *b, uint32_t rank, const void *buf, size_t len);
+
+/* Load FFxy_single_cksum.bin, taylorx_cksum.bin, taylory_cksum.bin from dir (io.c:42-288).
+ * Returns 0 on success, 1 on a missing file, bad magic, bad dimension or checksum mismatch. */
+int oq_load(const char *dir, oq_tables *T);
+void oq_free(oq_tables *T);
+
+/* Public kernels (src/LIBFILES/kernels.c:82-231).  k has 384 doubles, flat index
+ * 64*rhosig + 16*mu + 4*nu + lambda. */
+void oq_L0(const oq_tables *T, const double x[4], const double y[4], double k[384]);
+void oq_L1(const oq_tables *T, const double x[4], const double y[4], double k[384]);
+void oq_L2(const oq_tables *T, const double x[4], const double y[4], double k[384]);
+void oq_L3(const oq_tables *T, const double x[4], const double y[4], double k[384]);
+void oq_ML2(const oq_tables *T, double M, const double x[4], const double y[4], double k[384]);
+
+/* Batched driver (OpenMP over pairs) for the CPU baseline: which = 0..3 for L0..L3, 4 = M-L2. */
+void oq_batch(const oq_tables *T, int which, double M, size_t n, const double *x, const double *y,
+              double *out, int nthreads);
+
+/* Pieces exposed for unit tests. */
+double oq_chebU(int order, int nk, double x, const double *co);          /* cheby.c:13-91 */
+double oq_chebU_avx(int order, int nk, double x, const double *co, const oq_tables *T); /* simd.c:182-336 */
+int oq_bsrch(const double *arr, double target, int lo, int hi);         /* getff-new.c:199-210 */
+/* S/T/V tensors of the standard path (init_STV, QED_kernel.c:427-520); returns 1 when out of grid.
+ * stv layout: Vv[64], Txv[64], Tyv[64], Sxv[4], Syv[4]. */
+int oq_stv(const oq_tables *T, const double x[4], const double y[4], double stv[200]);
+/* Tabd_xeq0 / Tabd_yeq0 (Tabd.c:23-316): t = tI[64], tII[64], tIII[64]. */
+int oq_tabd_xeq0(const oq_tables *T, const double y[4], double t[192]);
+int oq_tabd_yeq0(const oq_tables *T, const double x[4], double t[192]);
+/* Dense assembly maps (384 x 200 and 384 x 192) built from the Dirac traces; row-major. */
+void oq_assembly_maps(double *Mstd, double *Meq0);
+/* The 32(+2) interpolated weight-function scalars of the standard regime, in this order:
+ * QG0{dcb,dx,dy,v}, QL2{4}, QL4{4}, QG1{4}, QG3dy, QG2(d0){4}, dxQG2{4}, dxQG3{4}, QG2(d1){4},
+ * QG3(d1){4}, swap: fq1,fq2(QG2), fq1,fq2(QG3).  Returns the regime code (see oq_regime). */
+int oq_interp_scalars(const oq_tables *T, const double x[4], const double y[4], double s[45]);
+/* Regime of a pair: 0 zero, 1 x=0, 2 y=0, 3 x=y, 4 Taylor-x, 5 standard, 6 out of grid;
+ * +8 when the x<->x-y swap is active. */
+int oq_regime(const oq_tables *T, const double x[4], const double y[4]);
+
+/* Algorithmic FP64 flop count of the last oq_L0 calls on this thread (reference algorithm). */
+double oq_flops_get(void);
+void oq_flops_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
