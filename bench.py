@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 QED kernel L(x,y) on one MI355X, in evaluations per second.
+
+Workload (the plain-FP64 GPU configuration): 2^24 (x,y) pairs, |x| and |y| log-uniform in
+[1e-3, 4.0], isotropic directions (seed 2), FP64 in and out, synthetic form-factor table "full"
+(reference file format; the real table is not available).  One step = one batched evaluation of
+QED_kernel_L0 over all pairs (inputs resident in HBM, 384 FP64 outputs per pair written to HBM).
+L1, L2, L3 and M-L2 (M = 0.5) rates are measured on the same inputs and reported beside it.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on a single GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+PEAK_FP64 = 78.6e12  # MI355X FP64 vector / matrix, FLOP/s
+PEAK_HBM = 8.0e12
+
+
+def workload(n, seed=2):
+    from pairs import log_uniform
+    return log_uniform(seed, n)
+
+
+def cpu_baseline(table, x, y, budget_s=12.0):
+    """The CPU oracle (restatement of the reference's AVX+FMA FP64 path), OpenMP on host cores."""
+    from oracle_ctypes import Oracle
+    o = Oracle(table)
+    cores = min(16, os.cpu_count() or 1)
+    done, t0, chunk = 0, time.time(), 1024 * cores
+    sample_out = None
+    while time.time() - t0 < budget_s and done + chunk <= len(x):
+        r = o.batch(0, x[done:done + chunk], y[done:done + chunk], nthreads=cores)
+        if sample_out is None:
+            sample_out = r
+        done += chunk
+    dt = time.time() - t0
+    o.flops_reset()
+    o.batch(0, x[:400], y[:400], nthreads=1)
+    flops = o.flops() / 400
+    o.close()
+    return {"value": done / dt, "unit": "evals/s", "cores": cores, "kind": "port",
+            "sample": f"first {done} pairs of the workload, L0, {dt:.1f} s"}, flops, sample_out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1 << 24)
+    ap.add_argument("--table", default="full")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import kqed_amd
+    from pairs import normwise
+
+    table = kqed_amd.table_dir(a.table)
+    t = kqed_amd.QEDKernelTemps.initialise(table)
+    x, y = workload(a.n)
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(x).to(dev)
+    yd = torch.from_numpy(y).to(dev)
+    out = torch.empty((a.n, 384), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(which=0, M=0.0, n=a.n):
+        t.batch_device(which, n, xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=M, stream=stream)
+
+    for _ in range(a.warmup):
+        step()
+    torch.cuda.synchronize()
+    t.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stages = t.profile(False)
+    evals = a.n * a.steps / dt
+
+    # parity of the measured output against the oracle on a sample of the workload
+    extra = {}
+    cpu = None
+    if not a.no_cpu:
+        cpu, flops_per_eval, ref = cpu_baseline(table, x, y)
+        m = len(ref)
+        got = out[:m].cpu().numpy()
+        err = normwise(got, ref)
+        extra["parity_sample"] = {"pairs": m, "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err))}
+    else:
+        flops_per_eval = None
+
+    if not a.no_extra:
+        rates = {}
+        for name, which, M in (("L1", 1, 0.0), ("L2", 2, 0.0), ("L3", 3, 0.0), ("ML2_M0.5", 4, 0.5)):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            step(which, M)
+            torch.cuda.synchronize()
+            rates[name] = a.n / (time.perf_counter() - t1)
+        extra["evals_per_s_other_kernels"] = rates
+
+    ms_i, cnt_i = stages["interp"]
+    ms_a, cnt_a = stages["assemble"]
+    ms_s, cnt_s = stages["sort"]
+    dom = "interp" if ms_i >= ms_a else "assemble"
+    roof = None
+    if flops_per_eval:
+        # algorithmic FP64 flops of the whole reference L0 evaluation, attributed to the launches
+        pairs_per_launch = a.n / max(cnt_i, 1)
+        avg = (ms_i + ms_a) / max(cnt_i, 1) / 1e3
+        achieved = flops_per_eval * pairs_per_launch / avg
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64, "traffic": None,
+                "kernel": "k_interp + k_assemble (one L0 chunk)",
+                "flops_per_eval": flops_per_eval}
+    res = {
+        "metric": "L(x,y) evals/s, FP64 plain vs surrogate+FP32 at <=1e-5 max rel err",
+        "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "QED_kernel_L0, 2^24 pairs, |x|,|y| log-uniform [1e-3,4], isotropic, FP64",
+                   "pairs": a.n, "table": a.table, "kernel": "L0", "path": "plain FP64"},
+        "stages_ms_per_step": {"sort": ms_s / a.steps, "interp": ms_i / a.steps, "assemble": ms_a / a.steps},
+        "dominant_kernel": dom,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    res.update(extra)
+    t.free()
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+/*
+ * kqed.h -- C ABI of the MI355X implementation of KQED's position-space QED kernel
+ * L_[rho,sigma];mu,nu,lambda(x,y).
+ *
+ * The first block keeps the reference library's public API (reference src/HEADERS/KQED.h,
+ * init.h, kernels.h) so that code written against it recompiles unchanged.  The second block adds
+ * batched, device-pointer entry points.  All pointers are plain host or device pointers; no HIP
+ * or torch type appears in a signature (streams are passed as void*, a hipStream_t).
+ *
+ * Output layout everywhere: double kerv[6][4][4][4], flat index 64*rhosig + 16*mu + 4*nu + lambda,
+ * rhosig enumerating (0,1),(0,2),(0,3),(1,2),(1,3),(2,3) (reference pi_pert.c:196-199).
+ */
+#ifndef KQED_H
+#define KQED_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- reference-compatible types */
+/* Replaces struct Grid_coeffs (reference KQED.h:69-87).  Host mirrors of the grids are kept for
+ * callers that read them; the tables themselves live in device memory behind `dev`. */
+struct Grid_coeffs {
+  int Nffa;
+  double *XX;
+  double *YY;
+  double xstp;
+  double ystp;
+  int nstpx;
+  int nstpy;
+  int NtayY;
+  int NY_tay;
+  int NtayX;
+  int NX_tay;
+  int *nfx;
+};
+
+/* Replaces struct QED_kernel_temps (reference KQED.h:90-93), passed by value as in the reference. */
+struct QED_kernel_temps {
+  struct Grid_coeffs Grid;
+  int *G8;                     /* 65536 eight-gamma traces, as the reference's init_g8 */
+  struct kqed_device *dev;     /* opaque: device-resident tables and assembly maps */
+};
+
+/* ---------------------------------------------------------------- reference-compatible API */
+/* Loads FFxy_single_cksum.bin, taylorx_cksum.bin and taylory_cksum.bin from $KQED_TABLE_DIR
+ * (else the compile-time default directory), verifies their checksums, builds the device tables.
+ * Returns 0 on success, 1 on error (message on stderr).  Replaces reference init.c:142-190. */
+int initialise(struct QED_kernel_temps *t);
+/* Frees host and device memory; safe on a failed or zeroed struct.  Replaces init.c:38-138. */
+void free_QED_temps(struct QED_kernel_temps *t);
+
+/* One (x,y) pair each, synchronous, host pointers; overwrite all 384 outputs.  Out-of-grid inputs
+ * give an all-zero kernel, as in the reference.  Replace reference kernels.h:4-33 /
+ * kernels.c:82-231. */
+void QED_kernel_L0(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                   double kerv[6][4][4][4]);
+void QED_kernel_L1(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                   double kerv[6][4][4][4]);
+void QED_kernel_L2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                   double kerv[6][4][4][4]);
+void QED_kernel_L3(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                   double kerv[6][4][4][4]);
+void QED_Mkernel_L2(const double M, const double xv[4], const double yv[4],
+                    const struct QED_kernel_temps t, double kerv[6][4][4][4]);
+
+/* ---------------------------------------------------------------- batched entry points (new) */
+enum kqed_kernel { KQED_L0 = 0, KQED_L1 = 1, KQED_L2 = 2, KQED_L3 = 3, KQED_ML2 = 4 };
+enum kqed_layout { KQED_AOS = 0, /* out[n][384] */ KQED_SOA = 1 /* out[384][n] */ };
+
+/* Initialise from an explicit table directory (initialise() with a path argument). */
+int kqed_initialise_dir(struct QED_kernel_temps *t, const char *table_dir);
+
+/* n pairs, device pointers: x[n][4], y[n][4] (FP64), out FP64 in `layout`.  Enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream); returns 0 or a negative error code without waiting.
+ * Replaces the reference GPU port's per-array wrapper cu_arr_QED_kernel_L0
+ * (reference src_cuda/amu_forlattice.cu:184-202) and extends it to L1..L3 and M-L2. */
+int kqed_batch(const struct QED_kernel_temps *t, int which, double M, size_t n,
+               const double *x_dev, const double *y_dev, double *out_dev, int layout, void *stream);
+
+/* Same on host buffers (copies in and out, synchronous). */
+int kqed_batch_host(const struct QED_kernel_temps *t, int which, double M, size_t n,
+                    const double *x, const double *y, double *out, int layout);
+
+/* Bytes of device scratch kqed_batch needs for n pairs (allocated and cached by the library). */
+size_t kqed_batch_workspace(size_t n);
+
+/* Stage timing of kqed_batch on its own stream (HIP events around every kernel launch).
+ * Reads and resets the accumulated milliseconds and launch counts of the stages
+ * {0: cell sort, 1: interpolation kernel, 2: assembly kernel} (either pointer may be NULL; arrays
+ * of 3), then turns timing on (enable != 0) or off.  Waits for the recorded events. */
+int kqed_profile(const struct QED_kernel_temps *t, int enable, double *ms, long *count);
+
+/* Last error text of this thread ("" if none). */
+const char *kqed_last_error(void);
+
+/* ---------------------------------------------------------------- table utilities (new) */
+/* Write a synthetic form-factor table in the reference's file format (see kqed_tables.h). */
+int kqed_write_synthetic_ff(const char *path, int nstpx, int nstpy, int nf0, int nf1, double xh,
+                            double yh, unsigned long long seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""kqed_amd -- MI355X (gfx950) implementation of KQED's position-space QED kernel.
+
+Host-side mirror of the reference C interface (reference src/HEADERS/init.h, kernels.h) over the
+C ABI of ``libkqed_hip.so`` (declared in ``include/kqed.h``):
+
+    t = QEDKernelTemps.initialise(table_dir)      # initialise()
+    k = t.QED_kernel_L0(x, y)                     # double kerv[6][4][4][4] as a (6,4,4,4) array
+    t.QED_kernel_L1 / L2 / L3, t.QED_Mkernel_L2(M, x, y)
+    t.free()                                      # free_QED_temps()
+
+plus the batched entry points (host arrays or device pointers).  Every evaluation runs in the HIP
+library; there is no CPU fallback, and loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+import shutil
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libkqed_hip.so")
+GOLDEN_PRECOMP = os.path.join(ROOT, "tests", "golden", "PRECOMP")
+
+KQED_L0, KQED_L1, KQED_L2, KQED_L3, KQED_ML2 = range(5)
+KQED_AOS, KQED_SOA = 0, 1
+
+_lib = None
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [
+        ("Nffa", ctypes.c_int), ("XX", ctypes.POINTER(ctypes.c_double)), ("YY", ctypes.POINTER(ctypes.c_double)),
+        ("xstp", ctypes.c_double), ("ystp", ctypes.c_double), ("nstpx", ctypes.c_int), ("nstpy", ctypes.c_int),
+        ("NtayY", ctypes.c_int), ("NY_tay", ctypes.c_int), ("NtayX", ctypes.c_int), ("NX_tay", ctypes.c_int),
+        ("nfx", ctypes.POINTER(ctypes.c_int)),
+    ]
+
+
+class _Temps(ctypes.Structure):
+    _fields_ = [("Grid", _Grid), ("G8", ctypes.POINTER(ctypes.c_int)), ("dev", ctypes.c_void_p)]
+
+
+def lib():
+    """Load libkqed_hip.so (built by kqed_amd.build); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m kqed_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER(_Temps)
+    arr = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+    L.initialise.argtypes = [P]
+    L.kqed_initialise_dir.argtypes = [P, ctypes.c_char_p]
+    L.free_QED_temps.argtypes = [P]
+    for name in ("QED_kernel_L0", "QED_kernel_L1", "QED_kernel_L2", "QED_kernel_L3"):
+        getattr(L, name).argtypes = [arr, arr, _Temps, arr]
+        getattr(L, name).restype = None
+    L.QED_Mkernel_L2.argtypes = [ctypes.c_double, arr, arr, _Temps, arr]
+    L.QED_Mkernel_L2.restype = None
+    L.kqed_batch.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.kqed_batch_host.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, arr, arr, arr, ctypes.c_int]
+    L.kqed_batch_workspace.argtypes = [ctypes.c_size_t]
+    L.kqed_batch_workspace.restype = ctypes.c_size_t
+    L.kqed_last_error.restype = ctypes.c_char_p
+    L.kqed_profile.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]
+    L.kqed_write_synthetic_ff.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]
+    _lib = L
+    return L
+
+
+# ----------------------------------------------------------------------------- table directories
+# Synthetic form-factor tables (the reference's real FFxy_single_cksum.bin is not available):
+#   "full":  100 x-nodes (the reference's x-grid layout), 810 y-nodes on the Taylor-x y step
+#            (6.54/810), 32..64 Chebyshev terms per x-node (capped by float range at small x)
+#   "small": the same x-grid, 218 y-nodes of step 0.03, 8..16 terms (fast CPU tests)
+TABLES = {
+    "full": dict(nstpx=100, nstpy=810, nf0=32, nf1=64, xh=6.54 / 810, yh=6.54 / 810, seed=20240611),
+    "small": dict(nstpx=100, nstpy=218, nf0=8, nf1=16, xh=6.54 / 810, yh=0.03, seed=20240611),
+}
+
+
+def table_dir(name="small", root=None):
+    """Directory holding the three table files for the named synthetic table (created on demand).
+
+    Uses $KQED_TABLE_DIR unchanged when it is set (e.g. the real form-factor table)."""
+    env = os.environ.get("KQED_TABLE_DIR")
+    if env and name == "env":
+        return env
+    root = root or os.environ.get("KQED_TABLE_CACHE", os.path.join(ROOT, "_tables"))
+    d = os.path.join(root, name)
+    ff = os.path.join(d, "FFxy_single_cksum.bin")
+    if not os.path.exists(ff):
+        os.makedirs(d, exist_ok=True)
+        p = TABLES[name]
+        tmp = ff + f".tmp{os.getpid()}"
+        rc = lib().kqed_write_synthetic_ff(tmp.encode(), p["nstpx"], p["nstpy"], p["nf0"], p["nf1"], p["xh"], p["yh"],
+                                           p["seed"])
+        if rc != 0:
+            raise RuntimeError("synthetic table generation failed")
+        os.replace(tmp, ff)
+    for f in ("taylorx_cksum.bin", "taylory_cksum.bin"):
+        dst = os.path.join(d, f)
+        if not os.path.exists(dst):
+            shutil.copyfile(os.path.join(GOLDEN_PRECOMP, f), dst)
+    return d
+
+
+# ----------------------------------------------------------------------------- kernels
+class QEDKernelTemps:
+    """struct QED_kernel_temps of the reference, owning device-resident tables."""
+
+    def __init__(self, t):
+        self._t = t
+
+    @classmethod
+    def initialise(cls, table_dir=None):
+        L = lib()
+        t = _Temps()
+        rc = L.kqed_initialise_dir(ctypes.byref(t), table_dir.encode()) if table_dir else L.initialise(ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError("initialise failed: " + L.kqed_last_error().decode())
+        return cls(t)
+
+    def free(self):
+        if self._t is not None and self._t.dev:
+            lib().free_QED_temps(ctypes.byref(self._t))
+        self._t = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    @property
+    def XX(self):
+        g = self._t.Grid
+        return np.ctypeslib.as_array(g.XX, (g.nstpx,)).copy()
+
+    @property
+    def YY(self):
+        g = self._t.Grid
+        return np.ctypeslib.as_array(g.YY, (g.nstpy,)).copy()
+
+    @property
+    def nfx(self):
+        g = self._t.Grid
+        return np.ctypeslib.as_array(g.nfx, (g.nstpx,)).copy()
+
+    @property
+    def G8(self):
+        return np.ctypeslib.as_array(self._t.G8, (65536,)).copy()
+
+    def _one(self, name, x, y, M=None):
+        x = np.ascontiguousarray(x, np.float64).reshape(4)
+        y = np.ascontiguousarray(y, np.float64).reshape(4)
+        k = np.zeros(384)
+        if M is None:
+            getattr(lib(), name)(x, y, self._t, k)
+        else:
+            lib().QED_Mkernel_L2(float(M), x, y, self._t, k)
+        return k.reshape(6, 4, 4, 4)
+
+    def QED_kernel_L0(self, x, y):
+        return self._one("QED_kernel_L0", x, y)
+
+    def QED_kernel_L1(self, x, y):
+        return self._one("QED_kernel_L1", x, y)
+
+    def QED_kernel_L2(self, x, y):
+        return self._one("QED_kernel_L2", x, y)
+
+    def QED_kernel_L3(self, x, y):
+        return self._one("QED_kernel_L3", x, y)
+
+    def QED_Mkernel_L2(self, M, x, y):
+        return self._one(None, x, y, M)
+
+    def batch_host(self, which, x, y, M=0.0, layout=KQED_AOS):
+        """n pairs from host arrays x[n,4], y[n,4] -> (n,384) (AoS) or (384,n) (SoA) float64."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        n = x.shape[0]
+        out = np.zeros((n, 384) if layout == KQED_AOS else (384, n))
+        rc = lib().kqed_batch_host(ctypes.byref(self._t), int(which), float(M), n, x.reshape(-1), y.reshape(-1),
+                                   out.reshape(-1), int(layout))
+        if rc != 0:
+            raise RuntimeError("kqed_batch_host failed: " + lib().kqed_last_error().decode())
+        return out
+
+    def profile(self, enable):
+        """Read-and-reset the per-stage kernel times (ms, launches) and switch timing on/off."""
+        ms = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_long * 3)()
+        lib().kqed_profile(ctypes.byref(self._t), int(bool(enable)), ms, cnt)
+        names = ("sort", "interp", "assemble")
+        return {n: (ms[i], cnt[i]) for i, n in enumerate(names)}
+
+    def batch_device(self, which, n, x_ptr, y_ptr, out_ptr, M=0.0, layout=KQED_AOS, stream=None):
+        """n pairs on device pointers (ints); enqueued on `stream` (an int hipStream_t or None)."""
+        rc = lib().kqed_batch(ctypes.byref(self._t), int(which), float(M), int(n), ctypes.c_void_p(x_ptr),
+                              ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr), int(layout),
+                              ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_batch failed: " + lib().kqed_last_error().decode())

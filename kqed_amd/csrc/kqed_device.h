@@ -1,0 +1,61 @@
+/*
+ * kqed_device.h -- device-side data layout shared by the HIP kernels and the host API.
+ *
+ * HBM layout of the tables (all built once by initialise()):
+ *   P   [14][nstpx][nstpy][ks] double2 : Clenshaw inputs of the packed (vector) interpolation path,
+ *                                        .x = value array, .y = d/dy array, per grid node
+ *   PNC [4][nstpx][nstpy][ks]  double  : d/dy arrays of the derivative-only variant for the tables
+ *                                        QG2, dxQG2, QG3, dxQG3
+ *   S   [14][nstpx][nstpy][ks] double2 : Clenshaw inputs of the scalar interpolation path (Taylor
+ *                                        region and grid edges), .x = value, .y = d/dy
+ *   LUT [ks][8] double                 : Clenshaw derivative ratios, slot 2d-1/2d for order d
+ * ks = nfmax + 2 (longest Clenshaw: nfx + mm, mm <= 2).
+ *
+ * Per-pair record written by the interpolation kernel and read by the assembly kernel:
+ *   rec[REC_LEN] doubles: x[4], y[4], coef[4], M, gaussX, gaussY, pad, main payload [36],
+ *   three Tabd payloads [20]
+ *   kinds (int): 3 bits per evaluation slot (KIND_*)
+ */
+#ifndef KQED_DEVICE_H
+#define KQED_DEVICE_H
+
+#include <hip/hip_runtime.h>
+
+#define KQED_REC_LEN 112
+#define KQED_REC_X 0      /* x[4] */
+#define KQED_REC_Y 4      /* y[4] */
+#define KQED_REC_COEF 8   /* coefficient of each evaluation slot [4] */
+#define KQED_REC_M 12     /* M of M-L2 */
+#define KQED_REC_GX 13    /* exp(-M x^2 / 2) */
+#define KQED_REC_GY 14    /* exp(-M y^2 / 2) */
+#define KQED_REC_MAIN 16  /* slot 0 payload [36] */
+#define KQED_REC_AUX 52   /* slots 1..3 payloads [20] each */
+#define KQED_AUX_LEN 20
+
+enum {
+  KIND_NONE = 0,
+  KIND_STD = 1,       /* standard / Taylor / swap path: S, T, V tensors (200 inputs) */
+  KIND_XEQ0 = 2,      /* L0(0, v): Tabd_xeq0 tensors (192 inputs) */
+  KIND_YEQ0 = 3,      /* L0(v, 0): Tabd_yeq0 tensors */
+  KIND_YEQ0_MULAM = 4,/* L0(v, v) = -L0_{mu<->lambda}(v, 0) */
+  KIND_XEQ0_M = 5,    /* M-L2 subtraction term built on L0(0, y) */
+  KIND_YEQ0_M = 6     /* M-L2 subtraction term built on L0(x, 0) */
+};
+
+struct kqed_dev_tables {
+  int nstpx, nstpy, nfmax, ks;
+  const double *XX, *YY;
+  const int *nfx;
+  const double *TX;     /* [23][810] */
+  const double *TY;     /* [14][100] */
+  const double *LUT;    /* [ks][8] */
+  const double2 *P;
+  const double *PNC;
+  const double2 *S;
+  /* assembly maps: 0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign */
+  const int *rowptr[3];
+  const short *col[3];
+  const double *val[3];
+};
+
+#endif

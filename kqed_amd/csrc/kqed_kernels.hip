@@ -1,0 +1,1272 @@
+/*
+ * kqed_kernels.hip -- HIP kernels (gfx950) for the position-space QED kernel L0..L3 / M-L2, FP64.
+ *
+ * Pipeline for a batch of (x,y) pairs (driven by kqed_api.hip):
+ *   k_classify   regime + interpolation-cell key per pair
+ *   k_count / k_scan / k_scatter   counting sort of the pairs by cell (table locality)
+ *   k_interp     one thread per pair, in cell order: invariants, weight-function interpolation
+ *                (Clenshaw sums over the precomputed per-node arrays), the chain-rule algebra of
+ *                chnr_dS/dT/dV and the Taylor-table lerps; writes a per-pair record
+ *   k_assemble   one block per group of pairs: S/T/V (or Taylor-table) tensors into LDS, then
+ *                one thread per output component applies the Dirac-trace assembly map and the
+ *                L1/L2/L3/M-L2 subtractions; coalesced stores
+ * Built with -ffp-contract=off: every fused multiply-add below is written explicitly, where the
+ * reference's AVX+FMA build fuses (simd.c:79-118, 424-435, 54-63; getff-new.c:69-90).
+ */
+#include <stdint.h>
+
+#include "kqed_device.h"
+
+#define DEV __device__ __forceinline__
+typedef kqed_dev_tables DT;
+
+/* form-factor tables (reference KQED.h:52-54) */
+enum { QG0, QG1, QG2, QG3, QL4, QL2, dxQG0, dxQG1, dxQG2, dxQG3, dxQL4, dxQL2, d2xQG2, d2xQG3 };
+/* Taylor-x rows (reference KQED.h:57) */
+enum { TYY = 0, G0dy, G0dx, Gl2, Gl2dy, Gl21, Gl21dy, Gl3, Gl3dy, G21, G21dy, G22A, G22B, G22Ady,
+       G22Bdy, G3A, G3B, G3Ady, G3Bdy, G31A, G31B, G31Ady, G31Bdy };
+/* Taylor-y rows (reference KQED.h:60) */
+enum { alpha0dx_0p = 0, alpha0_1p, alpha3_0p, beta2_1p, alpha3_1p, alpha1_0p, beta4_1p, alpha1_1p,
+       alpha1dx_1p, alpha3dx_0p, alpha3dxdx_0p, beta2dx_1p, alpha3dx_1p, alpha1dx_0p };
+
+#define NYTAY 810
+#define NXTAY 100
+
+/* ------------------------------------------------------------------ small helpers */
+DEV double lerp(double a, double t1, double t2) { return a * t1 + (1.0 - a) * t2; }
+DEV double scal(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3]; }
+
+/* y-power case of a table (reference getff-new.c:116-123): ypk 0: 1, 1: 1/y, 2: y, 3: 1/y^2 */
+DEV void ffcase(int nm, int &ypk, int &mshm)
+{
+  switch (nm) {
+  case QG2: case dxQG2: case d2xQG2: ypk = 1; mshm = 3; break;
+  case QL4: case dxQL4: ypk = 2; mshm = 1; break;
+  case QL2: case dxQL2: ypk = 3; mshm = 4; break;
+  default: ypk = 0; mshm = 2; break;
+  }
+}
+DEV int mm_of(int nm)
+{
+  int ypk, mshm;
+  ffcase(nm, ypk, mshm);
+  const int d = 2 - mshm;
+  return d < 0 ? -d : d;
+}
+
+/* bisection of reference getff-new.c:199-210 */
+DEV int bsrch(const double *__restrict__ arr, double target, int lo, int hi)
+{
+  while (hi - lo >= 2) {
+    const int mid = (hi + lo) >> 1;
+    if (arr[mid] > target) hi = mid; else lo = mid;
+  }
+  return lo;
+}
+
+struct Ip { double A, B, C1, C2, D, lA, y1, y2; int idx; };
+struct Inv {
+  double x, cb, y, xsq, ysq, xdoty, cborig, yorig, xmysq, xmy;
+  bool flag2;
+  Ip INVx, INVy;
+};
+
+/* precompute_INVx / precompute_INV (reference getff-new.c:30-56, simd.c:10-37) */
+DEV void precomp(Ip &P, double y, double y1, double y2, int idx)
+{
+  P.idx = idx;
+  const double dy = y1 - y2, ymy1 = y - y1, ymy2 = y - y2;
+  const double ym2sq = ymy2 * ymy2, ym1sq = ymy1 * ymy1;
+  P.A = -(ym2sq) * (2 * y - 3 * y1 + y2);
+  P.B = (ym1sq) * (2 * y + y1 - 3 * y2);
+  P.C1 = (ymy1) * (ym2sq) * dy;
+  P.C2 = (ym1sq) * (ymy2) * dy;
+  P.D = 1. / (dy * dy * dy);
+  P.lA = (y2 - y) / (y2 - y1);
+  P.y1 = y1;
+  P.y2 = y2;
+}
+
+/* set_invariants (reference getff-new.c:316-370) */
+DEV Inv set_inv(const DT &T, const double *xv, const double *yv)
+{
+  Inv I;
+  const double EPSIN = 1E-7;
+  I.xsq = scal(xv, xv); I.xsq = I.xsq < EPSIN ? EPSIN : I.xsq; I.x = sqrt(fabs(I.xsq));
+  I.ysq = scal(yv, yv); I.ysq = I.ysq < EPSIN ? EPSIN : I.ysq; I.y = sqrt(fabs(I.ysq));
+  I.xdoty = scal(xv, yv);
+  I.cb = I.xdoty / (I.x * I.y);
+  I.cborig = I.cb;
+  I.yorig = I.y;
+  I.xmysq = (I.xsq + I.ysq) * (1.00000000000001) - 2.0 * I.xdoty;
+  I.xmy = sqrt(fabs(I.xmysq));
+  I.flag2 = false;
+  const double rx = (I.x > I.xmy ? I.xmy / I.x : I.x / I.xmy);
+  const double rxy = (I.x > I.y ? I.y / I.x : I.x / I.y);
+  if (rx < rxy && I.xmy < T.YY[T.nstpy - 1] && fabs(I.xmysq) > 1E-28) {
+    I.cb = (I.x - I.y * I.cb) / I.xmy;
+    I.y = I.xmy;
+    I.flag2 = true;
+  }
+  const int ix1 = bsrch(T.XX, I.x, 0, T.nstpx);
+  const int ix2 = ix1 + 1 >= T.nstpx ? ix1 : ix1 + 1;
+  precomp(I.INVx, I.x, T.XX[ix1], T.XX[ix2], ix1);
+  const int iy1 = bsrch(T.YY, I.y, 0, T.nstpy);
+  const int iy2 = iy1 + 1 >= T.nstpy ? iy1 : iy1 + 1;
+  precomp(I.INVy, I.y, T.YY[iy1], T.YY[iy2], iy1);
+  return I;
+}
+
+DEV bool is_zero(const double *x) { return (x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3]) < 1E-28; }
+DEV bool x_is_y(const double *x, const double *y)
+{
+  const double d[4] = { x[0] - y[0], x[1] - y[1], x[2] - y[2], x[3] - y[3] };
+  return is_zero(d);
+}
+
+DEV size_t node(const DT &T, int nm, int ix, int iy) { return (((size_t)nm * T.nstpx + ix) * T.nstpy + iy) * T.ks; }
+
+/* ------------------------------------------------------------------ table precompute */
+/* One thread per (table, x node, y node): the per-node arrays the reference rebuilds for every
+ * pair in getff6/getff7 (simd.c:343-613, FMA build) and getff2 (getff-new.c:104-156). */
+__global__ void k_build_arrays(DT T, const float *__restrict__ Fm, const float *__restrict__ Fp,
+                               double2 *__restrict__ P, double *__restrict__ PNC, double2 *__restrict__ S)
+{
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nodes = (long)T.nstpx * T.nstpy;
+  if (id >= 14 * nodes) return;
+  const int nm = (int)(id / nodes);
+  const int ix = (int)((id % nodes) / T.nstpy);
+  const int iy = (int)(id % T.nstpy);
+  const double y = T.YY[iy];
+  const int nf = T.nfx[ix];
+  int ypk, mshm;
+  ffcase(nm, ypk, mshm);
+  int mshp = 2 - mshm;
+  const int mm = mshp < 0 ? -mshp : mshp;
+  const float *fm = Fm + (((size_t)nm * T.nstpx + ix) * T.nstpy + iy) * T.nfmax;
+  const float *fp = Fp + (((size_t)nm * T.nstpx + ix) * T.nstpy + iy) * T.nfmax;
+  double2 *p = P + node(T, nm, ix, iy);
+  double2 *s = S + node(T, nm, ix, iy);
+  for (int k = 0; k < T.ks; k++) { p[k] = make_double2(0.0, 0.0); s[k] = make_double2(0.0, 0.0); }
+  const double y1 = 1.0 / y;
+  const double yp = ypk == 0 ? 1.0 : ypk == 1 ? y1 : ypk == 2 ? y : y1 * y1;
+  /* packed path */
+  {
+    double facm = y1 * yp, facp = yp;
+    facm = facm * y1;
+    for (int j = 0; j < mm; j++) { facm = facm * y1; facp = facp * y; }
+    double MSHM = mshm + mm, MSHP = mshp + mm;
+    for (int j = 0; j < nf; j++) {
+      const double vm = facm * (double)fm[j], vp = facp * (double)fp[j];
+      double A = MSHM * vm;
+      const double val = vm + vp;
+      A = fma(MSHP, vp, -A);
+      facm = facm * y1;
+      facp = facp * y;
+      MSHM = MSHM + 1.0;
+      MSHP = MSHP + 1.0;
+      p[mm + j] = make_double2(val, y1 * A);
+    }
+  }
+  /* derivative-only variant (getff6 with calc = false, simd.c:452-475) */
+  const int nci = nm == QG2 ? 0 : nm == dxQG2 ? 1 : nm == QG3 ? 2 : nm == dxQG3 ? 3 : -1;
+  if (nci >= 0) {
+    double *q = PNC + (((size_t)nci * T.nstpx + ix) * T.nstpy + iy) * T.ks;
+    for (int k = 0; k < T.ks; k++) q[k] = 0.0;
+    double facm = y1 * yp, facp = yp;
+    facm = facm * y1;
+    for (int j = 0; j < mm; j++) { facm = facm * y1; facp = facp * y; }
+    double MSHM = mshm + mm, MSHP = mshp + mm;
+    facm = facm * y1;
+    facp = facp * y1;
+    for (int j = 0; j < nf; j++) {
+      const double vm = facm * (double)fm[j], vp = facp * (double)fp[j];
+      const double A = MSHM * vm;
+      facm = facm * y1;
+      facp = facp * y;
+      MSHM = MSHM + 1.0;
+      q[mm + j] = fma(MSHP, vp, -A);
+      MSHP = MSHP + 1.0;
+    }
+  }
+  /* scalar path */
+  {
+    double facm = y1 * y1 * yp, facp = yp;
+    for (int j = 0; j < mm; j++) { facm *= y1; facp *= y; }
+    int im = mshm + mm, ip = mshp + mm;
+    for (int j = 0; j < nf; j++) {
+      const double dv = y1 * ip * facp * (double)fp[j] - y1 * im * facm * (double)fm[j];
+      const double v = facm * (double)fm[j] + facp * (double)fp[j];
+      s[mm + j] = make_double2(v, dv);
+      facm *= y1; facp *= y; im++; ip++;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------ Clenshaw sums */
+/* Packed-path Clenshaw (reference simd.c:79-118, 182-336, FMA build) over one node's arrays for
+ * orders O0 .. O0+NO-1 of both the value (.x) and d/dy (.y) arrays in one pass.  Zero coefficients
+ * past a node's length leave every chain at exact zero, so each node runs at its own length. */
+template <int O0, int NO>
+DEV void clen_node(const double2 *__restrict__ co, int nk, double twox, const double *__restrict__ LUT,
+                   double res[NO][2])
+{
+  double ya[NO][2], yb[NO][2];
+#pragma unroll
+  for (int i = 0; i < NO; i++) { ya[i][0] = ya[i][1] = yb[i][0] = yb[i][1] = 0.0; }
+  for (int k = nk - 1; k >= O0; --k) {
+    const double2 c = co[k];
+    const double *L = LUT + 8 * k;
+#pragma unroll
+    for (int i = 0; i < NO; i++) {
+      const int d = O0 + i;
+      if (d > 0 && k < d) continue;
+      if (d == 0) {
+        double t0 = yb[i][0], t1 = yb[i][1];
+        ya[i][0] = ya[i][0] - c.x; ya[i][1] = ya[i][1] - c.y;
+        yb[i][0] = fma(twox, yb[i][0], -ya[i][0]);
+        yb[i][1] = fma(twox, yb[i][1], -ya[i][1]);
+        ya[i][0] = t0; ya[i][1] = t1;
+      } else {
+        const double PA = L[2 * d - 1], PB = L[2 * d];
+        double t0 = yb[i][0], t1 = yb[i][1];
+        ya[i][0] = fma(PB, ya[i][0], -c.x);
+        ya[i][1] = fma(PB, ya[i][1], -c.y);
+        yb[i][0] = yb[i][0] * twox;
+        yb[i][1] = yb[i][1] * twox;
+        yb[i][0] = fma(PA, yb[i][0], -ya[i][0]);
+        yb[i][1] = fma(PA, yb[i][1], -ya[i][1]);
+        ya[i][0] = t0; ya[i][1] = t1;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NO; i++) {
+    const int d = O0 + i;
+    const double f = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
+    res[i][0] = d == 0 ? yb[i][0] : yb[i][0] * f;
+    res[i][1] = d == 0 ? yb[i][1] : yb[i][1] * f;
+  }
+}
+
+/* single packed chain over a plain double array */
+DEV double clen_one(const double *__restrict__ co, int nk, int d, double twox, const double *__restrict__ LUT)
+{
+  double ya = 0.0, yb = 0.0, t;
+  if (d == 0) {
+    for (int k = nk - 1; k >= 0; --k) { t = yb; ya = ya - co[k]; yb = fma(twox, yb, -ya); ya = t; }
+    return yb;
+  }
+  for (int k = nk - 1; k >= d; --k) {
+    t = yb;
+    ya = fma(LUT[8 * k + 2 * d], ya, -co[k]);
+    yb = yb * twox;
+    yb = fma(LUT[8 * k + 2 * d - 1], yb, -ya);
+    ya = t;
+  }
+  return yb * (d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0);
+}
+
+/* scalar-path Clenshaw with divisions (reference cheby.c:13-91); comp selects .x or .y */
+DEV double cheb_s(int order, int nk, double x, const double2 *__restrict__ co, int comp)
+{
+  double ya = 0.0, yb = 0.0, ytmp;
+  const double twox = 2.0 * x;
+  int j;
+#define C(j) (comp ? co[j].y : co[j].x)
+  switch (order) {
+  case 0:
+    for (j = nk - 1; j > 0; j--) { ytmp = yb; yb = twox * yb - ya + C(j); ya = ytmp; }
+    return -ya + twox * yb + C(0);
+  case 1:
+    for (j = nk - 1; j > 0; j--) { ytmp = yb; yb = twox * (j + 1) * yb / j - (j + 3) * ya / (j + 1) + C(j); ya = ytmp; }
+    return 2.0 * yb;
+  case 2:
+    for (j = nk - 1; j > 1; j--) {
+      ytmp = yb; yb = twox * (j + 1) * yb / (j - 1); yb -= (j + 4) * ya / j; yb += C(j); ya = ytmp;
+    }
+    return 8 * yb;
+  default:
+    for (j = nk - 1; j > 2; j--) { ytmp = yb; yb = twox * (j + 1) * yb / (j - 2) - (j + 5) * ya / (j - 1) + C(j); ya = ytmp; }
+    return 48 * yb;
+  }
+#undef C
+}
+
+/* ------------------------------------------------------------------ scalar interpolation path */
+DEV double interpol3(double y, double y1, double y2, double dy, double f1, double f2, double g1, double g2)
+{
+  return ((-f1 * (y - y2) * (y - y2) * (2 * y - 3 * y1 + y2) +
+           (y - y1) * (f2 * (y - y1) * (2 * y + y1 - 3 * y2) + (y - y2) * dy * (g1 * y + g2 * y - g2 * y1 - g1 * y2))) /
+          (dy * dy * dy));
+}
+
+/* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays */
+DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm, bool ndy, int ndcb,
+                     double cb, double y)
+{
+  const int mm = mm_of(nm);
+  const int nk = T.nfx[ix] + mm;
+  const int ord = mm + ndcb;
+  const double2 *a = T.S + node(T, nm, ix, iy);
+  const double r11 = cheb_s(ord, nk, cb, a, 1);
+  const double r10 = ndy ? r11 : cheb_s(ord, nk, cb, a, 0);
+  if (!flag_hy) {
+    const double y1 = T.YY[iy], y2 = T.YY[iy + 1];
+    const double2 *b = T.S + node(T, nm, ix, iy + 1);
+    const double r21 = cheb_s(ord, nk, cb, b, 1);
+    const double r20 = ndy ? r21 : cheb_s(ord, nk, cb, b, 0);
+    if (use_y_derivs) return interpol3(y, y1, y2, y1 - y2, r10, r20, r11, r21);
+    return lerp((y2 - y) / (y2 - y1), r10, r20);
+  }
+  return r10;
+}
+
+DEV double interpol4(const Ip &P, double f1, double f2, double g1, double g2)
+{
+  return (f1 * P.A + f2 * P.B + g1 * P.C1 + g2 * P.C2) * P.D;
+}
+
+/* extractff on the scalar path (reference getff-new.c:236-257), used when a grid-edge flag is set */
+DEV double extractff_s(const DT &T, int nm, bool ndy, int ndcb, const Inv &I)
+{
+  const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
+  const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
+  const bool use_x_derivs = (nm < dxQG0 || nm == dxQG2 || nm == dxQG3);
+  const int ix = I.INVx.idx, iy = I.INVy.idx;
+  const bool use_y_derivs = !ndy;
+  const double f1iy = accessv_s(T, flag_hy, use_y_derivs, ix, iy, nm, ndy, ndcb, I.cb, I.y);
+  if (!flag_hx) {
+    const double f2iy = accessv_s(T, flag_hy, use_y_derivs, ix + 1, iy, nm, ndy, ndcb, I.cb, I.y);
+    if (use_x_derivs) {
+      const int offset = nm < dxQG0 ? dxQG0 : QL4;
+      const double g1iy = accessv_s(T, flag_hy, use_y_derivs, ix, iy, nm + offset, ndy, ndcb, I.cb, I.y);
+      const double g2iy = accessv_s(T, flag_hy, use_y_derivs, ix + 1, iy, nm + offset, ndy, ndcb, I.cb, I.y);
+      return interpol4(I.INVx, f1iy, f2iy, g1iy, g2iy);
+    }
+    return lerp(I.INVx.lA, f1iy, f2iy);
+  }
+  return f1iy;
+}
+
+/* extractff2 on the scalar path (reference getff-new.c:278-292) */
+DEV void extractff2_s(const DT &T, int nm, int ndcb, const Inv &I, double F[4])
+{
+  const int ndcb2 = ndcb + 1 < 5 ? ndcb + 1 : ndcb;
+  const int dxmap[14] = { dxQG0, dxQG1, dxQG2, dxQG3, dxQL4, dxQL2, dxQG0, dxQG1, d2xQG2, d2xQG3, dxQL4, dxQL2, d2xQG2, d2xQG3 };
+  F[0] = extractff_s(T, nm, false, ndcb2, I);
+  F[1] = extractff_s(T, dxmap[nm], false, ndcb, I);
+  F[2] = extractff_s(T, nm, true, ndcb, I);
+  F[3] = extractff_s(T, nm, false, ndcb, I);
+}
+
+/* ------------------------------------------------------------------ packed interpolation path */
+/* y step of one x node: INBLOCK (simd.c:54-63) and the d/dy lerp (simd.c:637-645) */
+DEV double hy(const Ip &P, double v1, double v2, double d1, double d2)
+{
+  double t = v1 * P.A;
+  t = fma(v2, P.B, t);
+  t = fma(d1, P.C1, t);
+  t = fma(d2, P.C2, t);
+  return t * P.D;
+}
+DEV double ly(const Ip &P, double d1, double d2) { return fma(d1 - d2, P.lA, d2); }
+
+/* interpol5 (getff-new.c:69-90): f = {H(nm,ix), H(nm,ix+1), H(dm,ix), H(dm,ix+1), L(nm,ix), L(nm,ix+1),
+ * L(dm,ix), L(dm,ix+1), H'(nm,ix), H'(nm,ix+1), H'(dm,ix), H'(dm,ix+1)}, H' at order d+1. */
+DEV void interpol5(double F[4], const Ip &P, const double f[12])
+{
+  double t;
+  t = f[8] * P.A; t = fma(f[9], P.B, t); t = fma(f[10], P.C1, t); t = fma(f[11], P.C2, t); F[0] = t * P.D;
+  const double ia = 1.0 / P.D, ib = P.lA / P.D, ic = -P.lA / P.D;
+  t = f[3] * ia; t = fma(f[2], ib, t); t = fma(f[3], ic, t); t = t + 0.0; F[1] = t * P.D;
+  t = f[4] * P.A; t = fma(f[5], P.B, t); t = fma(f[6], P.C1, t); t = fma(f[7], P.C2, t); F[2] = t * P.D;
+  t = f[0] * P.A; t = fma(f[1], P.B, t); t = fma(f[2], P.C1, t); t = fma(f[3], P.C2, t); F[3] = t * P.D;
+}
+
+/* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of one table at NO orders. */
+template <int O0, int NO>
+DEV void table_xnode(const DT &T, int nm, int ixn, const Inv &I, double twox, double H[NO], double Lv[NO])
+{
+  const int mm = mm_of(nm);
+  const int nk = T.nfx[ixn] + mm;
+  double r1[NO][2], r2[NO][2];
+  clen_node<O0, NO>(T.P + node(T, nm, ixn, I.INVy.idx), nk, twox, T.LUT, r1);
+  clen_node<O0, NO>(T.P + node(T, nm, ixn, I.INVy.idx + 1), nk, twox, T.LUT, r2);
+#pragma unroll
+  for (int i = 0; i < NO; i++) {
+    H[i] = hy(I.INVy, r1[i][0], r2[i][0], r1[i][1], r2[i][1]);
+    Lv[i] = ly(I.INVy, r1[i][1], r2[i][1]);
+  }
+}
+
+/* derivative-only variant (accessv6 with ndy, simd.c:616-650) + x step: extractff(nm, ndy=true) */
+DEV double ndy_packed(const DT &T, int nci_nm, int nci_dm, int nm, const Inv &I, double twox, int d)
+{
+  double f[4];
+  for (int xn = 0; xn < 2; xn++) {
+    const int ixn = I.INVx.idx + xn;
+    const int nk = T.nfx[ixn] + mm_of(nm);
+    for (int t = 0; t < 2; t++) {
+      const int nci = t == 0 ? nci_nm : nci_dm;
+      const double *q1 = T.PNC + (((size_t)nci * T.nstpx + ixn) * T.nstpy + I.INVy.idx) * T.ks;
+      const double *q2 = q1 + T.ks;
+      const double a = clen_one(q1, nk, d, twox, T.LUT), b = clen_one(q2, nk, d, twox, T.LUT);
+      f[2 * t + xn] = ly(I.INVy, a, b);
+    }
+  }
+  /* QG2 and QG3 take the Hermite x step (use_x_derivs) */
+  return interpol4(I.INVx, f[0], f[1], f[2], f[3]);
+}
+
+/* Two-table request (nm, dm) at orders d, d+1 assembled from per-x-node H/L values. */
+DEV void request(const Ip &INVx, const double Hn[2][2], const double Ln[2], const double Hd[2][2], const double Ld[2],
+                 double F[4])
+{
+  /* H?[xn][order offset 0/1], L?[xn] at order d */
+  const double f[12] = { Hn[0][0], Hn[1][0], Hd[0][0], Hd[1][0], Ln[0], Ln[1], Ld[0], Ld[1],
+                         Hn[0][1], Hn[1][1], Hd[0][1], Hd[1][1] };
+  interpol5(F, INVx, f);
+}
+
+/* Simple groups: one request (nm, nm+6) at orders d, d+1 (d+1 < 4 always here). */
+template <int D>
+DEV void simple_group(const DT &T, int nm, const Inv &I, double twox, double F[4])
+{
+  double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+  for (int xn = 0; xn < 2; xn++) {
+    double H[2], L[2];
+    table_xnode<D, 2>(T, nm, I.INVx.idx + xn, I, twox, H, L);
+    Hn[xn][0] = H[0]; Hn[xn][1] = H[1]; Ln[xn] = L[0];
+    table_xnode<D, 2>(T, nm + 6, I.INVx.idx + xn, I, twox, H, L);
+    Hd[xn][0] = H[0]; Hd[xn][1] = H[1]; Ld[xn] = L[0];
+  }
+  request(I.INVx, Hn, Ln, Hd, Ld, F);
+}
+
+/* ------------------------------------------------------------------ chain-rule algebra */
+struct Tt {
+  double ell1, ell2, ell3, v1, ell4, dv1dx, dv1dy, dv1dcb, dell2dx, dell2dy, dell2dcb;
+  double dell1dx, dell1dy, dell1dcb, dell3dx, dell3dy, dell3dcb, dell4dy, dell4dx, dell4dcb;
+};
+struct Vt {
+  double g2, dg1dx, dg1dy, dg1dcb, dg2dx, dg2dy, dg2dcb, dg3dx, dg3dy, dg3dcb;
+  double ddg2dxdx, ddg2dxdy, ddg2dxdcb, ddg2dydcb, ddg2dcbdcb;
+  double ddg3dxdx, ddg3dxdy, ddg3dxdcb, ddg3dydcb, ddg3dcbdcb;
+  double ddg1dxdx, ddg1dxdy, ddg1dxdcb, ddg1dydcb, ddg1dcbdcb;
+  double ddg1dydy, ddg2dydy;
+};
+
+/* chnr_dT standard branch algebra (reference chnr_dT.c:222-251) */
+DEV void T_algebra(Tt &T, const Inv &I, double ysq)
+{
+  const double x = I.x, y = I.y, cb = I.cb, xsq = I.xsq;
+  T.ell3 = T.ell4 / (2.0 * xsq * ysq) - y * cb * T.ell2 / x;
+  T.ell1 = 4.0 / (3.0 * xsq * xsq) * (T.v1 - xsq * ysq * (cb * cb - 0.25) * T.ell2 - 1.5 * xsq * x * y * cb * T.ell3);
+  T.dell3dy = (-T.ell4 / (ysq * y) + T.dell4dy / (2.0 * ysq) - cb * x * T.ell2 - y * x * cb * T.dell2dy) / (xsq);
+  T.dell3dcb = (T.dell4dcb / (2.0 * x * ysq) - y * T.ell2 - y * cb * T.dell2dcb) / x;
+  T.dell3dx = (-T.ell4 / (x * ysq) + T.dell4dx / (2.0 * ysq) + y * cb * T.ell2 - y * x * cb * T.dell2dx) / xsq;
+  T.dell1dy = 4.0 / (3.0 * xsq * xsq) * (T.dv1dy - 2.0 * xsq * y * (cb * cb - 0.25) * T.ell2
+                                         - xsq * ysq * (cb * cb - 0.25) * T.dell2dy
+                                         - 1.5 * xsq * x * cb * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dy);
+  T.dell1dcb = 4.0 / (3.0 * xsq * xsq) * (T.dv1dcb - 2.0 * xsq * ysq * cb * T.ell2
+                                          - xsq * ysq * (cb * cb - 0.25) * T.dell2dcb
+                                          - 1.5 * xsq * x * y * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dcb);
+  T.dell1dx = -4.0 * T.ell1 / x
+    + 4.0 / (3.0 * xsq * xsq) * (T.dv1dx - 2.0 * x * ysq * (cb * cb - 0.25) * T.ell2
+                                 - xsq * ysq * (cb * cb - 0.25) * T.dell2dx
+                                 - 4.5 * xsq * y * cb * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dx);
+}
+
+/* chnr_dV standard branch algebra (reference chnr_dV.c:413-442) */
+DEV void V_algebra(Vt &D, const Inv &I)
+{
+  const double x = I.x, y = I.y, cb = I.cb, xsq = I.xsq;
+  D.dg1dx = D.dg3dx + y / xsq * cb * D.g2 - y / x * cb * D.dg2dx;
+  D.dg1dy = D.dg3dy - cb / x * D.g2 - (y / x) * cb * D.dg2dy;
+  D.dg1dcb = D.dg3dcb - y / x * D.g2 - y / x * cb * D.dg2dcb;
+  D.ddg1dxdx = D.ddg3dxdx - 2.0 * y / (xsq * x) * cb * D.g2 + y / xsq * cb * D.dg2dx + y / xsq * cb * D.dg2dx
+    - y / x * cb * D.ddg2dxdx;
+  D.ddg1dxdy = D.ddg3dxdy + cb / xsq * D.g2 + y * cb / xsq * D.dg2dy - cb / x * D.dg2dx - y / x * cb * D.ddg2dxdy;
+  D.ddg1dxdcb = D.ddg3dxdcb + y / xsq * D.g2 + y / xsq * cb * D.dg2dcb - y / x * D.dg2dx - y / x * cb * D.ddg2dxdcb;
+  D.ddg1dydcb = D.ddg3dydcb - D.g2 / x - cb / x * D.dg2dcb - y / x * D.dg2dy - y / x * cb * D.ddg2dydcb;
+  D.ddg1dcbdcb = D.ddg3dcbdcb - 2.0 * y / x * D.dg2dcb - y / x * cb * D.ddg2dcbdcb;
+}
+
+/* Taylor-x branch of chnr_dS (reference chnr_dS.c:10-45) */
+DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, const Inv &I)
+{
+  const int iy_tay = bsrch(T.TX + TYY * NYTAY, I.y, 0, NYTAY);
+  const int iy = bsrch(T.YY, I.y, 0, T.nstpy);
+  if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
+  const int i2 = iy_tay + 1;
+  const double *TX = T.TX;
+#define TXr(r, i) TX[(r) * NYTAY + (i)]
+  const double ay = (TXr(TYY, i2) - I.y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
+  const double ax = (T.XX[0] - I.x) / (T.XX[0]);
+  double f1 = I.cb * lerp(ay, TXr(G0dx, iy_tay), TXr(G0dx, i2));
+  double f2 = accessv_s(T, false, true, 0, iy, dxQG0, false, 0, I.cb, I.y);
+  dg0dx = lerp(ax, f1, f2);
+  f1 = lerp(ay, TXr(G0dy, iy_tay), TXr(G0dy, i2));
+  f2 = accessv_s(T, false, false, 0, iy, QG0, true, 0, I.cb, I.y);
+  dg0dy = lerp(ax, f1, f2);
+  f2 = accessv_s(T, false, true, 0, iy, QG0, false, 1, I.cb, I.y);
+  dg0dcb = lerp(ax, 0.0, f2);
+  return 0;
+}
+
+/* Taylor-x branch of chnr_dT (reference chnr_dT.c:15-143) */
+DEV int T_taylor(const DT &T_, Tt &T, const Inv &I)
+{
+  const double ysq = I.flag2 ? I.xmysq : I.ysq;
+  const int iy_tay = bsrch(T_.TX + TYY * NYTAY, I.y, 0, NYTAY);
+  const int iy = bsrch(T_.YY, I.y, 0, T_.nstpy);
+  if (iy_tay == NYTAY - 1 || iy == T_.nstpy - 1) return 1;
+  const int i2 = iy_tay + 1;
+  const double *TX = T_.TX;
+  const double ay = (TXr(TYY, i2) - I.y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
+  const double ax = (T_.XX[0] - I.x) / (T_.XX[0]);
+  const double xb = T_.XX[0], xbsq = xb * xb, cb = I.cb, y = I.y;
+  const double ell2a = lerp(ay, TXr(Gl2, iy_tay), TXr(Gl2, i2));
+  const double ell2b = accessv_s(T_, false, true, 0, iy, QL2, false, 0, cb, y);
+  T.ell2 = lerp(ax, ell2a, ell2b);
+  const double dell2adx = lerp(ay, TXr(Gl21, iy_tay), TXr(Gl21, i2)) * cb;
+  const double dell2bdx = accessv_s(T_, false, true, 0, iy, dxQL2, false, 0, cb, y);
+  T.dell2dx = lerp(ax, dell2adx, dell2bdx);
+  const double dell2ady = lerp(ay, TXr(Gl2dy, iy_tay), TXr(Gl2dy, i2));
+  const double dell2bdy = accessv_s(T_, false, false, 0, iy, QL2, true, 0, cb, y);
+  T.dell2dy = lerp(ax, dell2ady, dell2bdy);
+  const double dell2bdcb = accessv_s(T_, false, true, 0, iy, QL2, false, 1, cb, y);
+  T.dell2dcb = lerp(ax, 0.0, dell2bdcb);
+  const double v1b = accessv_s(T_, false, true, 0, iy, QG1, false, 0, cb, y);
+  const double dv1bdx = accessv_s(T_, false, true, 0, iy, dxQG1, false, 0, cb, y);
+  const double dv1bdy = accessv_s(T_, false, false, 0, iy, QG1, true, 0, cb, y);
+  const double dv1bdcb = accessv_s(T_, false, true, 0, iy, QG1, false, 1, cb, y);
+  const double ell4b = accessv_s(T_, false, true, 0, iy, QL4, false, 0, cb, y);
+  const double dell4bdx = accessv_s(T_, false, true, 0, iy, dxQL4, false, 0, cb, y);
+  const double dell4bdy = accessv_s(T_, false, false, 0, iy, QL4, true, 0, cb, y);
+  const double dell4bdcb = accessv_s(T_, false, true, 0, iy, QL4, false, 1, cb, y);
+  const double ell3a = lerp(ay, TXr(Gl3, iy_tay), TXr(Gl3, i2));
+  const double ell3b = ell4b / (2.0 * xbsq * ysq) - y * cb * ell2b / xb;
+  T.ell3 = lerp(ax, ell3a, ell3b);
+  const double ell1b = 4.0 / (3.0 * xbsq * xbsq) * (v1b - xbsq * ysq * (cb * cb - 0.25) * ell2b - 1.5 * xbsq * xb * y * cb * ell3b);
+  T.ell1 = ell1b;
+  const double dell3ady = lerp(ay, TXr(Gl3dy, iy_tay), TXr(Gl3dy, i2));
+  const double dell3bdy = (-ell4b / (ysq * y) + dell4bdy / (2.0 * ysq) - cb * xb * ell2b - y * xb * cb * dell2bdy) / (xbsq);
+  T.dell3dy = lerp(ax, dell3ady, dell3bdy);
+  const double dell3bdcb = (dell4bdcb / (2.0 * xb * ysq) - y * ell2b - y * cb * dell2bdcb) / xb;
+  T.dell3dcb = lerp(ax, 0.0, dell3bdcb);
+  const double dell3bdx = (-ell4b / (xb * ysq) + dell4bdx / (2.0 * ysq) + y * cb * ell2b - y * xb * cb * dell2bdx) / xbsq;
+  T.dell3dx = dell3bdx;
+  T.dell1dy = 4.0 / (3.0 * xbsq * xbsq) * (dv1bdy - 2.0 * xbsq * y * (cb * cb - 0.25) * ell2b
+                                           - xbsq * ysq * (cb * cb - 0.25) * dell2bdy
+                                           - 1.5 * xbsq * xb * cb * ell3b - 1.5 * xbsq * xb * y * cb * dell3bdy);
+  T.dell1dcb = 4.0 / (3.0 * xbsq * xbsq) * (dv1bdcb - 2.0 * xbsq * ysq * cb * ell2b - xbsq * ysq * (cb * cb - 0.25) * dell2bdcb
+                                            - 1.5 * xbsq * xb * y * ell3b - 1.5 * xbsq * xb * y * cb * dell3bdcb);
+  T.dell1dx = -4.0 * ell1b / xb + 4.0 / (3.0 * xbsq * xbsq)
+    * (dv1bdx - 2.0 * xb * ysq * (cb * cb - 0.25) * ell2b - xbsq * ysq * (cb * cb - 0.25) * dell2bdx
+       - 4.5 * xbsq * y * cb * ell3b - 1.5 * xbsq * xb * y * cb * dell3bdx);
+  return 0;
+}
+
+/* Taylor-x branch of chnr_dV (reference chnr_dV.c:18-174) */
+DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2, double cb)
+{
+  const int iy_tay = bsrch(T.TX + TYY * NYTAY, y, 0, NYTAY);
+  const int iy = bsrch(T.YY, y, 0, T.nstpy);
+  if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
+  const int i2 = iy_tay + 1;
+  const double *TX = T.TX;
+  const double ay = (TXr(TYY, i2) - y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
+  const double ax = (T.XX[0] - x) / (T.XX[0]);
+  const double xb = T.XX[0], xbsq = xb * xb;
+  const double g2b = accessv_s(T, false, true, 0, iy, QG2, false, 0, cb, y);
+  D.g2 = lerp(ax, 0.0, g2b);
+  const double ddg2adxdcb = lerp(ay, TXr(G21, iy_tay), TXr(G21, i2));
+  const double ddg2bdxdcb = accessv_s(T, false, true, 0, iy, dxQG2, false, 1, cb, y);
+  D.ddg2dxdcb = lerp(ax, ddg2adxdcb, ddg2bdxdcb);
+  const double dg2bdx = accessv_s(T, false, true, 0, iy, dxQG2, false, 0, cb, y);
+  D.dg2dx = D.ddg2dxdcb * cb;
+  const double dg2bdcb = accessv_s(T, false, true, 0, iy, QG2, false, 1, cb, y);
+  D.dg2dcb = D.ddg2dxdcb * x;
+  const double dg2bdy = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, y);
+  D.dg2dy = lerp(ax, 0.0, dg2bdy);
+  const double ddg2adxdy = cb * lerp(ay, TXr(G21dy, iy_tay), TXr(G21dy, i2));
+  const double ddg2bdxdy = accessv_s(T, false, false, 0, iy, dxQG2, true, 0, cb, y);
+  D.ddg2dxdy = lerp(ax, ddg2adxdy, ddg2bdxdy);
+  const double ddg2adxdx = 2.0 * (lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2))
+                                  + lerp(ay, TXr(G22B, iy_tay), TXr(G22B, i2)) * (0.4 + 0.6 * (2.0 * cb * cb - 1)));
+  const double ddg2bdxdx = accessv_s(T, false, true, 0, iy, d2xQG2, false, 0, cb, y);
+  D.ddg2dxdx = lerp(ax, ddg2adxdx, ddg2bdxdx);
+  const double ddg2bdcbdcb = accessv_s(T, false, true, 0, iy, QG2, false, 2, cb, y);
+  D.ddg2dcbdcb = ddg2bdcbdcb * xsq / xbsq;
+  const double ddg2bdydcb = accessv_s(T, false, false, 0, iy, QG2, true, 1, cb, y);
+  D.ddg2dydcb = lerp(ax, 0.0, ddg2bdydcb);
+  const double dg3bdy = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, y);
+  const double ddg3bdxdy = accessv_s(T, false, false, 0, iy, dxQG3, true, 0, cb, y);
+  const double ddg3bdxdcb = accessv_s(T, false, true, 0, iy, dxQG3, false, 1, cb, y);
+  const double ddg3bdxdx = accessv_s(T, false, true, 0, iy, d2xQG3, false, 0, cb, y);
+  const double ddg3bdcbdcb = accessv_s(T, false, true, 0, iy, QG3, false, 2, cb, y);
+  const double ddg3bdydcb = accessv_s(T, false, false, 0, iy, QG3, true, 1, cb, y);
+  const double dg1bdy = dg3bdy - cb / xb * g2b - (y / xb) * cb * dg2bdy;
+  const double ddg1bdxdx = ddg3bdxdx - 2.0 * y / (xbsq * xb) * cb * g2b + y / xbsq * cb * dg2bdx + y / xbsq * cb * dg2bdx
+    - y / xb * cb * ddg2bdxdx;
+  const double ddg1bdxdy = ddg3bdxdy + cb / xbsq * g2b + y * cb / xbsq * dg2bdy - cb / xb * dg2bdx - y / xb * cb * ddg2bdxdy;
+  const double ddg1bdxdcb = ddg3bdxdcb + y / xbsq * g2b + y / xbsq * cb * dg2bdcb - y / xb * dg2bdx - y / xb * cb * ddg2bdxdcb;
+  const double ddg1bdydcb = ddg3bdydcb - g2b / xb - cb / xb * dg2bdcb - y / xb * dg2bdy - y / xb * cb * ddg2bdydcb;
+  const double ddg1bdcbdcb = ddg3bdcbdcb - 2.0 * y / xb * dg2bdcb - y / xb * cb * ddg2bdcbdcb;
+  const double dg1ady = lerp(ay, TXr(G3Ady, iy_tay), TXr(G3Ady, i2)) - lerp(ay, TXr(G3Bdy, iy_tay), TXr(G3Bdy, i2));
+  const double ddg1adxdy = cb * (lerp(ay, TXr(G31Ady, iy_tay), TXr(G31Ady, i2))
+                                 - (2 / 3.) * lerp(ay, TXr(G31Bdy, iy_tay), TXr(G31Bdy, i2))
+                                 - lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2))
+                                 - y * lerp(ay, TXr(G22Ady, iy_tay), TXr(G22Ady, i2)));
+  const double ddg1adxdcb = lerp(ay, TXr(G31A, iy_tay), TXr(G31A, i2)) - (2 / 3.) * lerp(ay, TXr(G31B, iy_tay), TXr(G31B, i2))
+    - y * lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2));
+  const double ddg1adxdx = ddg1bdxdx;
+  D.ddg1dxdcb = lerp(ax, ddg1adxdcb, ddg1bdxdcb);
+  D.dg1dx = D.ddg1dxdcb * cb;
+  D.dg1dcb = D.ddg1dxdcb * x;
+  D.dg1dy = lerp(ax, dg1ady, dg1bdy);
+  D.ddg1dxdy = lerp(ax, ddg1adxdy, ddg1bdxdy);
+  D.ddg1dxdx = lerp(ax, ddg1adxdx, ddg1bdxdx);
+  D.ddg1dcbdcb = ddg1bdcbdcb * xsq / xbsq;
+  D.ddg1dydcb = lerp(ax, 0.0, ddg1bdydcb);
+  if (flag2) {
+    const double ya = T.YY[iy], yb = T.YY[iy + 1];
+    const double dg2bdy_ya = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, ya);
+    const double dg2bdy_yb = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, yb);
+    const double ddg2bdydy = (dg2bdy_yb - dg2bdy_ya) / (T.YY[1] - T.YY[0]);
+    D.ddg2dydy = ax * 0.0 + (1.0 - ax) * ddg2bdydy;
+    const double ddg1adydy = (TXr(G3Ady, i2) - TXr(G3Bdy, i2) - (TXr(G3Ady, iy_tay) - TXr(G3Bdy, iy_tay)))
+      / (TXr(TYY, i2) - TXr(TYY, iy_tay));
+    const double dg3bdy_ya = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, ya);
+    const double dg3bdy_yb = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, yb);
+    const double ddg3bdydy = (dg3bdy_yb - dg3bdy_ya) / (T.YY[1] - T.YY[0]);
+    const double ddg1bdydy = ddg3bdydy - (y * ddg2bdydy + 2.0 * dg2bdy) * cb / x;
+    D.ddg1dydy = ax * ddg1adydy + (1.0 - ax) * ddg1bdydy;
+  }
+  return 0;
+}
+
+/* x <-> x-y swap chain rules (reference chnr_dS.c:74-83, chnr_dT.c:146-180, chnr_dV.c:195-275) */
+DEV void swap_S(double &dg0dx, double &dg0dcb, double &dg0dy, const Inv &I)
+{
+  const double dgs0dx = dg0dx, dgs0dcb = dg0dcb, dgs0dy = dg0dy;
+  const double ca = I.cb;
+  const double cd = (I.yorig - I.x * I.cborig) / I.xmy;
+  dg0dx = dgs0dx + (1.0 - ca * ca) / I.xmy * dgs0dcb + ca * dgs0dy;
+  dg0dcb = -(I.ysq * cd * dgs0dcb + I.x * I.yorig * I.xmy * dgs0dy) / I.xmysq;
+  dg0dy = -(I.cborig + ca * cd) / I.xmy * dgs0dcb + cd * dgs0dy;
+}
+
+DEV void swap_T(Tt &T, const Inv &I)
+{
+  const double ell2s = T.ell2, dells2dx = T.dell2dx, dells2dcb = T.dell2dcb, dells2dy = T.dell2dy;
+  const double cd = (I.yorig - I.x * I.cborig) / I.xmy, cb = I.cb;
+  T.ell2 = ell2s;
+  T.dell2dx = dells2dx + (1.0 - cb * cb) / I.xmy * dells2dcb + cb * dells2dy;
+  T.dell2dcb = -(I.ysq * cd * dells2dcb + I.x * I.yorig * I.xmy * dells2dy) / I.xmysq;
+  T.dell2dy = -(I.cborig + cb * cd) / I.xmy * dells2dcb + cd * dells2dy;
+  const double ell3s = -T.ell3, dells3dx = -T.dell3dx, dells3dcb = -T.dell3dcb, dells3dy = -T.dell3dy;
+  T.ell3 = ell3s - T.ell2;
+  T.dell3dx = dells3dx + (1.0 - cb * cb) / I.xmy * dells3dcb + cb * dells3dy - T.dell2dx;
+  T.dell3dcb = -(I.ysq * cd * dells3dcb + I.x * I.yorig * I.xmy * dells3dy) / I.xmysq - T.dell2dcb;
+  T.dell3dy = -(I.cborig + cb * cd) / I.xmy * dells3dcb + cd * dells3dy - T.dell2dy;
+  const double ell1s = T.ell1, dells1dx = T.dell1dx, dells1dcb = T.dell1dcb, dells1dy = T.dell1dy;
+  T.ell1 = ell1s - T.ell2 - 2.0 * T.ell3;
+  T.dell1dx = dells1dx + (1.0 - cb * cb) / I.xmy * dells1dcb + cb * dells1dy - T.dell2dx - 2.0 * T.dell3dx;
+  T.dell1dcb = -(I.ysq * cd * dells1dcb + I.x * I.yorig * I.xmy * dells1dy) / I.xmysq - T.dell2dcb - 2.0 * T.dell3dcb;
+  T.dell1dy = -(I.cborig + cb * cd) / I.xmy * dells1dcb + cd * dells1dy - T.dell2dy - 2.0 * T.dell3dy;
+}
+
+DEV void swap_V(Vt &D, double x, double xmy, double y, double xsq, double xmysq, double ysq, double ca, double cborig)
+{
+  Vt F = D;
+  F.g2 = -D.g2; F.dg2dx = -D.dg2dx; F.dg2dy = -D.dg2dy; F.dg2dcb = -D.dg2dcb;
+  F.ddg2dxdx = -D.ddg2dxdx; F.ddg2dxdy = -D.ddg2dxdy; F.ddg2dydy = -D.ddg2dydy;
+  F.ddg2dxdcb = -D.ddg2dxdcb; F.ddg2dydcb = -D.ddg2dydcb; F.ddg2dcbdcb = -D.ddg2dcbdcb;
+  const double cd = (y - x * cborig) / xmy;
+  D.g2 = F.g2;
+  D.dg2dx = F.dg2dx + (1.0 - ca * ca) / xmy * F.dg2dcb + ca * F.dg2dy;
+  D.dg2dcb = -(ysq * cd * F.dg2dcb + x * y * xmy * F.dg2dy) / xmysq;
+  D.dg2dy = -(cborig + ca * cd) / xmy * F.dg2dcb + cd * F.dg2dy;
+  D.dg1dx = F.dg1dx + (1.0 - ca * ca) / xmy * F.dg1dcb + ca * F.dg1dy - D.dg2dx;
+  D.dg1dcb = -(ysq * cd * F.dg1dcb + x * y * xmy * F.dg1dy) / xmysq - D.dg2dcb;
+  D.dg1dy = -(cborig + ca * cd) / xmy * F.dg1dcb + cd * F.dg1dy - D.dg2dy;
+  D.ddg2dxdx = (F.ddg2dxdx + (1.0 - ca * ca) / xmy * F.ddg2dxdcb + ca * F.ddg2dxdy)
+    - 3.0 * ca * (1.0 - ca * ca) / xmysq * F.dg2dcb
+    + (1.0 - ca * ca) / xmy * (F.ddg2dxdcb + (1.0 - ca * ca) / xmy * F.ddg2dcbdcb + ca * F.ddg2dydcb)
+    + (1.0 - ca * ca) / xmy * F.dg2dy + ca * (F.ddg2dxdy + (1.0 - ca * ca) / xmy * F.ddg2dydcb + ca * F.ddg2dydy);
+  D.ddg2dxdcb = ysq / (xmysq * xmy) * (cborig + 3.0 * ca * cd) * F.dg2dcb
+    - ysq / xmysq * cd * (F.ddg2dxdcb + (1.0 - ca * ca) / xmy * F.ddg2dcbdcb + ca * F.ddg2dydcb)
+    + y / xmysq * (x * ca - xmy) * F.dg2dy - x * y / xmy * (F.ddg2dxdy + (1.0 - ca * ca) / xmy * F.ddg2dydcb + ca * F.ddg2dydy);
+  D.ddg2dxdy = (-(cborig + ca * cd) / xmy * F.ddg2dxdcb + cd * F.ddg2dxdy)
+    + (2.0 * ca * cborig + cd * (3.0 * ca * ca - 1.0)) / xmysq * F.dg2dcb
+    + (1.0 - ca * ca) / xmy * (-(cborig + ca * cd) / xmy * F.ddg2dcbdcb + cd * F.ddg2dydcb)
+    - (cborig + ca * cd) / xmy * F.dg2dy + ca * (-(cborig + ca * cd) / xmy * F.ddg2dydcb + cd * F.ddg2dydy);
+  D.ddg2dydcb = y / (xmysq * xmy) * ((3.0 * cd * cd - 1.0) * y - 2 * xmy * cd) * F.dg2dcb
+    - ysq / (xmysq) * cd * (-(cborig + ca * cd) / xmy * F.ddg2dcbdcb + cd * F.ddg2dydcb)
+    + x / xmysq * (y * cd - xmy) * F.dg2dy - x * y / xmy * (-(cborig + ca * cd) / xmy * F.ddg2dydcb + cd * F.ddg2dydy);
+  D.ddg2dcbdcb = x * ysq / (xmysq * xmysq) * (xmy - 3.0 * y * cd) * F.dg2dcb
+    - ysq / (xmysq) * cd * (-(ysq * cd * F.ddg2dcbdcb + x * y * xmy * F.ddg2dydcb) / xmysq)
+    - xsq * ysq / (xmysq * xmy) * F.dg2dy
+    - x * y / xmy * (-(ysq * cd * F.ddg2dydcb + x * y * xmy * F.ddg2dydy) / xmysq);
+  D.ddg1dxdx = (F.ddg1dxdx + (1.0 - ca * ca) / xmy * F.ddg1dxdcb + ca * F.ddg1dxdy)
+    - 3.0 * ca * (1.0 - ca * ca) / xmysq * F.dg1dcb
+    + (1.0 - ca * ca) / xmy * (F.ddg1dxdcb + (1.0 - ca * ca) / xmy * F.ddg1dcbdcb + ca * F.ddg1dydcb)
+    + (1.0 - ca * ca) / xmy * F.dg1dy + ca * (F.ddg1dxdy + (1.0 - ca * ca) / xmy * F.ddg1dydcb + ca * F.ddg1dydy)
+    - D.ddg2dxdx;
+  D.ddg1dxdcb = ysq / (xmysq * xmy) * (cborig + 3.0 * ca * cd) * F.dg1dcb
+    - ysq / xmysq * cd * (F.ddg1dxdcb + (1.0 - ca * ca) / xmy * F.ddg1dcbdcb + ca * F.ddg1dydcb)
+    + y / xmysq * (x * ca - xmy) * F.dg1dy
+    - x * y / xmy * (F.ddg1dxdy + (1.0 - ca * ca) / xmy * F.ddg1dydcb + ca * F.ddg1dydy)
+    - D.ddg2dxdcb;
+  D.ddg1dxdy = (-(cborig + ca * cd) / xmy * F.ddg1dxdcb + cd * F.ddg1dxdy)
+    + (2.0 * ca * cborig + cd * (3.0 * ca * ca - 1.0)) / xmysq * F.dg1dcb
+    + (1.0 - ca * ca) / xmy * (-(cborig + ca * cd) / xmy * F.ddg1dcbdcb + cd * F.ddg1dydcb)
+    - (cborig + ca * cd) / xmy * F.dg1dy + ca * (-(cborig + ca * cd) / xmy * F.ddg1dydcb + cd * F.ddg1dydy)
+    - D.ddg2dxdy;
+  D.ddg1dydcb = y / (xmysq * xmy) * ((3.0 * cd * cd - 1.0) * y - 2 * xmy * cd) * F.dg1dcb
+    - ysq / (xmysq) * cd * (-(cborig + ca * cd) / xmy * F.ddg1dcbdcb + cd * F.ddg1dydcb)
+    + x / xmysq * (y * cd - xmy) * F.dg1dy - x * y / xmy * (-(cborig + ca * cd) / xmy * F.ddg1dydcb + cd * F.ddg1dydy)
+    - D.ddg2dydcb;
+  D.ddg1dcbdcb = x * ysq / (xmysq * xmysq) * (xmy - 3.0 * y * cd) * F.dg1dcb
+    - ysq / (xmysq) * cd * (-(ysq * cd * F.ddg1dcbdcb + x * y * xmy * F.ddg1dydcb) / xmysq)
+    - xsq * ysq / (xmysq * xmy) * F.dg1dy
+    - x * y / xmy * (-(ysq * cd * F.ddg1dydcb + x * y * xmy * F.ddg1dydy) / xmysq)
+    - D.ddg2dcbdcb;
+}
+
+/* ------------------------------------------------------------------ standard-path evaluation */
+/* Computes the 36 STD primitives of one pair (S: 3, T: 12, invariants: 6, V: 15).
+ * Returns 0 on success, 1 when the pair is outside the grid (the reference then leaves zeros). */
+DEV int std_eval(const DT &T, const double *xv, const double *yv, double *out)
+{
+  const Inv I = set_inv(T, xv, yv);
+  double dg0dx, dg0dy, dg0dcb;
+  Tt Q;
+  Vt D;
+  for (int i = 0; i < (int)(sizeof(Tt) / 8); i++) ((double *)&Q)[i] = 0.0;
+  for (int i = 0; i < (int)(sizeof(Vt) / 8); i++) ((double *)&D)[i] = 0.0;
+  const double ysqT = I.flag2 ? I.xmysq : I.ysq;
+  if (I.x < T.XX[0]) {
+    /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0) */
+    if (S_taylor(T, dg0dy, dg0dx, dg0dcb, I)) return 1;
+    if (T_taylor(T, Q, I)) return 1;
+    if (V_taylor(T, D, I.x, I.y, I.xsq, I.flag2, I.cb)) return 1;
+  } else if (I.x > T.XX[T.nstpx - 1]) {
+    return 1;
+  } else {
+    const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
+    const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
+    double F[4];
+    if (!flag_hx && !flag_hy) {
+      const double twox = 2. * I.cb;
+      /* chnr_dS: QG0 */
+      simple_group<0>(T, QG0, I, twox, F);
+      dg0dcb = F[0]; dg0dx = F[1]; dg0dy = F[2];
+      /* chnr_dT: QL2, QL4, QG1 */
+      simple_group<2>(T, QL2, I, twox, F);
+      Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
+      simple_group<1>(T, QL4, I, twox, F);
+      Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
+      simple_group<0>(T, QG1, I, twox, F);
+      Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
+      /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
+      {
+        double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
+        for (int xn = 0; xn < 2; xn++) {
+          table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, I, twox, H2[xn], L2[xn]);
+          table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, I, twox, H8[xn], L8[xn]);
+          table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, I, twox, H12[xn], L12[xn]);
+        }
+        double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+        /* extractff2(QG2, d0cb): d = 1 */
+        for (int xn = 0; xn < 2; xn++) {
+          Hn[xn][0] = H2[xn][0]; Hn[xn][1] = H2[xn][1]; Ln[xn] = L2[xn][0];
+          Hd[xn][0] = H8[xn][0]; Hd[xn][1] = H8[xn][1]; Ld[xn] = L8[xn][0];
+        }
+        request(I.INVx, Hn, Ln, Hd, Ld, F);
+        D.g2 = F[3]; D.dg2dy = F[2];
+        /* extractff2(QG2, d1cb): d = 2 */
+        for (int xn = 0; xn < 2; xn++) {
+          Hn[xn][0] = H2[xn][1]; Hn[xn][1] = H2[xn][2]; Ln[xn] = L2[xn][1];
+          Hd[xn][0] = H8[xn][1]; Hd[xn][1] = H8[xn][2]; Ld[xn] = L8[xn][1];
+        }
+        request(I.INVx, Hn, Ln, Hd, Ld, F);
+        D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
+        /* extractff2(dxQG2, d0cb): d = 1 over (dxQG2, d2xQG2) */
+        for (int xn = 0; xn < 2; xn++) {
+          Hn[xn][0] = H8[xn][0]; Hn[xn][1] = H8[xn][1]; Ln[xn] = L8[xn][0];
+          Hd[xn][0] = H12[xn][0]; Hd[xn][1] = H12[xn][1]; Ld[xn] = L12[xn][0];
+        }
+        request(I.INVx, Hn, Ln, Hd, Ld, F);
+        D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+      }
+      /* chnr_dV, QG3 family: QG3 (orders 1-2), dxQG3 (0-2), d2xQG3 (0-1) */
+      {
+        double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
+        for (int xn = 0; xn < 2; xn++) {
+          table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, I, twox, H3[xn], L3[xn]);
+          table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, I, twox, H9[xn], L9[xn]);
+          table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, I, twox, H13[xn], L13[xn]);
+        }
+        double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+        /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
+        for (int xn = 0; xn < 2; xn++) {
+          Hn[xn][0] = H9[xn][0]; Hn[xn][1] = H9[xn][1]; Ln[xn] = L9[xn][0];
+          Hd[xn][0] = H13[xn][0]; Hd[xn][1] = H13[xn][1]; Ld[xn] = L13[xn][0];
+        }
+        request(I.INVx, Hn, Ln, Hd, Ld, F);
+        D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
+        /* extractff2(QG3, d1cb): d = 1 over (QG3, dxQG3) */
+        for (int xn = 0; xn < 2; xn++) {
+          Hn[xn][0] = H3[xn][0]; Hn[xn][1] = H3[xn][1]; Ln[xn] = L3[xn][0];
+          Hd[xn][0] = H9[xn][1]; Hd[xn][1] = H9[xn][2]; Ld[xn] = L9[xn][1];
+        }
+        request(I.INVx, Hn, Ln, Hd, Ld, F);
+        D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+      }
+      /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
+      D.dg3dy = ndy_packed(T, 2, 3, QG3, I, twox, 0);
+      V_algebra(D, I);
+      if (I.flag2) {
+        /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
+         * so they coincide unless the upper end sits on the top y node */
+        const int iy1 = I.INVy.idx;
+        const double fq1 = ndy_packed(T, 0, 1, QG2, I, twox, 1);
+        double fq2 = fq1;
+        Inv I2 = I;
+        I2.y = T.YY[iy1 + 1];
+        if (iy1 + 1 == T.nstpy - 1) fq2 = extractff_s(T, QG2, true, 0, I2);
+        D.ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+        const double gq1 = D.dg3dy;
+        double gq2 = gq1;
+        if (iy1 + 1 == T.nstpy - 1) gq2 = extractff_s(T, QG3, true, 0, I2);
+        const double ddg3dydy = (gq2 - gq1) / (T.YY[1] - T.YY[0]);
+        D.ddg1dydy = ddg3dydy - (I.y * D.ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
+      }
+    } else {
+      /* grid-edge fallback: every extractff2 on the scalar path */
+      extractff2_s(T, QG0, 0, I, F); dg0dcb = F[0]; dg0dx = F[1]; dg0dy = F[2];
+      extractff2_s(T, QL2, 0, I, F); Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
+      extractff2_s(T, QL4, 0, I, F); Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
+      extractff2_s(T, QG1, 0, I, F); Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
+      D.dg3dy = extractff_s(T, QG3, true, 0, I);
+      extractff2_s(T, QG2, 0, I, F); D.g2 = F[3]; D.dg2dy = F[2];
+      extractff2_s(T, dxQG2, 0, I, F); D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+      extractff2_s(T, dxQG3, 0, I, F); D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
+      extractff2_s(T, QG2, 1, I, F); D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
+      extractff2_s(T, QG3, 1, I, F); D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+      V_algebra(D, I);
+      if (I.flag2) {
+        const int iy1 = bsrch(T.YY, I.y, 0, T.nstpy);
+        if (iy1 == T.nstpy - 1) return 1;
+        Inv I1 = I, I2 = I;
+        I1.y = T.YY[iy1];
+        I2.y = T.YY[iy1 + 1];
+        double fq1 = extractff_s(T, QG2, true, 0, I1), fq2 = extractff_s(T, QG2, true, 0, I2);
+        D.ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+        fq1 = extractff_s(T, QG3, true, 0, I1);
+        fq2 = extractff_s(T, QG3, true, 0, I2);
+        const double ddg3dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+        D.ddg1dydy = ddg3dydy - (I.y * D.ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
+      }
+    }
+    T_algebra(Q, I, ysqT);
+  }
+  if (I.flag2) {
+    swap_S(dg0dx, dg0dcb, dg0dy, I);
+    swap_T(Q, I);
+    swap_V(D, I.x, I.xmy, I.yorig, I.xsq, I.xmysq, I.ysq, I.cb, I.cborig);
+  }
+  out[0] = dg0dx; out[1] = dg0dy; out[2] = dg0dcb;
+  out[3] = Q.ell1; out[4] = Q.ell2; out[5] = Q.ell3;
+  out[6] = Q.dell1dx; out[7] = Q.dell1dy; out[8] = Q.dell1dcb;
+  out[9] = Q.dell2dx; out[10] = Q.dell2dy; out[11] = Q.dell2dcb;
+  out[12] = Q.dell3dx; out[13] = Q.dell3dy; out[14] = Q.dell3dcb;
+  out[15] = I.x; out[16] = I.yorig; out[17] = I.cborig; out[18] = I.xsq; out[19] = I.ysq; out[20] = I.xdoty;
+  out[21] = D.dg1dx; out[22] = D.dg2dx; out[23] = D.dg1dcb; out[24] = D.dg2dcb; out[25] = D.dg1dy;
+  out[26] = D.ddg1dxdx; out[27] = D.ddg2dxdx; out[28] = D.ddg1dxdy; out[29] = D.ddg2dxdy;
+  out[30] = D.ddg1dxdcb; out[31] = D.ddg2dxdcb; out[32] = D.ddg1dydcb; out[33] = D.ddg2dydcb;
+  out[34] = D.ddg1dcbdcb; out[35] = D.ddg2dcbdcb;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ Taylor-table primitives */
+/* Tabd_xeq0 lerps (reference Tabd.c:23-69): out = v[4], y, ysq, fx, fy, ell2, dell2adx, dell2ady,
+ * ell3a, dg2dx, ddg2dxdy, dg1dx, dg1dy, phi1, phi2.  Returns 1 when |v| is past the table. */
+DEV int xeq0_prims(const DT &T, const double *yv, double *o)
+{
+  const double ysq = scal(yv, yv);
+  const double y = sqrt(fabs(ysq));
+  const int i1 = bsrch(T.TX + TYY * NYTAY, y, 0, NYTAY);
+  if (i1 == NYTAY - 1) return 1;
+  const int i2 = i1 + 1;
+  const double *TX = T.TX;
+  const double ay = (TXr(TYY, i2) - y) / (TXr(TYY, i2) - TXr(TYY, i1));
+  for (int i = 0; i < 4; i++) o[i] = yv[i];
+  o[4] = y; o[5] = ysq;
+  o[6] = 0.5 * lerp(ay, TXr(G0dx, i1), TXr(G0dx, i2));
+  o[7] = 0.5 * lerp(ay, TXr(G0dy, i1), TXr(G0dy, i2));
+  o[8] = lerp(ay, TXr(Gl2, i1), TXr(Gl2, i2));
+  o[9] = lerp(ay, TXr(Gl21, i1), TXr(Gl21, i2));
+  o[10] = lerp(ay, TXr(Gl2dy, i1), TXr(Gl2dy, i2));
+  o[11] = lerp(ay, TXr(Gl3, i1), TXr(Gl3, i2));
+  o[12] = lerp(ay, TXr(G21, i1), TXr(G21, i2));
+  o[13] = lerp(ay, TXr(G21dy, i1), TXr(G21dy, i2));
+  o[14] = (lerp(ay, TXr(G31A, i1), TXr(G31A, i2)) - (2. / 3) * (lerp(ay, TXr(G31B, i1), TXr(G31B, i2)))
+           - y * (lerp(ay, TXr(G22A, i1), TXr(G22A, i2))));
+  o[15] = (lerp(ay, TXr(G3Ady, i1), TXr(G3Ady, i2)) - lerp(ay, TXr(G3Bdy, i1), TXr(G3Bdy, i2)));
+  o[16] = (lerp(ay, TXr(G22A, i1), TXr(G22A, i2)) - lerp(ay, TXr(G22B, i1), TXr(G22B, i2)) / 5.);
+  o[17] = (lerp(ay, TXr(G22B, i1), TXr(G22B, i2))) * 1.2;
+  return 0;
+}
+#undef TXr
+
+/* Tabd_yeq0 lerps (reference Tabd.c:118-189): out = v[4], x, xsq, fx, fy, ell1, ell3, dell1dx,
+ * dell1dycb, dg1dx, ddg1dxdx, dg2dx, dg1dycb, ddg1dxdycb.  Returns 1 when |v| > XX[last]. */
+DEV int yeq0_prims(const DT &T, const double *xv, double *o)
+{
+  const double xsq = scal(xv, xv);
+  const double x = sqrt(fabs(xsq));
+  int ia = bsrch(T.XX, x, 0, T.nstpx);
+  if (x > T.XX[T.nstpx - 1]) return 1;
+  if (ia > T.nstpx - 2) ia = T.nstpx - 2; /* x on the last node: use the last cell */
+  const int ib = ia + 1;
+  const double *TY = T.TY;
+#define TYr(r, i) TY[(r) * NXTAY + (i)]
+  const double ax = (T.XX[ib] - x) / (T.XX[ib] - T.XX[ia]);
+  const double xa = T.XX[ia], xb = T.XX[ib];
+  const double xa2 = xa * xa, xa3 = xa2 * xa, xa4 = xa3 * xa, xb2 = xb * xb, xb3 = xb2 * xb, xb4 = xb3 * xb;
+  for (int i = 0; i < 4; i++) o[i] = xv[i];
+  o[4] = x; o[5] = xsq;
+  o[6] = 0.5 * lerp(ax, TYr(alpha0dx_0p, ia), TYr(alpha0dx_0p, ib));
+  o[7] = (lerp(ax, TYr(alpha0_1p, ia), TYr(alpha0_1p, ib)));
+  o[8] = 4 * (lerp(ax, TYr(alpha1_0p, ia) / (xa4), TYr(alpha1_0p, ib) / (xb4))) / 3.;
+  o[9] = lerp(ax, TYr(beta4_1p, ia) / (xa2), TYr(beta4_1p, ib) / (xb2));
+  o[10] = 4 * lerp(ax, (TYr(alpha1dx_0p, ia) - 4. * TYr(alpha1_0p, ia) / xa) / xa4,
+                   (TYr(alpha1dx_0p, ib) - 4. * TYr(alpha1_0p, ib) / xb) / xb4) / 3.;
+  o[11] = 2 * lerp(ax, (4. * TYr(alpha1_1p, ia) / (3 * xa) - TYr(beta4_1p, ia)) / xa3,
+                   (4. * TYr(alpha1_1p, ib) / (3 * xb) - TYr(beta4_1p, ib)) / xb3);
+  o[12] = lerp(ax, TYr(alpha3dx_0p, ia), TYr(alpha3dx_0p, ib));
+  o[13] = lerp(ax, TYr(alpha3dxdx_0p, ia), TYr(alpha3dxdx_0p, ib));
+  o[14] = 2 * lerp(ax, TYr(beta2dx_1p, ia), TYr(beta2dx_1p, ib));
+  o[15] = 2 * lerp(ax, TYr(alpha3_1p, ia) - TYr(beta2_1p, ia) / xa, TYr(alpha3_1p, ib) - TYr(beta2_1p, ib) / xb);
+  o[16] = 2 * lerp(ax, TYr(beta2_1p, ia) / (xa2) - TYr(beta2dx_1p, ia) / xa + TYr(alpha3dx_1p, ia),
+                   TYr(beta2_1p, ib) / (xb2) - TYr(beta2dx_1p, ib) / xb + TYr(alpha3dx_1p, ib));
+#undef TYr
+  return 0;
+}
+
+/* ------------------------------------------------------------------ classification + sort */
+/* key 0: Taylor-table regimes, zero, out of grid; 1 + ix*nstpy + iy: packed path; after those,
+ * one key per y node for the Taylor-x region */
+__global__ void k_classify(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
+                           unsigned *__restrict__ key, unsigned *__restrict__ count)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xv[4], yv[4];
+  for (int k = 0; k < 4; k++) { xv[k] = x[4 * i + k]; yv[k] = y[4 * i + k]; }
+  unsigned k = 0;
+  if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
+    const Inv I = set_inv(T, xv, yv);
+    if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (unsigned)I.INVy.idx;
+    else if (I.x <= T.XX[T.nstpx - 1]) k = 1 + (unsigned)(I.INVx.idx * T.nstpy + I.INVy.idx);
+  }
+  key[i] = k;
+  atomicAdd(&count[k], 1u);
+}
+
+__global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+/* exclusive scan of the bucket counts, one block */
+__global__ void k_scan(unsigned *__restrict__ count, unsigned nb)
+{
+  __shared__ unsigned part[1024];
+  const unsigned t = threadIdx.x, per = (nb + 1023) / 1024;
+  const unsigned lo = t * per, hi = min(nb, lo + per);
+  unsigned s = 0;
+  for (unsigned b = lo; b < hi; b++) s += count[b];
+  part[t] = s;
+  __syncthreads();
+  for (unsigned off = 1; off < 1024; off <<= 1) {
+    const unsigned v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = t ? part[t - 1] : 0u;
+  for (unsigned b = lo; b < hi; b++) { const unsigned c = count[b]; count[b] = run; run += c; }
+}
+
+__global__ void k_scatter(size_t n, const unsigned *__restrict__ key, unsigned *__restrict__ offs,
+                          unsigned *__restrict__ perm)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned pos = atomicAdd(&offs[key[i]], 1u);
+  perm[pos] = (unsigned)i;
+}
+
+/* ------------------------------------------------------------------ interpolation kernel */
+/* Evaluation plan of one pair (reference kernels.c:82-231): slot 0 holds L0(x,y) and slots 1-3
+ * the subtraction terms of L1, L2, L3 and M-L2, with their coefficients. */
+DEV void plan_tabd(const DT &T, int kind, const double *v, double *payload, unsigned &kinds, int slot)
+{
+  int bad = 0;
+  if (kind == KIND_XEQ0 || kind == KIND_XEQ0_M) bad = xeq0_prims(T, v, payload);
+  else bad = yeq0_prims(T, v, payload);
+  if (bad) kind = KIND_NONE;
+  kinds |= (unsigned)kind << (3 * slot);
+}
+
+/* L0(a,b) into slot `slot`: regime dispatch of reference kernels.c:82-115 */
+DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot)
+{
+  const bool az = is_zero(a), bz = is_zero(b);
+  double *pl = slot == 0 ? rec + KQED_REC_MAIN : rec + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
+  if (az && bz) return;
+  if (az) { plan_tabd(T, KIND_XEQ0, b, pl, kinds, slot); return; }
+  if (bz) { plan_tabd(T, KIND_YEQ0, a, pl, kinds, slot); return; }
+  if (x_is_y(a, b)) { plan_tabd(T, KIND_YEQ0_MULAM, a, pl, kinds, slot); return; }
+  if (slot != 0) return; /* only the main slot is ever a standard evaluation */
+  if (std_eval(T, a, b, pl) == 0) kinds |= (unsigned)KIND_STD;
+}
+
+__global__ void __launch_bounds__(256)
+k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
+         const unsigned *__restrict__ perm, int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t p = perm ? perm[i] : i;
+  double xv[4], yv[4];
+  for (int k = 0; k < 4; k++) { xv[k] = x[4 * p + k]; yv[k] = y[4 * p + k]; }
+  double *r = rec + p * KQED_REC_LEN;
+  for (int k = 0; k < 4; k++) { r[KQED_REC_X + k] = xv[k]; r[KQED_REC_Y + k] = yv[k]; }
+  unsigned kd = 0;
+  const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
+  double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  plan_L0(T, xv, yv, r, kd, 0);
+  switch (which) {
+  case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
+    plan_L0(T, xv, xv, r, kd, 1); plan_L0(T, yv, yv, r, kd, 2); c1 = -0.5; c2 = -0.5; break;
+  case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
+    plan_L0(T, z, yv, r, kd, 1); plan_L0(T, xv, z, r, kd, 2); c1 = -1.0; c2 = -1.0; break;
+  case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
+    plan_L0(T, xv, xv, r, kd, 1); plan_L0(T, z, xv, r, kd, 2); plan_L0(T, z, yv, r, kd, 3);
+    c1 = -1.0; c2 = 1.0; c3 = -1.0; break;
+  case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
+    if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
+    if (!is_zero(xv)) plan_tabd(T, KIND_YEQ0_M, xv, r + KQED_REC_AUX + KQED_AUX_LEN, kd, 2);
+    break;
+  default: break;
+  }
+  r[KQED_REC_COEF + 0] = 1.0;
+  r[KQED_REC_COEF + 1] = c1;
+  r[KQED_REC_COEF + 2] = c2;
+  r[KQED_REC_COEF + 3] = c3;
+  r[KQED_REC_M] = M;
+  r[KQED_REC_GX] = exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.);
+  r[KQED_REC_GY] = exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.);
+  r[15] = 0.0;
+  kinds[p] = kd;
+}
+
+/* ------------------------------------------------------------------ tensor entries */
+/* S/T/V entry c of the standard path from the 36 primitives (reference chnr_dS.c:85-93,
+ * chnr_dT.c:258-290, chnr_dV.c:277-359). c: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
+DEV double std_entry(const double *P, const double *xv, const double *yv, int c)
+{
+  const double x = P[15], yorig = P[16], cborig = P[17], xsq = P[18], ysq = P[19], xdoty = P[20];
+  if (c >= 192) {
+    const int bet = c & 3;
+    const double yhat = yv[bet] / yorig, xhat = xv[bet] / x;
+    const double c2 = (xhat - cborig * yhat) / yorig, c4 = (yhat - cborig * xhat) / x;
+    if (c < 196) return xhat * P[0] + c4 * P[2];
+    return yhat * P[1] + c2 * P[2];
+  }
+  const int alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
+  if (c >= 64) {
+    const double c1 = yv[bet] / yorig, c3 = xv[bet] / x;
+    const double c2 = (c3 - cborig * c1) / yorig, c4 = (c1 - cborig * c3) / x;
+    const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
+    const double t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
+    const double t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
+    const double t3 = xv[alf] * xv[dta] - d_ad * xsq / 4;
+    const double t4 = yv[alf] * yv[dta] - d_ad * ysq / 4;
+    const double t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * xdoty * d_ad;
+    const double ell1 = P[3], ell2 = P[4], ell3 = P[5];
+    if (c < 128)
+      return +t2 * ell1 + t1 * ell3 + t3 * (c3 * P[6] + c4 * P[8]) + t4 * (c3 * P[9] + c4 * P[11])
+             + t5 * (c3 * P[12] + c4 * P[14]);
+    return +t1 * ell2 + t2 * ell3 + t3 * (c1 * P[7] + c2 * P[8]) + t4 * (c1 * P[10] + c2 * P[11])
+           + t5 * (c1 * P[13] + c2 * P[14]);
+  }
+  /* get_dv (chnr_dV.c:277-359) with y = yorig, cb = cborig */
+  const double y = yorig, cb = cborig;
+  const double dg1dx = P[21], dg2dx = P[22], dg1dcb = P[23], dg2dcb = P[24], dg1dy = P[25];
+  const double xa = xv[alf] / x, xb = xv[bet] / x, ya = yv[alf] / y, yb = yv[bet] / y;
+  const double c4a = (ya - cb * xa) / x;
+  const double c2b = (xb - cb * yb) / y, c4b = (yb - cb * xb) / x;
+  const double D1 = xv[dta] * P[26] + yv[dta] * P[27];
+  const double D2 = xv[dta] * P[28] + yv[dta] * P[29];
+  const double D3 = xv[dta] * P[30] + yv[dta] * P[31];
+  const double D4 = xv[dta] * P[32] + yv[dta] * P[33];
+  const double D5 = xv[dta] * P[30] + yv[dta] * P[31];
+  const double D6 = xv[dta] * P[34] + yv[dta] * P[35];
+  const double D7 = (xv[dta] * dg1dx + yv[dta] * dg2dx) / x;
+  const double D8 = (xv[dta] * dg1dcb + yv[dta] * dg2dcb) / x;
+  const double astuff = (dg1dx + dg2dx) * xa + (dg1dcb + dg2dcb) * c4a;
+  const double bstuff = (dg1dx * xb + dg1dy * yb + dg1dcb * (xb / y + yb / x - cb * (xb / x + yb / y)));
+  const double c2c4 = c2b + c4b;
+  const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
+  const double xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
+  const double cc1 = ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) / x + (d_ab + xayb * cb - xaxb - yayb) / y);
+  double f = d_bd * astuff;
+  f += d_ad * bstuff;
+  f += D1 * xaxb + D2 * xayb;
+  f += (D3 * xb + D4 * yb) * c4a;
+  f += (D5 * xa + D6 * c4a) * c2c4;
+  f += D7 * (d_ab - xaxb);
+  return f + D8 * cc1;
+}
+
+/* Tabd_xeq0 tensors (reference Tabd.c:71-111). c: 0-63 tI, 64-127 tII, 128-191 tIII */
+DEV double xeq0_entry(const double *P, int c)
+{
+  const double *yv = P;
+  const double y = P[4], ysq = P[5], fx = P[6], fy = P[7], ell2 = P[8], dell2adx = P[9], dell2ady = P[10];
+  const double ell3a = P[11], dg2dx = P[12], ddg2dxdy = P[13], dg1dx = P[14], dg1dy = P[15], phi1 = P[16], phi2 = P[17];
+  const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
+  const double ha = yv[alf] / y, hb = yv[bet] / y, hd = yv[dta] / y;
+  const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
+  const double ysq_4 = ysq / 4.;
+  const double E = (ell3a + ell2), D2 = (dell2adx + dell2ady), D1 = (dg1dy + dg1dx), D12 = (dg1dx + dg2dx),
+               D3 = (y * ddg2dxdy - dg2dx);
+  if (t == 2) {
+    const double dgSUM = ((hb * fx / 2) + (hb * fy / 2));
+    const double tIIIsum = dgSUM - 0.5 * (yv[bet] * E) - ysq_4 * (hb * D2);
+    return (yv[alf] * yv[dta] * (hb * D2) + dab * (yv[dta] * E) + dbd * (yv[alf] * E) + dad * tIIIsum);
+  }
+  if (t == 1) {
+    const double yalf2dx = ha * (dell2adx);
+    const double tIIsum = (ha * fx / 2) - 0.5 * (yv[alf] * ell3a) - ysq_4 * yalf2dx;
+    return (yv[bet] * yv[dta] * yalf2dx + dbd * (tIIsum) + (dab * (yv[dta] * ell3a) + dad * (yv[bet] * ell3a)));
+  }
+  const double tIsum = 2 * (dab * phi1 + ha * hb * phi2);
+  return (dbd * (ha * D12) + (hd * dg2dx) * dab + (hb * D3) * ha * hd + (hb * D1) * dad + yv[dta] * tIsum);
+}
+
+/* Tabd_yeq0 tensors (reference Tabd.c:190-217, loop form) */
+DEV double yeq0_entry(const double *P, int c)
+{
+  const double *xv = P;
+  const double x = P[4], xsq = P[5], fx = P[6], fy = P[7], ell1 = P[8], ell3 = P[9], dell1dx = P[10];
+  const double dell1dycb = P[11], dg1dx = P[12], ddg1dxdx = P[13], dg2dx = P[14], dg1dycb = P[15], ddg1dxdycb = P[16];
+  const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
+  const double ha = xv[alf] / x, hb = xv[bet] / x, hd = xv[dta] / x;
+  const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
+  const double xsq_4 = xsq / 4., E = ell1 + ell3, D1 = dell1dx + dell1dycb, D2 = (ddg1dxdx + ddg1dxdycb),
+               D3 = (dg1dycb - dg2dx), D4 = (dg1dx + dg1dycb);
+  if (t == 2) {
+    const double Dg0SUM = ((hb * fx / 2) + (hb * fy / 2) - 0.5 * (xv[bet] * E));
+    const double xad = (xv[alf] * xv[dta] - xsq_4 * dad);
+    return ((xad * hb * D1 + dab * (xv[dta] * E) + dbd * (xv[alf] * E) + Dg0SUM * dad));
+  }
+  if (t == 1) {
+    const double xhd1dx = ha * (dell1dx);
+    const double tIIsum = ((ha * fx / 2) - 0.5 * (xv[alf] * ell1));
+    return ((xv[bet] * xv[dta] - xsq_4 * dbd) * xhd1dx + dab * (xv[dta] * ell1) + dad * (xv[bet] * ell1) + dbd * (tIIsum));
+  }
+  return ((dad * hb + dbd * ha + dab * hd - ha * hb * hd) * D4 + ha * (hb * xv[dta] * D2 - dbd * D3));
+}
+
+/* ------------------------------------------------------------------ assembly kernel */
+#define NB 16
+#define ASM_THREADS 384
+
+DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r,
+                   const double *Tv)
+{
+  double s = 0.0;
+  const int e = rp[r + 1];
+  for (int k = rp[r]; k < e; k++) s = fma(val[k], Tv[col[k]], s);
+  return s;
+}
+
+__global__ void __launch_bounds__(ASM_THREADS)
+k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, double *__restrict__ out,
+           int layout)
+{
+  __shared__ double sR[NB][KQED_REC_LEN];
+  __shared__ double sT[NB][200];
+  __shared__ unsigned sK[NB];
+  const int tid = threadIdx.x;
+  const int row = tid; /* one output component per thread */
+  for (size_t base = (size_t)blockIdx.x * NB; base < n; base += (size_t)gridDim.x * NB) {
+    const int nb = (int)min((size_t)NB, n - base);
+    __syncthreads();
+    for (int i = tid; i < nb * KQED_REC_LEN; i += ASM_THREADS) sR[i / KQED_REC_LEN][i % KQED_REC_LEN] = rec[base * KQED_REC_LEN + i];
+    if (tid < nb) sK[tid] = kinds[base + tid];
+    double acc[NB];
+#pragma unroll
+    for (int p = 0; p < NB; p++) acc[p] = 0.0;
+    for (int slot = 0; slot < 4; slot++) {
+      __syncthreads();
+      /* tensors of this slot for every pair */
+      for (int i = tid; i < nb * 200; i += ASM_THREADS) {
+        const int p = i / 200, c = i % 200;
+        const unsigned kd = (sK[p] >> (3 * slot)) & 7u;
+        const double *pl = slot == 0 ? &sR[p][KQED_REC_MAIN] : &sR[p][KQED_REC_AUX + KQED_AUX_LEN * (slot - 1)];
+        double v = 0.0;
+        if (kd == KIND_STD) v = std_entry(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], c);
+        else if (c < 192 && (kd == KIND_XEQ0 || kd == KIND_XEQ0_M)) v = xeq0_entry(pl, c);
+        else if (c < 192 && (kd == KIND_YEQ0 || kd == KIND_YEQ0_MULAM || kd == KIND_YEQ0_M)) v = yeq0_entry(pl, c);
+        sT[p][c] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int p = 0; p < NB; p++) {
+        const unsigned kd = p < nb ? (sK[p] >> (3 * slot)) & 7u : (unsigned)KIND_NONE;
+        if (kd != KIND_NONE) {
+        const int m = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
+        const double v = row_dot(T.rowptr[m], T.col[m], T.val[m], row, sT[p]);
+        if (slot == 0) {
+          acc[p] = v;
+        } else if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
+          /* reference kernels.c:292-323 */
+          const double M = sR[p][KQED_REC_M];
+          const int rs = row >> 6, mu = (row >> 4) & 3, nu = (row >> 2) & 3, la = row & 3;
+          double k4[4];
+          if (kd == KIND_XEQ0_M) {
+            const double *xv = &sR[p][KQED_REC_X];
+            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, sT[p]);
+            acc[p] -= sR[p][KQED_REC_GX] * (v - M * xv[mu] * (xv[0] * k4[0] + xv[1] * k4[1] + xv[2] * k4[2] + xv[3] * k4[3]));
+          } else {
+            const double *yv = &sR[p][KQED_REC_Y];
+            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, sT[p]);
+            acc[p] -= sR[p][KQED_REC_GY] * (v - M * yv[nu] * (yv[0] * k4[0] + yv[1] * k4[1] + yv[2] * k4[2] + yv[3] * k4[3]));
+          }
+        } else {
+          acc[p] = fma(v, sR[p][KQED_REC_COEF + slot], acc[p]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
+        }
+        }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < NB; p++) {
+      if (p < nb) {
+        if (layout == 0) out[(base + p) * 384 + row] = acc[p];
+        else out[(size_t)row * n + base + p] = acc[p];
+      }
+    }
+  }
+}

@@ -1,0 +1,72 @@
+"""FP64 HIP path vs the CPU oracle: every regime, every kernel (L0..L3, M-L2).
+
+Bar: per-pair normwise relative error max|gpu - oracle| / max|oracle| <= 1e-12 (FP64 target of the
+plain path); pairs the oracle returns as all-zero must be all-zero on the GPU too.
+"""
+import numpy as np
+import pytest
+
+import kqed_amd
+from pairs import log_uniform, normwise, regime_mix
+
+TOL = 1e-12
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(gpu, oracle, which, x, y, M=0.0, tags=None):
+    got = gpu.batch_host(which, x, y, M=M)
+    ref = oracle.batch(which, x, y, M=M)
+    err = normwise(got, ref)
+    zero_ref = np.abs(ref).max(axis=1) == 0
+    assert np.all(np.abs(got[zero_ref]).max(axis=1, initial=0) == 0), "GPU nonzero where the oracle is zero"
+    worst = int(np.argmax(err))
+    msg = f"which={which} worst pair {worst} tag={None if tags is None else tags[worst]} err={err[worst]:.3e}"
+    assert err.max() <= TOL, msg
+    return err
+
+
+@pytest.mark.parametrize("which", [0, 1, 2, 3])
+def test_regimes_small_table(gpu_small, oracle_small, which):
+    x, y, tags = regime_mix(11 + which)
+    _check(gpu_small, oracle_small, which, x, y, tags=tags)
+
+
+@pytest.mark.parametrize("M", [0.0, 0.5, 1.2])
+def test_mkernel_small_table(gpu_small, oracle_small, M):
+    x, y, tags = regime_mix(31)
+    _check(gpu_small, oracle_small, kqed_amd.KQED_ML2, x, y, M=M, tags=tags)
+
+
+@pytest.mark.parametrize("which", [0, 2, 3])
+def test_regimes_full_table(gpu_full, oracle_full, which):
+    x, y, tags = regime_mix(101 + which, n_each=24)
+    _check(gpu_full, oracle_full, which, x, y, tags=tags)
+
+
+def test_sorted_batch_full_table(gpu_full, oracle_full):
+    """A batch above the sort threshold (cell-sorted path) on the benchmark's input law."""
+    x, y = log_uniform(7, 6000)
+    got = gpu_full.batch_host(0, x, y)
+    idx = np.random.default_rng(3).choice(len(x), 600, replace=False)
+    ref = oracle_full.batch(0, x[idx], y[idx])
+    assert normwise(got[idx], ref).max() <= TOL
+
+
+def test_per_pair_api_matches_batch(gpu_small):
+    x, y, _ = regime_mix(5, n_each=3)
+    b = gpu_small.batch_host(0, x, y)
+    for i in range(0, len(x), 7):
+        k = gpu_small.QED_kernel_L0(x[i], y[i]).reshape(-1)
+        assert np.array_equal(k, b[i])
+    k2 = gpu_small.QED_kernel_L2(x[0], y[0]).reshape(-1)
+    assert np.array_equal(k2, gpu_small.batch_host(2, x[:1], y[:1])[0])
+    k3 = gpu_small.QED_Mkernel_L2(0.5, x[0], y[0]).reshape(-1)
+    assert np.array_equal(k3, gpu_small.batch_host(4, x[:1], y[:1], M=0.5)[0])
+
+
+def test_soa_layout(gpu_small):
+    x, y, _ = regime_mix(8, n_each=5)
+    a = gpu_small.batch_host(1, x, y, layout=kqed_amd.KQED_AOS)
+    s = gpu_small.batch_host(1, x, y, layout=kqed_amd.KQED_SOA)
+    assert np.array_equal(a, s.T)
