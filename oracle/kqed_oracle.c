@@ -171,7 +171,7 @@ int oq_load(const char *dir, oq_tables *T)
   T->NY_tay = 810;
   T->NX_tay = 100;
   /* ratio LUTs (simd.c:716-784); sized past the longest Clenshaw so no read runs off the end */
-  T->nLUT = T->nfmax + 4;
+  T->nLUT = T->nfmax + 4 < 160 ? 160 : T->nfmax + 4;
   for (int l = 1; l <= 6; l++) T->LUT[l] = calloc(T->nLUT, sizeof(double));
   for (int i = 1; i < T->nLUT; i++) {
     T->LUT[1][i] = (i + 1) / (double)i;
