@@ -48,6 +48,12 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m kqed_amd.build`")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 (same soname, another path).
+    # Loading torch first lets this library bind to that runtime instead of starting a second one.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.POINTER(_Temps)
     arr = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
