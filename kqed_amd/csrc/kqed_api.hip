@@ -24,9 +24,12 @@ __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
 __global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, const double *y, unsigned *key, unsigned *count);
 __global__ void k_scan(unsigned *count, unsigned nb);
 __global__ void k_scatter(size_t n, const unsigned *key, unsigned *offs, unsigned *perm);
-__global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, const double *y, const unsigned *perm, int which,
-                         double M, double *rec, unsigned *kinds);
-__global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out, int layout);
+__global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
+template <int CLS>
+__global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, const double *y, const unsigned *perm,
+                         const unsigned *bounds, int ilo, int ihi, int which, double M, double *rec, unsigned *kinds);
+__global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out, int layout,
+                           int nslots);
 
 #ifndef KQED_DEFAULT_TABLE_DIR
 #define KQED_DEFAULT_TABLE_DIR "/usr/local/share/kqed/PRECOMP"
@@ -216,6 +219,7 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     /* assembly maps from the Dirac traces */
     d->maps = (kqed_map *)calloc(3, sizeof(kqed_map));
     if (!d->maps || kqed_build_maps(&d->maps[0], &d->maps[1], &d->maps[2])) { set_err("assembly map build failed"); goto fail; }
+    if (d->maps[0].kmax > 14 || d->maps[1].kmax > 10 || d->maps[2].kmax > 10) { set_err("assembly map rows too long"); goto fail; }
     for (int m = 0; m < 3; m++) {
       int *rp = (int *)dev_alloc(d, 4 * 385);
       short *cl = (short *)dev_alloc(d, 2 * (size_t)d->maps[m].nnz);
@@ -316,13 +320,24 @@ extern "C" int kqed_batch(const struct QED_kernel_temps *t, int which, double M,
       prof_end(d, s, ev);
     }
     prof_begin(d, s, ev, ST_INTERP);
-    hipLaunchKernelGGL(k_interp, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, which, M, w.rec, w.kinds);
+    if (perm) {
+      /* after the scatter, count[b] is the end of bucket b: special pairs [0, count[0]),
+       * packed path [count[0], count[nstpx*nstpy]), Taylor-x region [count[nstpx*nstpy], c) */
+      const int ip = T.nstpx * T.nstpy;
+      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, -1, 0, which, M, w.rec, w.kinds);
+      hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, 0, ip, which, M, w.rec, w.kinds);
+      hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, ip, -1, which, M, w.rec, w.kinds);
+    } else {
+      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, -1, -1, which, M, w.rec, w.kinds);
+    }
+    hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
     double *ob = layout == KQED_AOS ? out_dev + 384 * b : out_dev;
     const size_t nblk = (c + 15) / 16;
     prof_begin(d, s, ev, ST_ASSEMBLE);
+    static const int nslots[5] = { 1, 3, 3, 4, 3 };
     hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 8192 ? nblk : 8192)), dim3(384), 0, s, T, c, w.rec, w.kinds,
-                       ob, layout);
+                       ob, layout, nslots[which]);
     prof_end(d, s, ev);
   }
   const hipError_t e = hipGetLastError();

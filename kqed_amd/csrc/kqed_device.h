@@ -12,8 +12,8 @@
  * ks = nfmax + 2 (longest Clenshaw: nfx + mm, mm <= 2).
  *
  * Per-pair record written by the interpolation kernel and read by the assembly kernel:
- *   rec[REC_LEN] doubles: x[4], y[4], coef[4], M, gaussX, gaussY, pad, main payload [36],
- *   three Tabd payloads [20]
+ *   rec[REC_LEN] doubles: x[4], y[4], coef[4], M, gaussX, gaussY, pad, main payload [64],
+ *   three Tabd payloads [20], pad
  *   kinds (int): 3 bits per evaluation slot (KIND_*)
  */
 #ifndef KQED_DEVICE_H
@@ -21,15 +21,15 @@
 
 #include <hip/hip_runtime.h>
 
-#define KQED_REC_LEN 112
+#define KQED_REC_LEN 144
 #define KQED_REC_X 0      /* x[4] */
 #define KQED_REC_Y 4      /* y[4] */
 #define KQED_REC_COEF 8   /* coefficient of each evaluation slot [4] */
 #define KQED_REC_M 12     /* M of M-L2 */
 #define KQED_REC_GX 13    /* exp(-M x^2 / 2) */
 #define KQED_REC_GY 14    /* exp(-M y^2 / 2) */
-#define KQED_REC_MAIN 16  /* slot 0 payload [36] */
-#define KQED_REC_AUX 52   /* slots 1..3 payloads [20] each */
+#define KQED_REC_MAIN 16  /* slot 0 payload [64]: stage-1 record, then the 36 primitives */
+#define KQED_REC_AUX 80   /* slots 1..3 payloads [20] each */
 #define KQED_AUX_LEN 20
 
 enum {

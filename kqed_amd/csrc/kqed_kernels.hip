@@ -739,159 +739,203 @@ DEV void swap_V(Vt &D, double x, double xmy, double y, double xsq, double xmysq,
 }
 
 /* ------------------------------------------------------------------ standard-path evaluation */
-/* Computes the 36 STD primitives of one pair (S: 3, T: 12, invariants: 6, V: 15).
- * Returns 0 on success, 1 when the pair is outside the grid (the reference then leaves zeros). */
-DEV int std_eval(const DT &T, const double *xv, const double *yv, double *out)
+/* Stage-1 record of a standard-regime pair: invariants and the interpolated weight functions
+ * before the chain-rule algebra (the interpolation kernels write it field by field as each
+ * table group finishes, so no group's results stay live in registers). */
+struct Stage1 {
+  double x, cb, y, xsq, ysq, xdoty, cborig, yorig, xmysq, xmy, flag2, need_alg;
+  double dg0dx, dg0dy, dg0dcb;
+  Tt T;
+  Vt V;
+};
+
+/* Interpolation of one pair into st.  CLS: 0 any pair, 1 packed-path pairs only, 2 Taylor-x pairs
+ * only (the sort sends each class to its own kernel, which then carries only its own code path).
+ * Returns 1 when the pair is outside the grid (the reference then leaves an all-zero kernel). */
+template <int CLS>
+DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st)
 {
   const Inv I = set_inv(T, xv, yv);
-  double dg0dx, dg0dy, dg0dcb;
-  Tt Q;
-  Vt D;
-  for (int i = 0; i < (int)(sizeof(Tt) / 8); i++) ((double *)&Q)[i] = 0.0;
-  for (int i = 0; i < (int)(sizeof(Vt) / 8); i++) ((double *)&D)[i] = 0.0;
-  const double ysqT = I.flag2 ? I.xmysq : I.ysq;
-  if (I.x < T.XX[0]) {
-    /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0) */
+  st->x = I.x; st->cb = I.cb; st->y = I.y; st->xsq = I.xsq; st->ysq = I.ysq; st->xdoty = I.xdoty;
+  st->cborig = I.cborig; st->yorig = I.yorig; st->xmysq = I.xmysq; st->xmy = I.xmy;
+  st->flag2 = I.flag2 ? 1.0 : 0.0;
+  if (CLS != 1 && (CLS == 2 || I.x < T.XX[0])) {
+    /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0); the Taylor
+     * branches already apply the algebra (chnr_dT.c:16-143, chnr_dV.c:19-174) */
+    double dg0dx, dg0dy, dg0dcb;
     if (S_taylor(T, dg0dy, dg0dx, dg0dcb, I)) return 1;
+    Tt Q;
+    for (int i = 0; i < (int)(sizeof(Tt) / 8); i++) ((double *)&Q)[i] = 0.0;
     if (T_taylor(T, Q, I)) return 1;
+    Vt D;
+    for (int i = 0; i < (int)(sizeof(Vt) / 8); i++) ((double *)&D)[i] = 0.0;
     if (V_taylor(T, D, I.x, I.y, I.xsq, I.flag2, I.cb)) return 1;
-  } else if (I.x > T.XX[T.nstpx - 1]) {
-    return 1;
-  } else {
-    const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
-    const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
-    double F[4];
-    if (!flag_hx && !flag_hy) {
-      const double twox = 2. * I.cb;
-      /* chnr_dS: QG0 */
-      simple_group<0>(T, QG0, I, twox, F);
-      dg0dcb = F[0]; dg0dx = F[1]; dg0dy = F[2];
-      /* chnr_dT: QL2, QL4, QG1 */
-      simple_group<2>(T, QL2, I, twox, F);
-      Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
-      simple_group<1>(T, QL4, I, twox, F);
-      Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
-      simple_group<0>(T, QG1, I, twox, F);
-      Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
-      /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
-      {
-        double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
-        for (int xn = 0; xn < 2; xn++) {
-          table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, I, twox, H2[xn], L2[xn]);
-          table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, I, twox, H8[xn], L8[xn]);
-          table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, I, twox, H12[xn], L12[xn]);
-        }
-        double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
-        /* extractff2(QG2, d0cb): d = 1 */
-        for (int xn = 0; xn < 2; xn++) {
-          Hn[xn][0] = H2[xn][0]; Hn[xn][1] = H2[xn][1]; Ln[xn] = L2[xn][0];
-          Hd[xn][0] = H8[xn][0]; Hd[xn][1] = H8[xn][1]; Ld[xn] = L8[xn][0];
-        }
-        request(I.INVx, Hn, Ln, Hd, Ld, F);
-        D.g2 = F[3]; D.dg2dy = F[2];
-        /* extractff2(QG2, d1cb): d = 2 */
-        for (int xn = 0; xn < 2; xn++) {
-          Hn[xn][0] = H2[xn][1]; Hn[xn][1] = H2[xn][2]; Ln[xn] = L2[xn][1];
-          Hd[xn][0] = H8[xn][1]; Hd[xn][1] = H8[xn][2]; Ld[xn] = L8[xn][1];
-        }
-        request(I.INVx, Hn, Ln, Hd, Ld, F);
-        D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
-        /* extractff2(dxQG2, d0cb): d = 1 over (dxQG2, d2xQG2) */
-        for (int xn = 0; xn < 2; xn++) {
-          Hn[xn][0] = H8[xn][0]; Hn[xn][1] = H8[xn][1]; Ln[xn] = L8[xn][0];
-          Hd[xn][0] = H12[xn][0]; Hd[xn][1] = H12[xn][1]; Ld[xn] = L12[xn][0];
-        }
-        request(I.INVx, Hn, Ln, Hd, Ld, F);
-        D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+    st->dg0dx = dg0dx; st->dg0dy = dg0dy; st->dg0dcb = dg0dcb;
+    st->T = Q;
+    st->V = D;
+    st->need_alg = 0.0;
+    return 0;
+  }
+  if (CLS == 2) return 1;
+  if (CLS != 1 && I.x > T.XX[T.nstpx - 1]) return 1;
+  st->need_alg = 1.0;
+  Tt &Q = st->T;
+  Vt &D = st->V;
+  const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
+  const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
+  double F[4];
+  if (CLS == 1 || (!flag_hx && !flag_hy)) {
+    const double twox = 2. * I.cb;
+    /* chnr_dS: QG0 */
+    simple_group<0>(T, QG0, I, twox, F);
+    st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
+    /* chnr_dT: QL2, QL4, QG1 */
+    simple_group<2>(T, QL2, I, twox, F);
+    Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
+    simple_group<1>(T, QL4, I, twox, F);
+    Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
+    simple_group<0>(T, QG1, I, twox, F);
+    Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
+    /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
+    {
+      double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
+      for (int xn = 0; xn < 2; xn++) {
+        table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, I, twox, H2[xn], L2[xn]);
+        table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, I, twox, H8[xn], L8[xn]);
+        table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, I, twox, H12[xn], L12[xn]);
       }
-      /* chnr_dV, QG3 family: QG3 (orders 1-2), dxQG3 (0-2), d2xQG3 (0-1) */
-      {
-        double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
-        for (int xn = 0; xn < 2; xn++) {
-          table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, I, twox, H3[xn], L3[xn]);
-          table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, I, twox, H9[xn], L9[xn]);
-          table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, I, twox, H13[xn], L13[xn]);
-        }
-        double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
-        /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
-        for (int xn = 0; xn < 2; xn++) {
-          Hn[xn][0] = H9[xn][0]; Hn[xn][1] = H9[xn][1]; Ln[xn] = L9[xn][0];
-          Hd[xn][0] = H13[xn][0]; Hd[xn][1] = H13[xn][1]; Ld[xn] = L13[xn][0];
-        }
-        request(I.INVx, Hn, Ln, Hd, Ld, F);
-        D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
-        /* extractff2(QG3, d1cb): d = 1 over (QG3, dxQG3) */
-        for (int xn = 0; xn < 2; xn++) {
-          Hn[xn][0] = H3[xn][0]; Hn[xn][1] = H3[xn][1]; Ln[xn] = L3[xn][0];
-          Hd[xn][0] = H9[xn][1]; Hd[xn][1] = H9[xn][2]; Ld[xn] = L9[xn][1];
-        }
-        request(I.INVx, Hn, Ln, Hd, Ld, F);
-        D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+      double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+      /* extractff2(QG2, d0cb): d = 1 */
+      for (int xn = 0; xn < 2; xn++) {
+        Hn[xn][0] = H2[xn][0]; Hn[xn][1] = H2[xn][1]; Ln[xn] = L2[xn][0];
+        Hd[xn][0] = H8[xn][0]; Hd[xn][1] = H8[xn][1]; Ld[xn] = L8[xn][0];
       }
-      /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
-      D.dg3dy = ndy_packed(T, 2, 3, QG3, I, twox, 0);
-      V_algebra(D, I);
-      if (I.flag2) {
-        /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
-         * so they coincide unless the upper end sits on the top y node */
-        const int iy1 = I.INVy.idx;
-        const double fq1 = ndy_packed(T, 0, 1, QG2, I, twox, 1);
-        double fq2 = fq1;
+      request(I.INVx, Hn, Ln, Hd, Ld, F);
+      D.g2 = F[3]; D.dg2dy = F[2];
+      /* extractff2(QG2, d1cb): d = 2 */
+      for (int xn = 0; xn < 2; xn++) {
+        Hn[xn][0] = H2[xn][1]; Hn[xn][1] = H2[xn][2]; Ln[xn] = L2[xn][1];
+        Hd[xn][0] = H8[xn][1]; Hd[xn][1] = H8[xn][2]; Ld[xn] = L8[xn][1];
+      }
+      request(I.INVx, Hn, Ln, Hd, Ld, F);
+      D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
+      /* extractff2(dxQG2, d0cb): d = 1 over (dxQG2, d2xQG2) */
+      for (int xn = 0; xn < 2; xn++) {
+        Hn[xn][0] = H8[xn][0]; Hn[xn][1] = H8[xn][1]; Ln[xn] = L8[xn][0];
+        Hd[xn][0] = H12[xn][0]; Hd[xn][1] = H12[xn][1]; Ld[xn] = L12[xn][0];
+      }
+      request(I.INVx, Hn, Ln, Hd, Ld, F);
+      D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+    }
+    /* chnr_dV, QG3 family: QG3 (orders 1-2), dxQG3 (0-2), d2xQG3 (0-1) */
+    {
+      double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
+      for (int xn = 0; xn < 2; xn++) {
+        table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, I, twox, H3[xn], L3[xn]);
+        table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, I, twox, H9[xn], L9[xn]);
+        table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, I, twox, H13[xn], L13[xn]);
+      }
+      double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+      /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
+      for (int xn = 0; xn < 2; xn++) {
+        Hn[xn][0] = H9[xn][0]; Hn[xn][1] = H9[xn][1]; Ln[xn] = L9[xn][0];
+        Hd[xn][0] = H13[xn][0]; Hd[xn][1] = H13[xn][1]; Ld[xn] = L13[xn][0];
+      }
+      request(I.INVx, Hn, Ln, Hd, Ld, F);
+      D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
+      /* extractff2(QG3, d1cb): d = 1 over (QG3, dxQG3) */
+      for (int xn = 0; xn < 2; xn++) {
+        Hn[xn][0] = H3[xn][0]; Hn[xn][1] = H3[xn][1]; Ln[xn] = L3[xn][0];
+        Hd[xn][0] = H9[xn][1]; Hd[xn][1] = H9[xn][2]; Ld[xn] = L9[xn][1];
+      }
+      request(I.INVx, Hn, Ln, Hd, Ld, F);
+      D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+    }
+    /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
+    const double dg3dy = ndy_packed(T, 2, 3, QG3, I, twox, 0);
+    D.dg3dy = dg3dy;
+    if (I.flag2) {
+      /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
+       * so they coincide unless the upper end sits on the top y node */
+      const int iy1 = I.INVy.idx;
+      const double fq1 = ndy_packed(T, 0, 1, QG2, I, twox, 1);
+      double fq2 = fq1, gq2 = dg3dy;
+      if (CLS == 0 && iy1 + 1 == T.nstpy - 1) {
         Inv I2 = I;
         I2.y = T.YY[iy1 + 1];
-        if (iy1 + 1 == T.nstpy - 1) fq2 = extractff_s(T, QG2, true, 0, I2);
-        D.ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
-        const double gq1 = D.dg3dy;
-        double gq2 = gq1;
-        if (iy1 + 1 == T.nstpy - 1) gq2 = extractff_s(T, QG3, true, 0, I2);
-        const double ddg3dydy = (gq2 - gq1) / (T.YY[1] - T.YY[0]);
-        D.ddg1dydy = ddg3dydy - (I.y * D.ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
+        fq2 = extractff_s(T, QG2, true, 0, I2);
+        gq2 = extractff_s(T, QG3, true, 0, I2);
       }
-    } else {
-      /* grid-edge fallback: every extractff2 on the scalar path */
-      extractff2_s(T, QG0, 0, I, F); dg0dcb = F[0]; dg0dx = F[1]; dg0dy = F[2];
-      extractff2_s(T, QL2, 0, I, F); Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
-      extractff2_s(T, QL4, 0, I, F); Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
-      extractff2_s(T, QG1, 0, I, F); Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
-      D.dg3dy = extractff_s(T, QG3, true, 0, I);
-      extractff2_s(T, QG2, 0, I, F); D.g2 = F[3]; D.dg2dy = F[2];
-      extractff2_s(T, dxQG2, 0, I, F); D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
-      extractff2_s(T, dxQG3, 0, I, F); D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
-      extractff2_s(T, QG2, 1, I, F); D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
-      extractff2_s(T, QG3, 1, I, F); D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
-      V_algebra(D, I);
-      if (I.flag2) {
-        const int iy1 = bsrch(T.YY, I.y, 0, T.nstpy);
-        if (iy1 == T.nstpy - 1) return 1;
-        Inv I1 = I, I2 = I;
-        I1.y = T.YY[iy1];
-        I2.y = T.YY[iy1 + 1];
-        double fq1 = extractff_s(T, QG2, true, 0, I1), fq2 = extractff_s(T, QG2, true, 0, I2);
-        D.ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
-        fq1 = extractff_s(T, QG3, true, 0, I1);
-        fq2 = extractff_s(T, QG3, true, 0, I2);
-        const double ddg3dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
-        D.ddg1dydy = ddg3dydy - (I.y * D.ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
-      }
+      const double ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+      D.ddg2dydy = ddg2dydy;
+      const double ddg3dydy = (gq2 - dg3dy) / (T.YY[1] - T.YY[0]);
+      D.ddg1dydy = ddg3dydy - (I.y * ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
     }
-    T_algebra(Q, I, ysqT);
+  } else if (CLS == 0) {
+    /* grid-edge fallback: every extractff2 on the scalar path */
+    extractff2_s(T, QG0, 0, I, F); st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
+    extractff2_s(T, QL2, 0, I, F); Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
+    extractff2_s(T, QL4, 0, I, F); Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
+    extractff2_s(T, QG1, 0, I, F); Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
+    D.dg3dy = extractff_s(T, QG3, true, 0, I);
+    extractff2_s(T, QG2, 0, I, F); D.g2 = F[3]; D.dg2dy = F[2];
+    extractff2_s(T, dxQG2, 0, I, F); D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+    extractff2_s(T, dxQG3, 0, I, F); D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
+    extractff2_s(T, QG2, 1, I, F); D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
+    extractff2_s(T, QG3, 1, I, F); D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+    if (I.flag2) {
+      const int iy1 = bsrch(T.YY, I.y, 0, T.nstpy);
+      if (iy1 == T.nstpy - 1) return 1;
+      Inv I1 = I, I2 = I;
+      I1.y = T.YY[iy1];
+      I2.y = T.YY[iy1 + 1];
+      double fq1 = extractff_s(T, QG2, true, 0, I1), fq2 = extractff_s(T, QG2, true, 0, I2);
+      const double ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+      D.ddg2dydy = ddg2dydy;
+      fq1 = extractff_s(T, QG3, true, 0, I1);
+      fq2 = extractff_s(T, QG3, true, 0, I2);
+      const double ddg3dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+      D.ddg1dydy = ddg3dydy - (I.y * ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
+    }
+  }
+  return 0;
+}
+
+/* Chain-rule algebra of chnr_dT/chnr_dV (standard branch), the x <-> x-y swap, and the 36
+ * primitives the assembly kernel reads (S: 3, T: 12, invariants: 6, V: 15), written in place. */
+__global__ void __launch_bounds__(256)
+k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds)
+{
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || (kinds[p] & 7u) != (unsigned)KIND_STD) return;
+  double *m = rec + p * KQED_REC_LEN + KQED_REC_MAIN;
+  Stage1 s = *(const Stage1 *)m;
+  Inv I;
+  I.x = s.x; I.cb = s.cb; I.y = s.y; I.xsq = s.xsq; I.ysq = s.ysq; I.xdoty = s.xdoty;
+  I.cborig = s.cborig; I.yorig = s.yorig; I.xmysq = s.xmysq; I.xmy = s.xmy; I.flag2 = s.flag2 != 0.0;
+  if (s.need_alg != 0.0) {
+    T_algebra(s.T, I, I.flag2 ? I.xmysq : I.ysq);
+    V_algebra(s.V, I);
   }
   if (I.flag2) {
-    swap_S(dg0dx, dg0dcb, dg0dy, I);
-    swap_T(Q, I);
-    swap_V(D, I.x, I.xmy, I.yorig, I.xsq, I.xmysq, I.ysq, I.cb, I.cborig);
+    swap_S(s.dg0dx, s.dg0dcb, s.dg0dy, I);
+    swap_T(s.T, I);
+    swap_V(s.V, I.x, I.xmy, I.yorig, I.xsq, I.xmysq, I.ysq, I.cb, I.cborig);
   }
-  out[0] = dg0dx; out[1] = dg0dy; out[2] = dg0dcb;
-  out[3] = Q.ell1; out[4] = Q.ell2; out[5] = Q.ell3;
-  out[6] = Q.dell1dx; out[7] = Q.dell1dy; out[8] = Q.dell1dcb;
-  out[9] = Q.dell2dx; out[10] = Q.dell2dy; out[11] = Q.dell2dcb;
-  out[12] = Q.dell3dx; out[13] = Q.dell3dy; out[14] = Q.dell3dcb;
-  out[15] = I.x; out[16] = I.yorig; out[17] = I.cborig; out[18] = I.xsq; out[19] = I.ysq; out[20] = I.xdoty;
-  out[21] = D.dg1dx; out[22] = D.dg2dx; out[23] = D.dg1dcb; out[24] = D.dg2dcb; out[25] = D.dg1dy;
-  out[26] = D.ddg1dxdx; out[27] = D.ddg2dxdx; out[28] = D.ddg1dxdy; out[29] = D.ddg2dxdy;
-  out[30] = D.ddg1dxdcb; out[31] = D.ddg2dxdcb; out[32] = D.ddg1dydcb; out[33] = D.ddg2dydcb;
-  out[34] = D.ddg1dcbdcb; out[35] = D.ddg2dcbdcb;
-  return 0;
+  const Tt &Q = s.T;
+  const Vt &D = s.V;
+  double o[36];
+  o[0] = s.dg0dx; o[1] = s.dg0dy; o[2] = s.dg0dcb;
+  o[3] = Q.ell1; o[4] = Q.ell2; o[5] = Q.ell3;
+  o[6] = Q.dell1dx; o[7] = Q.dell1dy; o[8] = Q.dell1dcb;
+  o[9] = Q.dell2dx; o[10] = Q.dell2dy; o[11] = Q.dell2dcb;
+  o[12] = Q.dell3dx; o[13] = Q.dell3dy; o[14] = Q.dell3dcb;
+  o[15] = I.x; o[16] = I.yorig; o[17] = I.cborig; o[18] = I.xsq; o[19] = I.ysq; o[20] = I.xdoty;
+  o[21] = D.dg1dx; o[22] = D.dg2dx; o[23] = D.dg1dcb; o[24] = D.dg2dcb; o[25] = D.dg1dy;
+  o[26] = D.ddg1dxdx; o[27] = D.ddg2dxdx; o[28] = D.ddg1dxdy; o[29] = D.ddg2dxdy;
+  o[30] = D.ddg1dxdcb; o[31] = D.ddg2dxdcb; o[32] = D.ddg1dydcb; o[33] = D.ddg2dydcb;
+  o[34] = D.ddg1dcbdcb; o[35] = D.ddg2dcbdcb;
+  for (int i = 0; i < 36; i++) m[i] = o[i];
 }
 
 /* ------------------------------------------------------------------ Taylor-table primitives */
@@ -961,8 +1005,30 @@ DEV int yeq0_prims(const DT &T, const double *xv, double *o)
 }
 
 /* ------------------------------------------------------------------ classification + sort */
-/* key 0: Taylor-table regimes, zero, out of grid; 1 + ix*nstpy + iy: packed path; after those,
- * one key per y node for the Taylor-x region */
+/* Atomic add on base[key] shared by the lanes of a wave that hold the same key: one atomic per
+ * distinct key; returns the old value plus this lane's rank among those lanes. */
+DEV unsigned wave_atomic_add(unsigned *base, unsigned key)
+{
+  const unsigned lane = __lane_id();
+  unsigned res = 0;
+  for (;;) {
+    const unsigned lead = __builtin_amdgcn_readfirstlane(key);
+    const unsigned long long same = __ballot(key == lead);
+    if (key == lead) {
+      const int first = __ffsll((long long)same) - 1;
+      const unsigned rank = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
+      unsigned old = 0;
+      if ((int)lane == first) old = atomicAdd(&base[lead], (unsigned)__popcll(same));
+      old = __builtin_amdgcn_readlane(old, first);
+      res = old + rank;
+      break;
+    }
+  }
+  return res;
+}
+
+/* key 0: special points, out of grid, grid edges; 1 + ix*nstpy + iy: packed path;
+ * 1 + nstpx*nstpy + iy: Taylor-x region */
 __global__ void k_classify(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
                            unsigned *__restrict__ key, unsigned *__restrict__ count)
 {
@@ -974,10 +1040,10 @@ __global__ void k_classify(DT T, size_t n, const double *__restrict__ x, const d
   if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
     const Inv I = set_inv(T, xv, yv);
     if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (unsigned)I.INVy.idx;
-    else if (I.x <= T.XX[T.nstpx - 1]) k = 1 + (unsigned)(I.INVx.idx * T.nstpy + I.INVy.idx);
+    else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1]) k = 1 + (unsigned)(I.INVx.idx * T.nstpy + I.INVy.idx);
   }
   key[i] = k;
-  atomicAdd(&count[k], 1u);
+  wave_atomic_add(count, k);
 }
 
 __global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
@@ -1011,7 +1077,7 @@ __global__ void k_scatter(size_t n, const unsigned *__restrict__ key, unsigned *
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const unsigned pos = atomicAdd(&offs[key[i]], 1u);
+  const unsigned pos = wave_atomic_add(offs, key[i]);
   perm[pos] = (unsigned)i;
 }
 
@@ -1028,6 +1094,7 @@ DEV void plan_tabd(const DT &T, int kind, const double *v, double *payload, unsi
 }
 
 /* L0(a,b) into slot `slot`: regime dispatch of reference kernels.c:82-115 */
+template <int CLS>
 DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot)
 {
   const bool az = is_zero(a), bz = is_zero(b);
@@ -1037,15 +1104,25 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
   if (bz) { plan_tabd(T, KIND_YEQ0, a, pl, kinds, slot); return; }
   if (x_is_y(a, b)) { plan_tabd(T, KIND_YEQ0_MULAM, a, pl, kinds, slot); return; }
   if (slot != 0) return; /* only the main slot is ever a standard evaluation */
-  if (std_eval(T, a, b, pl) == 0) kinds |= (unsigned)KIND_STD;
+  if (interp_eval<CLS>(T, a, b, (Stage1 *)pl) == 0) kinds |= (unsigned)KIND_STD;
 }
 
-__global__ void __launch_bounds__(256)
+/* Pair i of the sorted order (or of the input order when perm is null) inside the segment
+ * [lo, hi), lo = bounds[ilo] (0 when ilo < 0), hi = bounds[ihi] (n when ihi < 0). */
+/* occupancy: the packed-path kernel runs at two waves per SIMD (it then spills a few bytes), the
+ * scalar-path kernels keep all their registers at one */
+template <int CLS> struct InterpWaves { static constexpr int v = 1; };
+template <> struct InterpWaves<1> { static constexpr int v = 2; };
+
+template <int CLS>
+__global__ void __launch_bounds__(256, InterpWaves<CLS>::v)
 k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
-         const unsigned *__restrict__ perm, int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds)
+         const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi,
+         int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds)
 {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const size_t lo = ilo < 0 ? 0 : bounds[ilo], hi = ihi < 0 ? n : bounds[ihi];
+  const size_t i = lo + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
   const size_t p = perm ? perm[i] : i;
   double xv[4], yv[4];
   for (int k = 0; k < 4; k++) { xv[k] = x[4 * p + k]; yv[k] = y[4 * p + k]; }
@@ -1054,14 +1131,14 @@ k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict_
   unsigned kd = 0;
   const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
   double c1 = 0.0, c2 = 0.0, c3 = 0.0;
-  plan_L0(T, xv, yv, r, kd, 0);
+  plan_L0<CLS>(T, xv, yv, r, kd, 0);
   switch (which) {
   case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
-    plan_L0(T, xv, xv, r, kd, 1); plan_L0(T, yv, yv, r, kd, 2); c1 = -0.5; c2 = -0.5; break;
+    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, yv, yv, r, kd, 2); c1 = -0.5; c2 = -0.5; break;
   case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
-    plan_L0(T, z, yv, r, kd, 1); plan_L0(T, xv, z, r, kd, 2); c1 = -1.0; c2 = -1.0; break;
+    plan_L0<CLS>(T, z, yv, r, kd, 1); plan_L0<CLS>(T, xv, z, r, kd, 2); c1 = -1.0; c2 = -1.0; break;
   case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
-    plan_L0(T, xv, xv, r, kd, 1); plan_L0(T, z, xv, r, kd, 2); plan_L0(T, z, yv, r, kd, 3);
+    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, z, xv, r, kd, 2); plan_L0<CLS>(T, z, yv, r, kd, 3);
     c1 = -1.0; c2 = 1.0; c3 = -1.0; break;
   case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
     if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
@@ -1080,73 +1157,92 @@ k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict_
   kinds[p] = kd;
 }
 
+template __global__ void k_interp<0>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
+                                     int, int, double, double *, unsigned *);
+template __global__ void k_interp<1>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
+                                     int, int, double, double *, unsigned *);
+template __global__ void k_interp<2>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
+                                     int, int, double, double *, unsigned *);
+
 /* ------------------------------------------------------------------ tensor entries */
-/* S/T/V entry c of the standard path from the 36 primitives (reference chnr_dS.c:85-93,
- * chnr_dT.c:258-290, chnr_dV.c:277-359). c: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
-DEV double std_entry(const double *P, const double *xv, const double *yv, int c)
+/* Per-pair helper vectors of the standard path (shared by many tensor entries), each evaluated with
+ * the expression of the reference (chnr_dS.c:86-93, chnr_dT.c:259-263, chnr_dV.c:286-340):
+ *   0 xhat, 4 yhat, 8 c2v, 12 c4v, 16 D1, 20 D2, 24 D3 (= D5), 28 D4, 32 D6, 36 D7, 40 D8,
+ *   44 astuff[alpha], 48 bstuff[beta], 52/56/60 T_x coefficient vectors, 64/68/72 T_y ones,
+ *   76 c1[alpha][beta] of get_dv */
+#define NH 92
+DEV double std_helper(const double *P, const double *xv, const double *yv, int j)
 {
-  const double x = P[15], yorig = P[16], cborig = P[17], xsq = P[18], ysq = P[19], xdoty = P[20];
+  const double x = P[15], y = P[16], cb = P[17];
+  const int i = j & 3;
+  const double xh = xv[i] / x, yh = yv[i] / y;
+  if (j < 4) return xh;
+  if (j < 8) return yh;
+  const double c2 = (xh - cb * yh) / y, c4 = (yh - cb * xh) / x;
+  if (j < 12) return c2;
+  if (j < 16) return c4;
+  if (j < 20) return xv[i] * P[26] + yv[i] * P[27];
+  if (j < 24) return xv[i] * P[28] + yv[i] * P[29];
+  if (j < 28) return xv[i] * P[30] + yv[i] * P[31];
+  if (j < 32) return xv[i] * P[32] + yv[i] * P[33];
+  if (j < 36) return xv[i] * P[34] + yv[i] * P[35];
+  if (j < 40) return (xv[i] * P[21] + yv[i] * P[22]) / x;
+  if (j < 44) return (xv[i] * P[23] + yv[i] * P[24]) / x;
+  if (j < 48) return (P[21] + P[22]) * xh + (P[23] + P[24]) * c4;
+  if (j < 52) return (P[21] * xh + P[25] * yh + P[23] * (xh / y + yh / x - cb * (xh / x + yh / y)));
+  if (j < 56) return xh * P[6] + c4 * P[8];
+  if (j < 60) return xh * P[9] + c4 * P[11];
+  if (j < 64) return xh * P[12] + c4 * P[14];
+  if (j < 68) return yh * P[7] + c2 * P[8];
+  if (j < 72) return yh * P[10] + c2 * P[11];
+  if (j < 76) return yh * P[13] + c2 * P[14];
+  const int a = (j - 76) >> 2, b = (j - 76) & 3;
+  const double xa = xv[a] / x, xb = xv[b] / x, ya = yv[a] / y, yb = yv[b] / y;
+  const int d_ab = (a == b);
+  const double xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
+  return ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) / x + (d_ab + xayb * cb - xaxb - yayb) / y);
+}
+
+/* S/T/V entry c of the standard path: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
+DEV double std_entry(const double *P, const double *xv, const double *yv, const double *h, int c)
+{
   if (c >= 192) {
-    const int bet = c & 3;
-    const double yhat = yv[bet] / yorig, xhat = xv[bet] / x;
-    const double c2 = (xhat - cborig * yhat) / yorig, c4 = (yhat - cborig * xhat) / x;
-    if (c < 196) return xhat * P[0] + c4 * P[2];
-    return yhat * P[1] + c2 * P[2];
+    const int b = c & 3;
+    if (c < 196) return h[b] * P[0] + h[12 + b] * P[2];
+    return h[4 + b] * P[1] + h[8 + b] * P[2];
   }
   const int alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
+  const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
   if (c >= 64) {
-    const double c1 = yv[bet] / yorig, c3 = xv[bet] / x;
-    const double c2 = (c3 - cborig * c1) / yorig, c4 = (c1 - cborig * c3) / x;
-    const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
+    const double xsq = P[18], ysq = P[19], xdoty = P[20];
     const double t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
     const double t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
     const double t3 = xv[alf] * xv[dta] - d_ad * xsq / 4;
     const double t4 = yv[alf] * yv[dta] - d_ad * ysq / 4;
     const double t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * xdoty * d_ad;
-    const double ell1 = P[3], ell2 = P[4], ell3 = P[5];
-    if (c < 128)
-      return +t2 * ell1 + t1 * ell3 + t3 * (c3 * P[6] + c4 * P[8]) + t4 * (c3 * P[9] + c4 * P[11])
-             + t5 * (c3 * P[12] + c4 * P[14]);
-    return +t1 * ell2 + t2 * ell3 + t3 * (c1 * P[7] + c2 * P[8]) + t4 * (c1 * P[10] + c2 * P[11])
-           + t5 * (c1 * P[13] + c2 * P[14]);
+    if (c < 128) return +t2 * P[3] + t1 * P[5] + t3 * h[52 + bet] + t4 * h[56 + bet] + t5 * h[60 + bet];
+    return +t1 * P[4] + t2 * P[5] + t3 * h[64 + bet] + t4 * h[68 + bet] + t5 * h[72 + bet];
   }
-  /* get_dv (chnr_dV.c:277-359) with y = yorig, cb = cborig */
-  const double y = yorig, cb = cborig;
-  const double dg1dx = P[21], dg2dx = P[22], dg1dcb = P[23], dg2dcb = P[24], dg1dy = P[25];
-  const double xa = xv[alf] / x, xb = xv[bet] / x, ya = yv[alf] / y, yb = yv[bet] / y;
-  const double c4a = (ya - cb * xa) / x;
-  const double c2b = (xb - cb * yb) / y, c4b = (yb - cb * xb) / x;
-  const double D1 = xv[dta] * P[26] + yv[dta] * P[27];
-  const double D2 = xv[dta] * P[28] + yv[dta] * P[29];
-  const double D3 = xv[dta] * P[30] + yv[dta] * P[31];
-  const double D4 = xv[dta] * P[32] + yv[dta] * P[33];
-  const double D5 = xv[dta] * P[30] + yv[dta] * P[31];
-  const double D6 = xv[dta] * P[34] + yv[dta] * P[35];
-  const double D7 = (xv[dta] * dg1dx + yv[dta] * dg2dx) / x;
-  const double D8 = (xv[dta] * dg1dcb + yv[dta] * dg2dcb) / x;
-  const double astuff = (dg1dx + dg2dx) * xa + (dg1dcb + dg2dcb) * c4a;
-  const double bstuff = (dg1dx * xb + dg1dy * yb + dg1dcb * (xb / y + yb / x - cb * (xb / x + yb / y)));
-  const double c2c4 = c2b + c4b;
-  const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
-  const double xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
-  const double cc1 = ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) / x + (d_ab + xayb * cb - xaxb - yayb) / y);
-  double f = d_bd * astuff;
-  f += d_ad * bstuff;
-  f += D1 * xaxb + D2 * xayb;
-  f += (D3 * xb + D4 * yb) * c4a;
-  f += (D5 * xa + D6 * c4a) * c2c4;
-  f += D7 * (d_ab - xaxb);
-  return f + D8 * cc1;
+  const double xa = h[alf], xb = h[bet], ya = h[4 + alf], yb = h[4 + bet];
+  const double c4a = h[12 + alf], c2c4 = h[8 + bet] + h[12 + bet];
+  const double xaxb = xa * xb, xayb = xa * yb;
+  double f = d_bd * h[44 + alf];
+  f += d_ad * h[48 + bet];
+  f += h[16 + dta] * xaxb + h[20 + dta] * xayb;
+  f += (h[24 + dta] * xb + h[28 + dta] * yb) * c4a;
+  f += (h[24 + dta] * xa + h[32 + dta] * c4a) * c2c4;
+  f += h[36 + dta] * (d_ab - xaxb);
+  return f + h[40 + dta] * h[76 + 4 * alf + bet];
 }
 
-/* Tabd_xeq0 tensors (reference Tabd.c:71-111). c: 0-63 tI, 64-127 tII, 128-191 tIII */
-DEV double xeq0_entry(const double *P, int c)
+/* Tabd_xeq0 tensors (reference Tabd.c:71-111); h = y/|y|.  c: 0-63 tI, 64-127 tII, 128-191 tIII */
+DEV double xeq0_entry(const double *P, const double *h, int c)
 {
   const double *yv = P;
   const double y = P[4], ysq = P[5], fx = P[6], fy = P[7], ell2 = P[8], dell2adx = P[9], dell2ady = P[10];
   const double ell3a = P[11], dg2dx = P[12], ddg2dxdy = P[13], dg1dx = P[14], dg1dy = P[15], phi1 = P[16], phi2 = P[17];
   const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
-  const double ha = yv[alf] / y, hb = yv[bet] / y, hd = yv[dta] / y;
+  const double ha = h[alf], hb = h[bet], hd = h[dta];
   const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
   const double ysq_4 = ysq / 4.;
   const double E = (ell3a + ell2), D2 = (dell2adx + dell2ady), D1 = (dg1dy + dg1dx), D12 = (dg1dx + dg2dx),
@@ -1165,14 +1261,14 @@ DEV double xeq0_entry(const double *P, int c)
   return (dbd * (ha * D12) + (hd * dg2dx) * dab + (hb * D3) * ha * hd + (hb * D1) * dad + yv[dta] * tIsum);
 }
 
-/* Tabd_yeq0 tensors (reference Tabd.c:190-217, loop form) */
-DEV double yeq0_entry(const double *P, int c)
+/* Tabd_yeq0 tensors (reference Tabd.c:190-217, loop form); h = x/|x| */
+DEV double yeq0_entry(const double *P, const double *h, int c)
 {
   const double *xv = P;
-  const double x = P[4], xsq = P[5], fx = P[6], fy = P[7], ell1 = P[8], ell3 = P[9], dell1dx = P[10];
+  const double xsq = P[5], fx = P[6], fy = P[7], ell1 = P[8], ell3 = P[9], dell1dx = P[10];
   const double dell1dycb = P[11], dg1dx = P[12], ddg1dxdx = P[13], dg2dx = P[14], dg1dycb = P[15], ddg1dxdycb = P[16];
   const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
-  const double ha = xv[alf] / x, hb = xv[bet] / x, hd = xv[dta] / x;
+  const double ha = h[alf], hb = h[bet], hd = h[dta];
   const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
   const double xsq_4 = xsq / 4., E = ell1 + ell3, D1 = dell1dx + dell1dycb, D2 = (ddg1dxdx + ddg1dxdycb),
                D3 = (dg1dycb - dg2dx), D4 = (dg1dx + dg1dycb);
@@ -1192,6 +1288,8 @@ DEV double yeq0_entry(const double *P, int c)
 /* ------------------------------------------------------------------ assembly kernel */
 #define NB 16
 #define ASM_THREADS 384
+#define KSTD 14 /* longest row of the standard map (13) */
+#define KEQ 10  /* longest row of the Tabd maps (9) */
 
 DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r,
                    const double *Tv)
@@ -1202,15 +1300,47 @@ DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, co
   return s;
 }
 
+template <int K>
+struct RowMap {
+  int c[K];
+  float v[K];
+  int n;
+  DEV void load(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r)
+  {
+    const int b = rp[r];
+    n = rp[r + 1] - b;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const bool ok = k < n;
+      c[k] = ok ? col[b + k] : 0;
+      v[k] = ok ? (float)val[b + k] : 0.0f;
+    }
+  }
+  DEV double dot(const double *Tv) const
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (k < n) s = fma((double)v[k], Tv[c[k]], s);
+    return s;
+  }
+};
+
 __global__ void __launch_bounds__(ASM_THREADS)
 k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, double *__restrict__ out,
-           int layout)
+           int layout, int nslots)
 {
   __shared__ double sR[NB][KQED_REC_LEN];
+  __shared__ double sH[NB][NH];
   __shared__ double sT[NB][200];
   __shared__ unsigned sK[NB];
   const int tid = threadIdx.x;
   const int row = tid; /* one output component per thread */
+  RowMap<KSTD> mS;
+  RowMap<KEQ> mE, mM;
+  mS.load(T.rowptr[0], T.col[0], T.val[0], row);
+  mE.load(T.rowptr[1], T.col[1], T.val[1], row);
+  mM.load(T.rowptr[2], T.col[2], T.val[2], row);
   for (size_t base = (size_t)blockIdx.x * NB; base < n; base += (size_t)gridDim.x * NB) {
     const int nb = (int)min((size_t)NB, n - base);
     __syncthreads();
@@ -1219,7 +1349,18 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
     double acc[NB];
 #pragma unroll
     for (int p = 0; p < NB; p++) acc[p] = 0.0;
-    for (int slot = 0; slot < 4; slot++) {
+    for (int slot = 0; slot < nslots; slot++) {
+      __syncthreads();
+      /* per-pair helper vectors of this slot */
+      for (int i = tid; i < nb * NH; i += ASM_THREADS) {
+        const int p = i / NH, j = i % NH;
+        const unsigned kd = (sK[p] >> (3 * slot)) & 7u;
+        const double *pl = slot == 0 ? &sR[p][KQED_REC_MAIN] : &sR[p][KQED_REC_AUX + KQED_AUX_LEN * (slot - 1)];
+        double v = 0.0;
+        if (kd == KIND_STD) v = std_helper(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], j);
+        else if (kd != KIND_NONE && j < 4) v = pl[j] / pl[4];
+        sH[p][j] = v;
+      }
       __syncthreads();
       /* tensors of this slot for every pair */
       for (int i = tid; i < nb * 200; i += ASM_THREADS) {
@@ -1227,9 +1368,9 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         const unsigned kd = (sK[p] >> (3 * slot)) & 7u;
         const double *pl = slot == 0 ? &sR[p][KQED_REC_MAIN] : &sR[p][KQED_REC_AUX + KQED_AUX_LEN * (slot - 1)];
         double v = 0.0;
-        if (kd == KIND_STD) v = std_entry(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], c);
-        else if (c < 192 && (kd == KIND_XEQ0 || kd == KIND_XEQ0_M)) v = xeq0_entry(pl, c);
-        else if (c < 192 && (kd == KIND_YEQ0 || kd == KIND_YEQ0_MULAM || kd == KIND_YEQ0_M)) v = yeq0_entry(pl, c);
+        if (kd == KIND_STD) v = std_entry(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], sH[p], c);
+        else if (c < 192 && (kd == KIND_XEQ0 || kd == KIND_XEQ0_M)) v = xeq0_entry(pl, sH[p], c);
+        else if (c < 192 && (kd == KIND_YEQ0 || kd == KIND_YEQ0_MULAM || kd == KIND_YEQ0_M)) v = yeq0_entry(pl, sH[p], c);
         sT[p][c] = v;
       }
       __syncthreads();
@@ -1237,27 +1378,26 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       for (int p = 0; p < NB; p++) {
         const unsigned kd = p < nb ? (sK[p] >> (3 * slot)) & 7u : (unsigned)KIND_NONE;
         if (kd != KIND_NONE) {
-        const int m = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
-        const double v = row_dot(T.rowptr[m], T.col[m], T.val[m], row, sT[p]);
-        if (slot == 0) {
-          acc[p] = v;
-        } else if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
-          /* reference kernels.c:292-323 */
-          const double M = sR[p][KQED_REC_M];
-          const int rs = row >> 6, mu = (row >> 4) & 3, nu = (row >> 2) & 3, la = row & 3;
-          double k4[4];
-          if (kd == KIND_XEQ0_M) {
-            const double *xv = &sR[p][KQED_REC_X];
-            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, sT[p]);
-            acc[p] -= sR[p][KQED_REC_GX] * (v - M * xv[mu] * (xv[0] * k4[0] + xv[1] * k4[1] + xv[2] * k4[2] + xv[3] * k4[3]));
+          const double v = kd == KIND_STD ? mS.dot(sT[p]) : kd == KIND_YEQ0_MULAM ? mM.dot(sT[p]) : mE.dot(sT[p]);
+          if (slot == 0) {
+            acc[p] = v;
+          } else if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
+            /* reference kernels.c:292-323 */
+            const double M = sR[p][KQED_REC_M];
+            const int rs = row >> 6, mu = (row >> 4) & 3, nu = (row >> 2) & 3, la = row & 3;
+            double k4[4];
+            if (kd == KIND_XEQ0_M) {
+              const double *xv = &sR[p][KQED_REC_X];
+              for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, sT[p]);
+              acc[p] -= sR[p][KQED_REC_GX] * (v - M * xv[mu] * (xv[0] * k4[0] + xv[1] * k4[1] + xv[2] * k4[2] + xv[3] * k4[3]));
+            } else {
+              const double *yv = &sR[p][KQED_REC_Y];
+              for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, sT[p]);
+              acc[p] -= sR[p][KQED_REC_GY] * (v - M * yv[nu] * (yv[0] * k4[0] + yv[1] * k4[1] + yv[2] * k4[2] + yv[3] * k4[3]));
+            }
           } else {
-            const double *yv = &sR[p][KQED_REC_Y];
-            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, sT[p]);
-            acc[p] -= sR[p][KQED_REC_GY] * (v - M * yv[nu] * (yv[0] * k4[0] + yv[1] * k4[1] + yv[2] * k4[2] + yv[3] * k4[3]));
+            acc[p] = fma(v, sR[p][KQED_REC_COEF + slot], acc[p]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
           }
-        } else {
-          acc[p] = fma(v, sR[p][KQED_REC_COEF + slot], acc[p]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
-        }
         }
       }
     }
