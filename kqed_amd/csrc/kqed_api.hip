@@ -9,6 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <math.h>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -155,6 +157,57 @@ extern "C" void free_QED_temps(struct QED_kernel_temps *t)
   memset(t, 0, sizeof *t);
 }
 
+/* Deal the rows of each assembly map to the lanes of a wavefront (see kqed_device.h). */
+static int build_wave_maps(kqed_device *d)
+{
+  kqed_dev_tables &T = d->dt;
+  std::vector<unsigned short> E, R(3 * 384);
+  for (int q = 0; q < 3; q++) {
+    const kqed_map &m = d->maps[q];
+    int order[384];
+    for (int r = 0; r < 384; r++) order[r] = r;
+    std::stable_sort(order, order + 384, [&](int a, int b) {
+      return m.rowptr[a + 1] - m.rowptr[a] > m.rowptr[b + 1] - m.rowptr[b];
+    });
+    for (int sl = 0; sl < 6; sl++) {
+      T.wmOff[q][sl] = (int)E.size();
+      int K = 0;
+      for (int l = 0; l < 64; l++) {
+        const int r = order[64 * sl + l];
+        K = std::max(K, m.rowptr[r + 1] - m.rowptr[r]);
+        R[q * 384 + 64 * sl + l] = (unsigned short)r;
+      }
+      T.wmK[q][sl] = K;
+      for (int k = 0; k < K; k++)
+        for (int l = 0; l < 64; l++) {
+          const int r = order[64 * sl + l], b = m.rowptr[r], len = m.rowptr[r + 1] - b;
+          unsigned short e = 200 | (1u << 8); /* zero slot, coefficient +1 */
+          if (k < len) {
+            const double v = m.val[b + k];
+            const double a = fabs(v);
+            const int ex = a == 0.5 ? 0 : a == 1.0 ? 1 : a == 2.0 ? 2 : -1;
+            if (ex < 0 || m.col[b + k] >= 200) { set_err("assembly coefficient %g not encodable", v); return 1; }
+            e = (unsigned short)(m.col[b + k] | (ex << 8) | ((v < 0) << 10));
+          }
+          E.push_back(e);
+        }
+    }
+    T.wmOff[q][6] = (int)E.size();
+  }
+  T.wmTotal = (int)E.size();
+  if (T.wmTotal > KQED_WM_MAX) { set_err("assembly maps too large for LDS"); return 1; }
+  unsigned short *e = (unsigned short *)dev_alloc(d, 2 * E.size());
+  unsigned short *r = (unsigned short *)dev_alloc(d, 2 * R.size());
+  if (!e || !r || hipMemcpy(e, E.data(), 2 * E.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(r, R.data(), 2 * R.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_err("wave map upload failed");
+    return 1;
+  }
+  T.wmE = e;
+  T.wmRow = r;
+  return 0;
+}
+
 extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
 {
   memset(t, 0, sizeof *t);
@@ -220,6 +273,7 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     d->maps = (kqed_map *)calloc(3, sizeof(kqed_map));
     if (!d->maps || kqed_build_maps(&d->maps[0], &d->maps[1], &d->maps[2])) { set_err("assembly map build failed"); goto fail; }
     if (d->maps[0].kmax > 14 || d->maps[1].kmax > 10 || d->maps[2].kmax > 10) { set_err("assembly map rows too long"); goto fail; }
+    if (build_wave_maps(d)) goto fail;
     for (int m = 0; m < 3; m++) {
       int *rp = (int *)dev_alloc(d, 4 * 385);
       short *cl = (short *)dev_alloc(d, 2 * (size_t)d->maps[m].nnz);
@@ -333,11 +387,11 @@ extern "C" int kqed_batch(const struct QED_kernel_temps *t, int which, double M,
     hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
     double *ob = layout == KQED_AOS ? out_dev + 384 * b : out_dev;
-    const size_t nblk = (c + 15) / 16;
+    const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
     prof_begin(d, s, ev, ST_ASSEMBLE);
     static const int nslots[5] = { 1, 3, 3, 4, 3 };
-    hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 8192 ? nblk : 8192)), dim3(384), 0, s, T, c, w.rec, w.kinds,
-                       ob, layout, nslots[which]);
+    hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 16384 ? nblk : 16384)), dim3(64 * KQED_ASM_WAVES), 0, s, T, c,
+                       w.rec, w.kinds, ob, layout, nslots[which]);
     prof_end(d, s, ev);
   }
   const hipError_t e = hipGetLastError();

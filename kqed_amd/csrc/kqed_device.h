@@ -32,6 +32,9 @@
 #define KQED_REC_AUX 80   /* slots 1..3 payloads [20] each */
 #define KQED_AUX_LEN 20
 
+#define KQED_WM_MAX 7168    /* u16 entries of the three wave-dealt assembly maps (7104 used) */
+#define KQED_ASM_WAVES 8    /* wavefronts per assembly block */
+
 enum {
   KIND_NONE = 0,
   KIND_STD = 1,       /* standard / Taylor / swap path: S, T, V tensors (200 inputs) */
@@ -52,10 +55,18 @@ struct kqed_dev_tables {
   const double2 *P;
   const double *PNC;
   const double2 *S;
-  /* assembly maps: 0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign */
+  /* assembly maps: 0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign (CSR) */
   const int *rowptr[3];
   const short *col[3];
   const double *val[3];
+  /* the same maps dealt to the 64 lanes of a wavefront: rows sorted by length, row 64m + l of the
+   * sorted order goes to lane l, slot m; entries [m][k][lane] as u16 = col | exp << 8 | sign << 10
+   * with coefficient (sign ? -1 : 1) * 2^(exp-1); padded entries read the zero slot (col 200) */
+  const unsigned short *wmE;   /* all three maps, concatenated */
+  const unsigned short *wmRow; /* [3][6][64] output row of (map, slot, lane) */
+  int wmOff[3][7];             /* start of slot m of map q in wmE; wmOff[q][6] = end */
+  int wmK[3][6];               /* entries per lane in slot m of map q */
+  int wmTotal;
 };
 
 #endif

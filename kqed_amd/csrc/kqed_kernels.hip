@@ -1286,11 +1286,33 @@ DEV double yeq0_entry(const double *P, const double *h, int c)
 }
 
 /* ------------------------------------------------------------------ assembly kernel */
-#define NB 16
-#define ASM_THREADS 384
-#define KSTD 14 /* longest row of the standard map (13) */
-#define KEQ 10  /* longest row of the Tabd maps (9) */
+/* One wavefront per pair: the pair's record, helper vectors, tensors and its 384 outputs live in the
+ * wavefront's own LDS area; the three assembly maps are shared by the block.  Lane l, slot m
+ * computes output row wmRow[q][m][l] of map q; outputs are combined per slot in LDS and stored
+ * as 384 contiguous doubles (6 coalesced 512-byte stores per lane set). */
+#define WREC KQED_REC_LEN
+#define WH 92
+#define WT 384
 
+DEV void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+DEV double wm_dot(const unsigned short *__restrict__ e, int K, int lane, const double *Tv)
+{
+  double s = 0.0;
+  for (int k = 0; k < K; k++) {
+    const unsigned w = e[64 * k + lane];
+    const double c = ldexp((w & 0x400u) ? -1.0 : 1.0, (int)((w >> 8) & 3u) - 1);
+    s = fma(c, Tv[w & 0xFFu], s);
+  }
+  return s;
+}
+
+/* CSR row of map q (for the M-L2 alpha sums, which need rows other than the lane's own) */
 DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r,
                    const double *Tv)
 {
@@ -1300,113 +1322,82 @@ DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, co
   return s;
 }
 
-template <int K>
-struct RowMap {
-  int c[K];
-  float v[K];
-  int n;
-  DEV void load(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r)
-  {
-    const int b = rp[r];
-    n = rp[r + 1] - b;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-      const bool ok = k < n;
-      c[k] = ok ? col[b + k] : 0;
-      v[k] = ok ? (float)val[b + k] : 0.0f;
-    }
-  }
-  DEV double dot(const double *Tv) const
-  {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < K; k++)
-      if (k < n) s = fma((double)v[k], Tv[c[k]], s);
-    return s;
-  }
-};
-
-__global__ void __launch_bounds__(ASM_THREADS)
+__global__ void __launch_bounds__(64 * KQED_ASM_WAVES)
 k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, double *__restrict__ out,
            int layout, int nslots)
 {
-  __shared__ double sR[NB][KQED_REC_LEN];
-  __shared__ double sH[NB][NH];
-  __shared__ double sT[NB][200];
-  __shared__ unsigned sK[NB];
-  const int tid = threadIdx.x;
-  const int row = tid; /* one output component per thread */
-  RowMap<KSTD> mS;
-  RowMap<KEQ> mE, mM;
-  mS.load(T.rowptr[0], T.col[0], T.val[0], row);
-  mE.load(T.rowptr[1], T.col[1], T.val[1], row);
-  mM.load(T.rowptr[2], T.col[2], T.val[2], row);
-  for (size_t base = (size_t)blockIdx.x * NB; base < n; base += (size_t)gridDim.x * NB) {
-    const int nb = (int)min((size_t)NB, n - base);
-    __syncthreads();
-    for (int i = tid; i < nb * KQED_REC_LEN; i += ASM_THREADS) sR[i / KQED_REC_LEN][i % KQED_REC_LEN] = rec[base * KQED_REC_LEN + i];
-    if (tid < nb) sK[tid] = kinds[base + tid];
-    double acc[NB];
-#pragma unroll
-    for (int p = 0; p < NB; p++) acc[p] = 0.0;
+  __shared__ unsigned short sE[KQED_WM_MAX];
+  __shared__ unsigned short sRow[3 * 384];
+  __shared__ double sW[KQED_ASM_WAVES][WREC + WH + 201 + WT];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < T.wmTotal; i += blockDim.x) sE[i] = T.wmE[i];
+  for (int i = threadIdx.x; i < 3 * 384; i += blockDim.x) sRow[i] = T.wmRow[i];
+  __syncthreads();
+  double *R = sW[w], *H = R + WREC, *Tv = H + WH, *O = Tv + 201;
+  const size_t stride = (size_t)gridDim.x * KQED_ASM_WAVES;
+  for (size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w; p < n; p += stride) {
+    for (int i = lane; i < WREC; i += 64) R[i] = rec[p * WREC + i];
+    const unsigned kall = __builtin_amdgcn_readfirstlane(kinds[p]);
+    if (lane == 0) Tv[200] = 0.0;
+    for (int i = lane; i < 384; i += 64) O[i] = 0.0;
+    wave_sync();
     for (int slot = 0; slot < nslots; slot++) {
-      __syncthreads();
-      /* per-pair helper vectors of this slot */
-      for (int i = tid; i < nb * NH; i += ASM_THREADS) {
-        const int p = i / NH, j = i % NH;
-        const unsigned kd = (sK[p] >> (3 * slot)) & 7u;
-        const double *pl = slot == 0 ? &sR[p][KQED_REC_MAIN] : &sR[p][KQED_REC_AUX + KQED_AUX_LEN * (slot - 1)];
-        double v = 0.0;
-        if (kd == KIND_STD) v = std_helper(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], j);
-        else if (kd != KIND_NONE && j < 4) v = pl[j] / pl[4];
-        sH[p][j] = v;
+      const unsigned kd = (kall >> (3 * slot)) & 7u;
+      if (kd == KIND_NONE) continue;
+      const double *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
+      if (kd == KIND_STD) {
+        for (int j = lane; j < WH; j += 64) H[j] = std_helper(pl, R + KQED_REC_X, R + KQED_REC_Y, j);
+      } else if (lane < 4) {
+        H[lane] = pl[lane] / pl[4];
       }
-      __syncthreads();
-      /* tensors of this slot for every pair */
-      for (int i = tid; i < nb * 200; i += ASM_THREADS) {
-        const int p = i / 200, c = i % 200;
-        const unsigned kd = (sK[p] >> (3 * slot)) & 7u;
-        const double *pl = slot == 0 ? &sR[p][KQED_REC_MAIN] : &sR[p][KQED_REC_AUX + KQED_AUX_LEN * (slot - 1)];
-        double v = 0.0;
-        if (kd == KIND_STD) v = std_entry(pl, &sR[p][KQED_REC_X], &sR[p][KQED_REC_Y], sH[p], c);
-        else if (c < 192 && (kd == KIND_XEQ0 || kd == KIND_XEQ0_M)) v = xeq0_entry(pl, sH[p], c);
-        else if (c < 192 && (kd == KIND_YEQ0 || kd == KIND_YEQ0_MULAM || kd == KIND_YEQ0_M)) v = yeq0_entry(pl, sH[p], c);
-        sT[p][c] = v;
+      wave_sync();
+      if (kd == KIND_STD) {
+        for (int c = lane; c < 200; c += 64) Tv[c] = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+      } else if (kd == KIND_XEQ0 || kd == KIND_XEQ0_M) {
+        for (int c = lane; c < 192; c += 64) Tv[c] = xeq0_entry(pl, H, c);
+      } else {
+        for (int c = lane; c < 192; c += 64) Tv[c] = yeq0_entry(pl, H, c);
       }
-      __syncthreads();
+      wave_sync();
+      const int q = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
+      double v[6];
 #pragma unroll
-      for (int p = 0; p < NB; p++) {
-        const unsigned kd = p < nb ? (sK[p] >> (3 * slot)) & 7u : (unsigned)KIND_NONE;
-        if (kd != KIND_NONE) {
-          const double v = kd == KIND_STD ? mS.dot(sT[p]) : kd == KIND_YEQ0_MULAM ? mM.dot(sT[p]) : mE.dot(sT[p]);
-          if (slot == 0) {
-            acc[p] = v;
-          } else if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
-            /* reference kernels.c:292-323 */
-            const double M = sR[p][KQED_REC_M];
-            const int rs = row >> 6, mu = (row >> 4) & 3, nu = (row >> 2) & 3, la = row & 3;
-            double k4[4];
-            if (kd == KIND_XEQ0_M) {
-              const double *xv = &sR[p][KQED_REC_X];
-              for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, sT[p]);
-              acc[p] -= sR[p][KQED_REC_GX] * (v - M * xv[mu] * (xv[0] * k4[0] + xv[1] * k4[1] + xv[2] * k4[2] + xv[3] * k4[3]));
-            } else {
-              const double *yv = &sR[p][KQED_REC_Y];
-              for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, sT[p]);
-              acc[p] -= sR[p][KQED_REC_GY] * (v - M * yv[nu] * (yv[0] * k4[0] + yv[1] * k4[1] + yv[2] * k4[2] + yv[3] * k4[3]));
-            }
+      for (int m = 0; m < 6; m++) v[m] = wm_dot(sE + T.wmOff[q][m], T.wmK[q][m], lane, Tv);
+      if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
+        /* reference kernels.c:292-323: the alpha sums take other rows of L0(0,y) or L0(x,0),
+         * recomputed from the tensors with the row-ordered map */
+        const double Mv = R[KQED_REC_M];
+        const double g = kd == KIND_XEQ0_M ? R[KQED_REC_GX] : R[KQED_REC_GY];
+        const double *vv = kd == KIND_XEQ0_M ? R + KQED_REC_X : R + KQED_REC_Y;
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+          const int r = sRow[q * 384 + 64 * m + lane];
+          const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
+          double k4[4];
+          if (kd == KIND_XEQ0_M) {
+            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, Tv);
+            O[r] -= g * (v[m] - Mv * vv[mu] * (vv[0] * k4[0] + vv[1] * k4[1] + vv[2] * k4[2] + vv[3] * k4[3]));
           } else {
-            acc[p] = fma(v, sR[p][KQED_REC_COEF + slot], acc[p]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
+            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, Tv);
+            O[r] -= g * (v[m] - Mv * vv[nu] * (vv[0] * k4[0] + vv[1] * k4[1] + vv[2] * k4[2] + vv[3] * k4[3]));
           }
         }
-      }
-    }
+      } else {
+        const double cf = R[KQED_REC_COEF + slot];
 #pragma unroll
-    for (int p = 0; p < NB; p++) {
-      if (p < nb) {
-        if (layout == 0) out[(base + p) * 384 + row] = acc[p];
-        else out[(size_t)row * n + base + p] = acc[p];
+        for (int m = 0; m < 6; m++) {
+          const int r = sRow[q * 384 + 64 * m + lane];
+          O[r] = slot == 0 ? v[m] : fma(v[m], cf, O[r]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
+        }
       }
+      wave_sync();
     }
+    if (layout == 0) {
+      double *o = out + p * 384;
+      for (int i = lane; i < 384; i += 64) o[i] = O[i];
+    } else {
+      for (int i = lane; i < 384; i += 64) out[(size_t)i * n + p] = O[i];
+    }
+    wave_sync();
   }
 }
