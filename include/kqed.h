@@ -84,6 +84,15 @@ int kqed_batch(const struct QED_kernel_temps *t, int which, double M, size_t n,
 int kqed_batch_host(const struct QED_kernel_temps *t, int which, double M, size_t n,
                     const double *x, const double *y, double *out, int layout);
 
+/* Contraction with a per-site tensor: for j < ny,
+ *   s[j] = sum_{i<nx} sum_{c<384} L(x_i, y_j)_c * T_i[c],   L = L0..L3 or M-L2 per `which`,
+ * x_dev[nx][4] FP64 and T_dev[nx][384] FP32 on the device, y[ny][4] and s[ny] FP64 on the host.
+ * Accumulation is FP64 with a fixed summation order (deterministic).  Synchronous: returns after s
+ * is written, 0 or a negative error code.  The reference has no library routine for this; it is
+ * the pattern of its examples (reference amu_forlattice.c:99-107, pihat . kerv summed per point). */
+int kqed_contract(const struct QED_kernel_temps *t, int which, double M, size_t nx, const double *x_dev,
+                  const float *T_dev, size_t ny, const double *y, double *s, void *stream);
+
 /* Bytes of device scratch kqed_batch needs for n pairs (allocated and cached by the library). */
 size_t kqed_batch_workspace(size_t n);
 

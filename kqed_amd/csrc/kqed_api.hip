@@ -23,15 +23,24 @@
 /* kernels (kqed_kernels.hip) */
 __global__ void k_build_arrays(kqed_dev_tables T, const float *Fm, const float *Fp, double2 *P, double *PNC, double2 *S);
 __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
-__global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, const double *y, unsigned *key, unsigned *count);
+__global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, unsigned *key,
+                           unsigned *rank, unsigned *count);
 __global__ void k_scan(unsigned *count, unsigned nb);
-__global__ void k_scatter(size_t n, const unsigned *key, unsigned *offs, unsigned *perm);
+__global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm);
 __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
 template <int CLS>
-__global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, const double *y, const unsigned *perm,
-                         const unsigned *bounds, int ilo, int ihi, int which, double M, double *rec, unsigned *kinds);
+__global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys,
+                         const unsigned *perm, const unsigned *bounds, int ilo, int ihi, size_t c0, size_t c1, int which,
+                         double M, double *rec, unsigned *kinds);
 __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out, int layout,
-                           int nslots);
+                           int nslots, size_t ntot, const float *Tc, const double *Wc);
+__global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
+                                   double *W);
+__global__ void k_block_sum(size_t n, size_t per, const double *in, double *out);
+__global__ void k_colsum(size_t nx, size_t per, const float *Tc, double *part);
+__global__ void k_colsum_final(int nb, const double *part, double *sum);
+__global__ void k_combine(int ny, const double *sigma, const double *La, double ca, const double *sumT,
+                          const double *consts, double cc0, double cc1, double *s);
 
 #ifndef KQED_DEFAULT_TABLE_DIR
 #define KQED_DEFAULT_TABLE_DIR "/usr/local/share/kqed/PRECOMP"
@@ -63,9 +72,13 @@ static const size_t CHUNK = (size_t)1 << 20;
 static const size_t SORT_MIN = 2048;
 
 struct Workspace {
-  size_t cap = 0, nbuckets = 0;
-  unsigned *key = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
+  size_t cap = 0, ncap = 0, nbuckets = 0;
+  unsigned *key = nullptr, *rank = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
   double *rec = nullptr;
+  /* contraction: per-pair dots [xcap], per-site pulled-back tensors W [xcap][200], and the small
+   * reduction buffers (RED_LEN doubles + 4 * ycap * 384 + ...) */
+  size_t xcap = 0, ycap = 0;
+  double *dots = nullptr, *W = nullptr, *red = nullptr;
 };
 
 /* optional per-stage timing: events recorded on the launch stream around every kernel */
@@ -78,6 +91,10 @@ struct kqed_device {
   void *allocs[32];
   int nalloc = 0;
   kqed_map *maps = nullptr;
+  /* the standard assembly map by column (for the contraction pull-back) */
+  int *csc_cp = nullptr;
+  short *csc_rows = nullptr;
+  double *csc_vals = nullptr;
   std::mutex mu;
   std::unordered_map<void *, Workspace> ws;
   bool profiling = false;
@@ -134,7 +151,8 @@ static void *dev_alloc(kqed_device *d, size_t bytes)
 
 static void free_ws(Workspace &w)
 {
-  (void)hipFree(w.key); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds); (void)hipFree(w.rec);
+  (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
+  (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
   w = Workspace();
 }
 
@@ -233,7 +251,7 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     T.nstpx = h->nstpx;
     T.nstpy = h->nstpy;
     T.nfmax = h->nfmax;
-    T.ks = h->nfmax + 2;
+    T.ks = h->nfmax + 2 + KQED_CLEN_BLOCK - 1;
     const size_t nodes = (size_t)h->nstpx * h->nstpy;
     const size_t nraw = (size_t)KQED_NFFA * nodes * h->nfmax;
     double *XX = (double *)dev_alloc(d, 8 * h->nstpx), *YY = (double *)dev_alloc(d, 8 * h->nstpy);
@@ -284,6 +302,29 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
       HIPCHK(hipMemcpy(vl, d->maps[m].val, 8 * (size_t)d->maps[m].nnz, hipMemcpyHostToDevice));
       T.rowptr[m] = rp; T.col[m] = cl; T.val[m] = vl;
     }
+    {
+      /* standard map by column: entries of column c in row order */
+      const kqed_map &m0 = d->maps[0];
+      std::vector<int> cp(201, 0);
+      std::vector<short> rows(m0.nnz);
+      std::vector<double> vals(m0.nnz);
+      for (int e = 0; e < m0.nnz; e++) cp[m0.col[e] + 1]++;
+      for (int c = 0; c < 200; c++) cp[c + 1] += cp[c];
+      std::vector<int> fill(cp.begin(), cp.end() - 1);
+      for (int r = 0; r < 384; r++)
+        for (int e = m0.rowptr[r]; e < m0.rowptr[r + 1]; e++) {
+          const int k = fill[m0.col[e]]++;
+          rows[k] = (short)r;
+          vals[k] = m0.val[e];
+        }
+      d->csc_cp = (int *)dev_alloc(d, 4 * 201);
+      d->csc_rows = (short *)dev_alloc(d, 2 * (size_t)m0.nnz);
+      d->csc_vals = (double *)dev_alloc(d, 8 * (size_t)m0.nnz);
+      if (!d->csc_cp || !d->csc_rows || !d->csc_vals) { set_err("device allocation failed"); goto fail; }
+      HIPCHK(hipMemcpy(d->csc_cp, cp.data(), 4 * 201, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d->csc_rows, rows.data(), 2 * (size_t)m0.nnz, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d->csc_vals, vals.data(), 8 * (size_t)m0.nnz, hipMemcpyHostToDevice));
+    }
   }
   /* host-visible mirror of the grid (reference Grid_coeffs fields) */
   t->Grid.Nffa = KQED_NFFA;
@@ -324,23 +365,90 @@ extern "C" int initialise(struct QED_kernel_temps *t)
 extern "C" size_t kqed_batch_workspace(size_t n)
 {
   const size_t c = n < CHUNK ? n : CHUNK;
-  return c * (KQED_REC_LEN * 8 + 16) + 4 * (1 + 100 * 4096 + 4096);
+  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (1 + 100 * 4096 + 4096);
 }
 
-static int ensure_ws(kqed_device *d, Workspace &w, size_t c)
+/* records for one pass of c pairs; sort keys and permutation for the whole batch of n pairs */
+static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
 {
   const size_t nb = 1 + (size_t)d->dt.nstpx * d->dt.nstpy + d->dt.nstpy;
-  if (w.cap >= c && w.nbuckets == nb) return 0;
-  free_ws(w);
-  if (hipMalloc(&w.key, 4 * c) != hipSuccess || hipMalloc(&w.perm, 4 * c) != hipSuccess ||
-      hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * nb) != hipSuccess ||
-      hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
-    free_ws(w);
-    set_err("workspace allocation of %zu pairs failed", c);
-    return -2;
+  if (w.cap < c || w.nbuckets != nb) {
+    (void)hipFree(w.kinds); (void)hipFree(w.rec); (void)hipFree(w.count);
+    w.kinds = w.count = nullptr; w.rec = nullptr; w.cap = 0;
+    if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * nb) != hipSuccess ||
+        hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
+      set_err("workspace allocation of %zu pairs failed", c);
+      return -2;
+    }
+    w.cap = c;
+    w.nbuckets = nb;
   }
-  w.cap = c;
-  w.nbuckets = nb;
+  if (w.ncap < n) {
+    (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.perm);
+    w.key = w.rank = w.perm = nullptr; w.ncap = 0;
+    if (hipMalloc(&w.key, 4 * n) != hipSuccess || hipMalloc(&w.rank, 4 * n) != hipSuccess ||
+        hipMalloc(&w.perm, 4 * n) != hipSuccess) {
+      set_err("sort workspace allocation of %zu pairs failed", n);
+      return -2;
+    }
+    w.ncap = n;
+  }
+  return 0;
+}
+
+/* The pipeline over n pairs (x[xs*i..], y[ys*i..]), lock held: one cell sort of the whole batch,
+ * then passes of CHUNK pairs (interpolation per regime class, chain-rule algebra, assembly). */
+static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, double M, size_t n, const double *x_dev,
+                     int xs, const double *y_dev, int ys, double *out_dev, int layout, const float *Tc,
+                     const double *Wc)
+{
+  if (n == 0) return 0;
+  const bool sort = n >= SORT_MIN;
+  if (ensure_ws(d, w, n < CHUNK ? n : CHUNK, sort ? n : 0)) return -2;
+  const kqed_dev_tables &T = d->dt;
+  const unsigned *perm = nullptr;
+  StageEvent ev;
+  if (sort) {
+    /* one counting sort of the whole batch by interpolation cell; passes then walk the sorted
+     * order, so every pass sees the densest possible cells */
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    prof_begin(d, s, ev, ST_SORT);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((w.nbuckets + 255) / 256)), dim3(256), 0, s, w.count, w.nbuckets, 0u);
+    hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, w.key, w.rank, w.count);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm);
+    perm = w.perm;
+    prof_end(d, s, ev);
+  }
+  static const int nslots[5] = { 1, 3, 3, 4, 3 };
+  for (size_t c0 = 0; c0 < n; c0 += CHUNK) {
+    const size_t c = n - c0 < CHUNK ? n - c0 : CHUNK, c1 = c0 + c;
+    const unsigned blocks = (unsigned)((c + 255) / 256);
+    prof_begin(d, s, ev, ST_INTERP);
+    if (perm) {
+      /* count[b] is the start of bucket b: special pairs [0, count[1]), packed path
+       * [count[1], count[1 + nstpx*nstpy]), Taylor-x region [count[1 + nstpx*nstpy], n) */
+      const int ip = 1 + T.nstpx * T.nstpy;
+      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
+                         which, M, w.rec, w.kinds);
+      hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
+                         which, M, w.rec, w.kinds);
+      hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
+                         which, M, w.rec, w.kinds);
+    } else {
+      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
+                         which, M, w.rec, w.kinds);
+    }
+    hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
+    prof_end(d, s, ev);
+    const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
+    prof_begin(d, s, ev, ST_ASSEMBLE);
+    hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 16384 ? nblk : 16384)), dim3(64 * KQED_ASM_WAVES), 0, s, T, c,
+                       w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc, Wc);
+    prof_end(d, s, ev);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
   return 0;
 }
 
@@ -350,52 +458,116 @@ extern "C" int kqed_batch(const struct QED_kernel_temps *t, int which, double M,
   if (!t || !t->dev) { set_err("kqed_batch: tables not initialised"); return -1; }
   if (which < 0 || which > 4 || (layout != KQED_AOS && layout != KQED_SOA)) { set_err("kqed_batch: bad argument"); return -1; }
   if (n == 0) return 0;
-  if (layout == KQED_SOA && n > CHUNK) { set_err("kqed_batch: SoA layout supports at most %zu pairs per call", CHUNK); return -1; }
+  if (n > 0xFFFFFFFFull) { set_err("kqed_batch: at most 2^32-1 pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  return run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
+                   nullptr);
+}
+
+/* contraction buffers: dots [nx], W [nx][200] (shortcut only) and the reduction scratch `red`:
+ * block sums [RED_BLOCKS] | column partials [RED_BLOCKS][384] | sum_x T [384] | consts [2] |
+ * zero 4-vector [4] | y [ny][4] | y-only kernels [ny][384] | sigma [ny] | result [ny] */
+static const size_t RED_BLOCKS = 1024;
+static size_t red_len(size_t ny) { return RED_BLOCKS + 384 * RED_BLOCKS + 384 + 2 + 4 + ny * (4 + 384 + 2); }
+
+static int ensure_contract_ws(Workspace &w, size_t nx, size_t ny, bool need_W)
+{
+  if (w.xcap < nx || w.ycap < ny || (need_W && !w.W)) {
+    (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
+    w.dots = w.W = w.red = nullptr;
+    w.xcap = w.ycap = 0;
+    if (hipMalloc(&w.dots, 8 * nx) != hipSuccess || hipMalloc(&w.red, 8 * red_len(ny)) != hipSuccess ||
+        (need_W && hipMalloc(&w.W, 8 * 200 * nx) != hipSuccess)) {
+      set_err("contraction workspace allocation (%zu sites) failed", nx);
+      return -2;
+    }
+    w.xcap = nx;
+    w.ycap = ny;
+  }
+  return 0;
+}
+
+/* deterministic sum of in[0, n) into *out (two fixed-order passes; tmp holds RED_BLOCKS doubles) */
+static void dev_sum(hipStream_t s, size_t n, const double *in, double *tmp, double *out)
+{
+  const size_t nb = std::min(RED_BLOCKS, (n + 255) / 256);
+  const size_t per = (n + nb - 1) / nb;
+  hipLaunchKernelGGL(k_block_sum, dim3((unsigned)nb), dim3(256), 0, s, n, per, in, tmp);
+  hipLaunchKernelGGL(k_block_sum, dim3(1), dim3(256), 0, s, nb, nb, tmp, out);
+}
+
+extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double M, size_t nx, const double *x_dev,
+                             const float *T_dev, size_t ny, const double *y, double *s_out, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_contract: tables not initialised"); return -1; }
+  if (which < 0 || which > 4 || (ny && (!y || !s_out)) || (nx && (!x_dev || !T_dev))) {
+    set_err("kqed_contract: bad argument");
+    return -1;
+  }
+  if (nx > 0xFFFFFFFFull || ny > 65536) { set_err("kqed_contract: at most 2^32-1 sites and 65536 y points"); return -1; }
+  if (ny == 0) return 0;
+  if (nx == 0) { for (size_t j = 0; j < ny; j++) s_out[j] = 0.0; return 0; }
   kqed_device *d = t->dev;
   hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> lk(d->mu);
   Workspace &w = d->ws[stream];
-  const size_t c0 = n < CHUNK ? n : CHUNK;
-  if (ensure_ws(d, w, c0)) return -2;
-  const kqed_dev_tables &T = d->dt;
-  for (size_t b = 0; b < n; b += CHUNK) {
-    const size_t c = n - b < CHUNK ? n - b : CHUNK;
-    const double *xb = x_dev + 4 * b, *yb = y_dev + 4 * b;
-    const unsigned blocks = (unsigned)((c + 255) / 256);
-    const unsigned *perm = nullptr;
-    StageEvent ev;
-    if (c >= SORT_MIN) {
-      prof_begin(d, s, ev, ST_SORT);
-      hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((w.nbuckets + 255) / 256)), dim3(256), 0, s, w.count, w.nbuckets, 0u);
-      hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, w.key, w.count);
-      hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets);
-      hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, c, w.key, w.count, w.perm);
-      perm = w.perm;
-      prof_end(d, s, ev);
-    }
-    prof_begin(d, s, ev, ST_INTERP);
-    if (perm) {
-      /* after the scatter, count[b] is the end of bucket b: special pairs [0, count[0]),
-       * packed path [count[0], count[nstpx*nstpy]), Taylor-x region [count[nstpx*nstpy], c) */
-      const int ip = T.nstpx * T.nstpy;
-      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, -1, 0, which, M, w.rec, w.kinds);
-      hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, 0, ip, which, M, w.rec, w.kinds);
-      hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, ip, -1, which, M, w.rec, w.kinds);
-    } else {
-      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, c, xb, yb, perm, w.count, -1, -1, which, M, w.rec, w.kinds);
-    }
-    hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
-    prof_end(d, s, ev);
-    double *ob = layout == KQED_AOS ? out_dev + 384 * b : out_dev;
-    const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
-    prof_begin(d, s, ev, ST_ASSEMBLE);
-    static const int nslots[5] = { 1, 3, 3, 4, 3 };
-    hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 16384 ? nblk : 16384)), dim3(64 * KQED_ASM_WAVES), 0, s, T, c,
-                       w.rec, w.kinds, ob, layout, nslots[which]);
-    prof_end(d, s, ev);
+  /* L0..L3 take the linear shortcut: sum_x L0(x,y).T(x) through the pulled-back site tensors
+   * W = A^T T, and the subtraction terms factor into a y-only kernel contracted with sum_x T(x)
+   * and an x-only sum computed once per call.  M-L2 contracts every pair in full. */
+  const bool shortcut = which != KQED_ML2;
+  if (ensure_contract_ws(w, nx, ny, shortcut)) return -2;
+  double *part = w.red, *colpart = part + RED_BLOCKS, *sumT = colpart + 384 * RED_BLOCKS, *consts = sumT + 384,
+         *zero4 = consts + 2, *yd = zero4 + 4, *La = yd + 4 * ny, *sigma = La + 384 * ny, *sres = sigma + ny;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, (unsigned *)consts, (size_t)12, 0u);
+  hipError_t e = hipMemcpyAsync(yd, y, 8 * 4 * ny, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) { set_err("kqed_contract: copy failed: %s", hipGetErrorString(e)); return -3; }
+  int rc = 0;
+  if (shortcut)
+    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)((nx * 200 + 255) / 256)), dim3(256), 0, s, nx, T_dev, d->csc_cp,
+                       d->csc_rows, d->csc_vals, w.W);
+  for (size_t j = 0; j < ny && !rc; j++) {
+    rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, 2, T_dev,
+                   shortcut ? w.W : nullptr);
+    if (!rc) dev_sum(s, nx, w.dots, part, sigma + j);
   }
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
+  double ca = 0.0, cc0 = 0.0, cc1 = 0.0;
+  bool with_y = false;
+  if (!rc && shortcut && which != KQED_L0) {
+    const size_t nb = std::min(RED_BLOCKS, nx), per = (nx + nb - 1) / nb;
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nb), dim3(384), 0, s, nx, per, T_dev, colpart);
+    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(384), 0, s, (int)nb, colpart, sumT);
+    with_y = true;
+    /* x-only sums over the sites: consts[k] = sum_x L0(a_k(x), b_k(x)) . T(x) */
+    const double *ax[2] = { nullptr, nullptr }, *bx[2] = { nullptr, nullptr };
+    int as[2] = { 4, 4 }, bs[2] = { 4, 4 };
+    if (which == KQED_L1) {
+      /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 (reference kernels.c:119-135) */
+      ax[0] = x_dev; bx[0] = x_dev; cc0 = -0.5; ca = -0.5;
+      rc = run_pairs(d, w, s, KQED_L0, 0.0, ny, yd, 4, yd, 4, La, KQED_AOS, nullptr, nullptr);
+    } else if (which == KQED_L2) {
+      /* L2 = L0(x,y) - L0(0,y) - L0(x,0) (kernels.c:138-155) */
+      ax[0] = x_dev; bx[0] = zero4; bs[0] = 0; cc0 = -1.0; ca = -1.0;
+      rc = run_pairs(d, w, s, KQED_L0, 0.0, ny, zero4, 0, yd, 4, La, KQED_AOS, nullptr, nullptr);
+    } else {
+      /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) (kernels.c:158-177) */
+      ax[0] = x_dev; bx[0] = x_dev; cc0 = -1.0;
+      ax[1] = zero4; as[1] = 0; bx[1] = x_dev; cc1 = 1.0; ca = -1.0;
+      rc = run_pairs(d, w, s, KQED_L0, 0.0, ny, zero4, 0, yd, 4, La, KQED_AOS, nullptr, nullptr);
+    }
+    for (int k = 0; k < 2 && !rc; k++) {
+      if (!ax[k]) continue;
+      rc = run_pairs(d, w, s, KQED_L0, 0.0, nx, ax[k], as[k], bx[k], bs[k], w.dots, 2, T_dev, nullptr);
+      if (!rc) dev_sum(s, nx, w.dots, part, consts + k);
+    }
+  }
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_combine, dim3((unsigned)ny), dim3(64), 0, s, (int)ny, sigma, with_y ? La : nullptr, ca, sumT,
+                     with_y ? consts : nullptr, cc0, cc1, sres);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(s_out, sres, 8 * ny, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) { set_err("kqed_contract: %s", hipGetErrorString(e)); return -3; }
   return 0;
 }
 

@@ -9,10 +9,11 @@
  *   S   [14][nstpx][nstpy][ks] double2 : Clenshaw inputs of the scalar interpolation path (Taylor
  *                                        region and grid edges), .x = value, .y = d/dy
  *   LUT [ks][8] double                 : Clenshaw derivative ratios, slot 2d-1/2d for order d
- * ks = nfmax + 2 (longest Clenshaw: nfx + mm, mm <= 2).
+ * ks = nfmax + 2 + KQED_CLEN_BLOCK - 1: the longest Clenshaw (nfx + mm, mm <= 2) rounded up to whole
+ * blocks of KQED_CLEN_BLOCK steps; the slots past a node's length are zero.
  *
  * Per-pair record written by the interpolation kernel and read by the assembly kernel:
- *   rec[REC_LEN] doubles: x[4], y[4], coef[4], M, gaussX, gaussY, pad, main payload [64],
+ *   rec[REC_LEN] doubles: x[4], y[4], coef[4], M, gaussX, gaussY, input index, main payload [64],
  *   three Tabd payloads [20], pad
  *   kinds (int): 3 bits per evaluation slot (KIND_*)
  */
@@ -28,6 +29,8 @@
 #define KQED_REC_M 12     /* M of M-L2 */
 #define KQED_REC_GX 13    /* exp(-M x^2 / 2) */
 #define KQED_REC_GY 14    /* exp(-M y^2 / 2) */
+#define KQED_CLEN_BLOCK 8  /* Clenshaw steps per block of coefficient loads (at most) */
+#define KQED_REC_P 15     /* input index of the pair */
 #define KQED_REC_MAIN 16  /* slot 0 payload [64]: stage-1 record, then the 36 primitives */
 #define KQED_REC_AUX 80   /* slots 1..3 payloads [20] each */
 #define KQED_AUX_LEN 20

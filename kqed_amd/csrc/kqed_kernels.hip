@@ -2,7 +2,7 @@
  * kqed_kernels.hip -- HIP kernels (gfx950) for the position-space QED kernel L0..L3 / M-L2, FP64.
  *
  * Pipeline for a batch of (x,y) pairs (driven by kqed_api.hip):
- *   k_classify   regime + interpolation-cell key per pair
+ *   k_classify   regime + interpolation-cell key per pair, and its slot inside the cell
  *   k_count / k_scan / k_scatter   counting sort of the pairs by cell (table locality)
  *   k_interp     one thread per pair, in cell order: invariants, weight-function interpolation
  *                (Clenshaw sums over the precomputed per-node arrays), the chain-rule algebra of
@@ -205,39 +205,82 @@ __global__ void k_build_arrays(DT T, const float *__restrict__ Fm, const float *
 }
 
 /* ------------------------------------------------------------------ Clenshaw sums */
-/* Packed-path Clenshaw (reference simd.c:79-118, 182-336, FMA build) over one node's arrays for
- * orders O0 .. O0+NO-1 of both the value (.x) and d/dy (.y) arrays in one pass.  Zero coefficients
- * past a node's length leave every chain at exact zero, so each node runs at its own length. */
-template <int O0, int NO>
-DEV void clen_node(const double2 *__restrict__ co, int nk, double twox, const double *__restrict__ LUT,
-                   double res[NO][2])
+/* Wave-uniform maximum of v over the active lanes (ballot/readlane; the loop runs once per distinct
+ * value above the first lane's).  The result lives in a scalar register. */
+DEV int wave_max_active(int v)
 {
-  double ya[NO][2], yb[NO][2];
+  int m = __builtin_amdgcn_readfirstlane(v);
+  for (;;) {
+    const unsigned long long gt = __ballot(v > m);
+    if (!gt) return m;
+    m = __builtin_amdgcn_readlane(v, __builtin_ctzll(gt));
+  }
+}
+
+/* The ratio table read through the constant address space: with a wave-uniform index the loads
+ * become scalar loads, so the per-step ratios cost no vector-memory bandwidth. */
+typedef __attribute__((address_space(4))) const double cdouble;
+DEV const cdouble *lut_ptr(const double *p) { return (const cdouble *)p; }
+
+/* One packed-path Clenshaw step at index k (reference simd.c:79-118, 182-336, FMA build) for order d
+ * on the chain pair (ya, yb); the ratios of order d > 0 come from the ratio table. */
+template <int D>
+DEV void clen_step(double &ya, double &yb, double c, double twox, double PA, double PB)
+{
+  const double t = yb;
+  if (D == 0) {
+    ya = ya - c;
+    yb = fma(twox, yb, -ya);
+  } else {
+    ya = fma(PB, ya, -c);
+    yb = yb * twox;
+    yb = fma(PA, yb, -ya);
+  }
+  ya = t;
+}
+
+/* Packed-path Clenshaw over the arrays of two grid nodes of one x node (y nodes iy and iy+1) for
+ * orders O0 .. O0+NO-1 of both the value (.x) and d/dy (.y) arrays in one pass.  nk is wave-uniform
+ * (the largest length among the wave's nodes) and rounded up to whole blocks of U steps: the arrays
+ * are zero past each node's own length (up to ks), and zero coefficients leave every chain at exact
+ * zero, so a node's sums do not depend on how far above its length the chain starts.  A block's
+ * U coefficient loads are issued together, so one memory latency covers U steps. */
+template <int O0, int NO, int U>
+DEV void clen_node2(const double2 *__restrict__ c1, const double2 *__restrict__ c2, int nk, double twox,
+                    const double *__restrict__ LUTg, double r1[NO][2], double r2[NO][2])
+{
+  static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
+  const cdouble *LUT = lut_ptr(LUTg);
+  double ya[2][NO][2], yb[2][NO][2];
 #pragma unroll
-  for (int i = 0; i < NO; i++) { ya[i][0] = ya[i][1] = yb[i][0] = yb[i][1] = 0.0; }
-  for (int k = nk - 1; k >= O0; --k) {
-    const double2 c = co[k];
-    const double *L = LUT + 8 * k;
+  for (int i = 0; i < NO; i++)
 #pragma unroll
-    for (int i = 0; i < NO; i++) {
-      const int d = O0 + i;
-      if (d > 0 && k < d) continue;
-      if (d == 0) {
-        double t0 = yb[i][0], t1 = yb[i][1];
-        ya[i][0] = ya[i][0] - c.x; ya[i][1] = ya[i][1] - c.y;
-        yb[i][0] = fma(twox, yb[i][0], -ya[i][0]);
-        yb[i][1] = fma(twox, yb[i][1], -ya[i][1]);
-        ya[i][0] = t0; ya[i][1] = t1;
-      } else {
-        const double PA = L[2 * d - 1], PB = L[2 * d];
-        double t0 = yb[i][0], t1 = yb[i][1];
-        ya[i][0] = fma(PB, ya[i][0], -c.x);
-        ya[i][1] = fma(PB, ya[i][1], -c.y);
-        yb[i][0] = yb[i][0] * twox;
-        yb[i][1] = yb[i][1] * twox;
-        yb[i][0] = fma(PA, yb[i][0], -ya[i][0]);
-        yb[i][1] = fma(PA, yb[i][1], -ya[i][1]);
-        ya[i][0] = t0; ya[i][1] = t1;
+    for (int a = 0; a < 2; a++) { ya[a][i][0] = ya[a][i][1] = yb[a][i][0] = yb[a][i][1] = 0.0; }
+  const int nsteps = nk - O0;
+  const int kt = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0) - 1;
+  for (int kb = kt; kb >= O0; kb -= U) {
+    double2 cc[U][2];
+#pragma unroll
+    for (int u = 0; u < U; u++) { cc[u][0] = c1[kb - u]; cc[u][1] = c2[kb - u]; }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = kb - u;
+#pragma unroll
+      for (int i = 0; i < NO; i++) {
+        const int d = O0 + i;
+        if (d > 0 && k < d) continue;
+        double PA = 0.0, PB = 0.0;
+        if (d > 0) { PA = LUT[8 * k + 2 * d - 1]; PB = LUT[8 * k + 2 * d]; }
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          if (d == 0) {
+            clen_step<0>(ya[a][i][0], yb[a][i][0], cc[u][a].x, twox, PA, PB);
+            clen_step<0>(ya[a][i][1], yb[a][i][1], cc[u][a].y, twox, PA, PB);
+          } else {
+            clen_step<1>(ya[a][i][0], yb[a][i][0], cc[u][a].x, twox, PA, PB);
+            clen_step<1>(ya[a][i][1], yb[a][i][1], cc[u][a].y, twox, PA, PB);
+          }
+        }
       }
     }
   }
@@ -245,27 +288,43 @@ DEV void clen_node(const double2 *__restrict__ co, int nk, double twox, const do
   for (int i = 0; i < NO; i++) {
     const int d = O0 + i;
     const double f = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
-    res[i][0] = d == 0 ? yb[i][0] : yb[i][0] * f;
-    res[i][1] = d == 0 ? yb[i][1] : yb[i][1] * f;
+    r1[i][0] = d == 0 ? yb[0][i][0] : yb[0][i][0] * f;
+    r1[i][1] = d == 0 ? yb[0][i][1] : yb[0][i][1] * f;
+    r2[i][0] = d == 0 ? yb[1][i][0] : yb[1][i][0] * f;
+    r2[i][1] = d == 0 ? yb[1][i][1] : yb[1][i][1] * f;
   }
 }
 
-/* single packed chain over a plain double array */
-DEV double clen_one(const double *__restrict__ co, int nk, int d, double twox, const double *__restrict__ LUT)
+/* two packed chains of one order over plain double arrays (same wave-uniform length, blocks of U) */
+template <int U>
+DEV void clen_two(const double *__restrict__ q1, const double *__restrict__ q2, int nk, int d, double twox,
+                  const double *__restrict__ LUTg, double &o1, double &o2)
 {
-  double ya = 0.0, yb = 0.0, t;
-  if (d == 0) {
-    for (int k = nk - 1; k >= 0; --k) { t = yb; ya = ya - co[k]; yb = fma(twox, yb, -ya); ya = t; }
-    return yb;
+  static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
+  const cdouble *LUT = lut_ptr(LUTg);
+  double ya1 = 0.0, yb1 = 0.0, ya2 = 0.0, yb2 = 0.0;
+  const int nsteps = nk - d;
+  const int kt = d + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0) - 1;
+  for (int kb = kt; kb >= d; kb -= U) {
+    double a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) { a[u] = q1[kb - u]; b[u] = q2[kb - u]; }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int k = kb - u;
+      if (d == 0) {
+        clen_step<0>(ya1, yb1, a[u], twox, 0.0, 0.0);
+        clen_step<0>(ya2, yb2, b[u], twox, 0.0, 0.0);
+      } else {
+        const double PA = LUT[8 * k + 2 * d - 1], PB = LUT[8 * k + 2 * d];
+        clen_step<1>(ya1, yb1, a[u], twox, PA, PB);
+        clen_step<1>(ya2, yb2, b[u], twox, PA, PB);
+      }
+    }
   }
-  for (int k = nk - 1; k >= d; --k) {
-    t = yb;
-    ya = fma(LUT[8 * k + 2 * d], ya, -co[k]);
-    yb = yb * twox;
-    yb = fma(LUT[8 * k + 2 * d - 1], yb, -ya);
-    ya = t;
-  }
-  return yb * (d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0);
+  const double f = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
+  o1 = d == 0 ? yb1 : yb1 * f;
+  o2 = d == 0 ? yb2 : yb2 * f;
 }
 
 /* scalar-path Clenshaw with divisions (reference cheby.c:13-91); comp selects .x or .y */
@@ -385,15 +444,15 @@ DEV void interpol5(double F[4], const Ip &P, const double f[12])
   t = f[0] * P.A; t = fma(f[1], P.B, t); t = fma(f[2], P.C1, t); t = fma(f[3], P.C2, t); F[3] = t * P.D;
 }
 
-/* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of one table at NO orders. */
+/* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of one table at NO orders.
+ * nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
 template <int O0, int NO>
-DEV void table_xnode(const DT &T, int nm, int ixn, const Inv &I, double twox, double H[NO], double Lv[NO])
+DEV void table_xnode(const DT &T, int nm, int ixn, int nfw, const Inv &I, double twox, double H[NO], double Lv[NO])
 {
-  const int mm = mm_of(nm);
-  const int nk = T.nfx[ixn] + mm;
+  const int nk = nfw + mm_of(nm);
   double r1[NO][2], r2[NO][2];
-  clen_node<O0, NO>(T.P + node(T, nm, ixn, I.INVy.idx), nk, twox, T.LUT, r1);
-  clen_node<O0, NO>(T.P + node(T, nm, ixn, I.INVy.idx + 1), nk, twox, T.LUT, r2);
+  const double2 *c1 = T.P + node(T, nm, ixn, I.INVy.idx);
+  clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(c1, c1 + T.ks, nk, twox, T.LUT, r1, r2);
 #pragma unroll
   for (int i = 0; i < NO; i++) {
     H[i] = hy(I.INVy, r1[i][0], r2[i][0], r1[i][1], r2[i][1]);
@@ -402,17 +461,17 @@ DEV void table_xnode(const DT &T, int nm, int ixn, const Inv &I, double twox, do
 }
 
 /* derivative-only variant (accessv6 with ndy, simd.c:616-650) + x step: extractff(nm, ndy=true) */
-DEV double ndy_packed(const DT &T, int nci_nm, int nci_dm, int nm, const Inv &I, double twox, int d)
+DEV double ndy_packed(const DT &T, int nci_nm, int nci_dm, int nm, const Inv &I, const int nfw[2], double twox, int d)
 {
   double f[4];
   for (int xn = 0; xn < 2; xn++) {
     const int ixn = I.INVx.idx + xn;
-    const int nk = T.nfx[ixn] + mm_of(nm);
+    const int nk = nfw[xn] + mm_of(nm);
     for (int t = 0; t < 2; t++) {
       const int nci = t == 0 ? nci_nm : nci_dm;
       const double *q1 = T.PNC + (((size_t)nci * T.nstpx + ixn) * T.nstpy + I.INVy.idx) * T.ks;
-      const double *q2 = q1 + T.ks;
-      const double a = clen_one(q1, nk, d, twox, T.LUT), b = clen_one(q2, nk, d, twox, T.LUT);
+      double a, b;
+      clen_two<8>(q1, q1 + T.ks, nk, d, twox, T.LUT, a, b);
       f[2 * t + xn] = ly(I.INVy, a, b);
     }
   }
@@ -432,14 +491,14 @@ DEV void request(const Ip &INVx, const double Hn[2][2], const double Ln[2], cons
 
 /* Simple groups: one request (nm, nm+6) at orders d, d+1 (d+1 < 4 always here). */
 template <int D>
-DEV void simple_group(const DT &T, int nm, const Inv &I, double twox, double F[4])
+DEV void simple_group(const DT &T, int nm, const Inv &I, const int nfw[2], double twox, double F[4])
 {
   double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
   for (int xn = 0; xn < 2; xn++) {
     double H[2], L[2];
-    table_xnode<D, 2>(T, nm, I.INVx.idx + xn, I, twox, H, L);
+    table_xnode<D, 2>(T, nm, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
     Hn[xn][0] = H[0]; Hn[xn][1] = H[1]; Ln[xn] = L[0];
-    table_xnode<D, 2>(T, nm + 6, I.INVx.idx + xn, I, twox, H, L);
+    table_xnode<D, 2>(T, nm + 6, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
     Hd[xn][0] = H[0]; Hd[xn][1] = H[1]; Ld[xn] = L[0];
   }
   request(I.INVx, Hn, Ln, Hd, Ld, F);
@@ -786,23 +845,26 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
   double F[4];
   if (CLS == 1 || (!flag_hx && !flag_hy)) {
     const double twox = 2. * I.cb;
+    /* Clenshaw lengths are made wave-uniform (zero padding keeps each node's sums exact), so the
+     * loops and ratio-table reads are scalar */
+    const int nfw[2] = { wave_max_active(T.nfx[I.INVx.idx]), wave_max_active(T.nfx[I.INVx.idx + 1]) };
     /* chnr_dS: QG0 */
-    simple_group<0>(T, QG0, I, twox, F);
+    simple_group<0>(T, QG0, I, nfw, twox, F);
     st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
     /* chnr_dT: QL2, QL4, QG1 */
-    simple_group<2>(T, QL2, I, twox, F);
+    simple_group<2>(T, QL2, I, nfw, twox, F);
     Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
-    simple_group<1>(T, QL4, I, twox, F);
+    simple_group<1>(T, QL4, I, nfw, twox, F);
     Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
-    simple_group<0>(T, QG1, I, twox, F);
+    simple_group<0>(T, QG1, I, nfw, twox, F);
     Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
     /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
     {
       double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
       for (int xn = 0; xn < 2; xn++) {
-        table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, I, twox, H2[xn], L2[xn]);
-        table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, I, twox, H8[xn], L8[xn]);
-        table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, I, twox, H12[xn], L12[xn]);
+        table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, nfw[xn], I, twox, H2[xn], L2[xn]);
+        table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, nfw[xn], I, twox, H8[xn], L8[xn]);
+        table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, nfw[xn], I, twox, H12[xn], L12[xn]);
       }
       double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
       /* extractff2(QG2, d0cb): d = 1 */
@@ -831,9 +893,9 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     {
       double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
       for (int xn = 0; xn < 2; xn++) {
-        table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, I, twox, H3[xn], L3[xn]);
-        table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, I, twox, H9[xn], L9[xn]);
-        table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, I, twox, H13[xn], L13[xn]);
+        table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, nfw[xn], I, twox, H3[xn], L3[xn]);
+        table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, nfw[xn], I, twox, H9[xn], L9[xn]);
+        table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, nfw[xn], I, twox, H13[xn], L13[xn]);
       }
       double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
       /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
@@ -852,13 +914,13 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
       D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
     }
     /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
-    const double dg3dy = ndy_packed(T, 2, 3, QG3, I, twox, 0);
+    const double dg3dy = ndy_packed(T, 2, 3, QG3, I, nfw, twox, 0);
     D.dg3dy = dg3dy;
     if (I.flag2) {
       /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
        * so they coincide unless the upper end sits on the top y node */
       const int iy1 = I.INVy.idx;
-      const double fq1 = ndy_packed(T, 0, 1, QG2, I, twox, 1);
+      const double fq1 = ndy_packed(T, 0, 1, QG2, I, nfw, twox, 1);
       double fq2 = fq1, gq2 = dg3dy;
       if (CLS == 0 && iy1 + 1 == T.nstpy - 1) {
         Inv I2 = I;
@@ -1029,13 +1091,13 @@ DEV unsigned wave_atomic_add(unsigned *base, unsigned key)
 
 /* key 0: special points, out of grid, grid edges; 1 + ix*nstpy + iy: packed path;
  * 1 + nstpx*nstpy + iy: Taylor-x region */
-__global__ void k_classify(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
-                           unsigned *__restrict__ key, unsigned *__restrict__ count)
+__global__ void k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
+                           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double xv[4], yv[4];
-  for (int k = 0; k < 4; k++) { xv[k] = x[4 * i + k]; yv[k] = y[4 * i + k]; }
+  for (int k = 0; k < 4; k++) { xv[k] = x[xs * i + k]; yv[k] = y[ys * i + k]; }
   unsigned k = 0;
   if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
     const Inv I = set_inv(T, xv, yv);
@@ -1043,7 +1105,7 @@ __global__ void k_classify(DT T, size_t n, const double *__restrict__ x, const d
     else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1]) k = 1 + (unsigned)(I.INVx.idx * T.nstpy + I.INVy.idx);
   }
   key[i] = k;
-  wave_atomic_add(count, k);
+  rank[i] = wave_atomic_add(count, k); /* slot of the pair inside its bucket */
 }
 
 __global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
@@ -1072,13 +1134,13 @@ __global__ void k_scan(unsigned *__restrict__ count, unsigned nb)
   for (unsigned b = lo; b < hi; b++) { const unsigned c = count[b]; count[b] = run; run += c; }
 }
 
-__global__ void k_scatter(size_t n, const unsigned *__restrict__ key, unsigned *__restrict__ offs,
-                          unsigned *__restrict__ perm)
+/* start[b] is the exclusive prefix of the bucket counts: pair i goes to start[key] + its rank */
+__global__ void k_scatter(size_t n, const unsigned *__restrict__ key, const unsigned *__restrict__ rank,
+                          const unsigned *__restrict__ start, unsigned *__restrict__ perm)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const unsigned pos = wave_atomic_add(offs, key[i]);
-  perm[pos] = (unsigned)i;
+  perm[start[key[i]] + rank[i]] = (unsigned)i;
 }
 
 /* ------------------------------------------------------------------ interpolation kernel */
@@ -1107,39 +1169,44 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
   if (interp_eval<CLS>(T, a, b, (Stage1 *)pl) == 0) kinds |= (unsigned)KIND_STD;
 }
 
-/* Pair i of the sorted order (or of the input order when perm is null) inside the segment
- * [lo, hi), lo = bounds[ilo] (0 when ilo < 0), hi = bounds[ihi] (n when ihi < 0). */
 /* occupancy: the packed-path kernel runs at two waves per SIMD (it then spills a few bytes), the
  * scalar-path kernels keep all their registers at one */
 template <int CLS> struct InterpWaves { static constexpr int v = 1; };
 template <> struct InterpWaves<1> { static constexpr int v = 2; };
 
+/* Sorted position i (input position when perm is null) in the window [c0, c1) of this pass and in
+ * the class segment [lo, hi), lo = bounds[ilo] (0 when ilo < 0), hi = bounds[ihi] (n when ihi < 0).
+ * The record of position i is rec[i - c0]; it carries the input index p.  Pair p is
+ * (x[xs*p .. +4], y[ys*p .. +4]): a stride of 0 repeats one vector for every pair. */
 template <int CLS>
 __global__ void __launch_bounds__(256, InterpWaves<CLS>::v)
-k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict__ y,
-         const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi,
+k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
+         const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi, size_t c0, size_t c1,
          int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds)
 {
-  const size_t lo = ilo < 0 ? 0 : bounds[ilo], hi = ihi < 0 ? n : bounds[ihi];
+  size_t lo = ilo < 0 ? 0 : bounds[ilo], hi = ihi < 0 ? n : bounds[ihi];
+  lo = lo > c0 ? lo : c0;
+  hi = hi < c1 ? hi : c1;
   const size_t i = lo + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= hi) return;
   const size_t p = perm ? perm[i] : i;
+  const size_t ri = i - c0;
   double xv[4], yv[4];
-  for (int k = 0; k < 4; k++) { xv[k] = x[4 * p + k]; yv[k] = y[4 * p + k]; }
-  double *r = rec + p * KQED_REC_LEN;
+  for (int k = 0; k < 4; k++) { xv[k] = x[xs * p + k]; yv[k] = y[ys * p + k]; }
+  double *r = rec + ri * KQED_REC_LEN;
   for (int k = 0; k < 4; k++) { r[KQED_REC_X + k] = xv[k]; r[KQED_REC_Y + k] = yv[k]; }
   unsigned kd = 0;
   const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
-  double c1 = 0.0, c2 = 0.0, c3 = 0.0;
+  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
   plan_L0<CLS>(T, xv, yv, r, kd, 0);
   switch (which) {
   case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
-    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, yv, yv, r, kd, 2); c1 = -0.5; c2 = -0.5; break;
+    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, yv, yv, r, kd, 2); s1 = -0.5; s2 = -0.5; break;
   case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
-    plan_L0<CLS>(T, z, yv, r, kd, 1); plan_L0<CLS>(T, xv, z, r, kd, 2); c1 = -1.0; c2 = -1.0; break;
+    plan_L0<CLS>(T, z, yv, r, kd, 1); plan_L0<CLS>(T, xv, z, r, kd, 2); s1 = -1.0; s2 = -1.0; break;
   case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
     plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, z, xv, r, kd, 2); plan_L0<CLS>(T, z, yv, r, kd, 3);
-    c1 = -1.0; c2 = 1.0; c3 = -1.0; break;
+    s1 = -1.0; s2 = 1.0; s3 = -1.0; break;
   case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
     if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
     if (!is_zero(xv)) plan_tabd(T, KIND_YEQ0_M, xv, r + KQED_REC_AUX + KQED_AUX_LEN, kd, 2);
@@ -1147,22 +1214,22 @@ k_interp(DT T, size_t n, const double *__restrict__ x, const double *__restrict_
   default: break;
   }
   r[KQED_REC_COEF + 0] = 1.0;
-  r[KQED_REC_COEF + 1] = c1;
-  r[KQED_REC_COEF + 2] = c2;
-  r[KQED_REC_COEF + 3] = c3;
+  r[KQED_REC_COEF + 1] = s1;
+  r[KQED_REC_COEF + 2] = s2;
+  r[KQED_REC_COEF + 3] = s3;
   r[KQED_REC_M] = M;
   r[KQED_REC_GX] = exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.);
   r[KQED_REC_GY] = exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.);
-  r[15] = 0.0;
-  kinds[p] = kd;
+  r[KQED_REC_P] = (double)p;
+  kinds[ri] = kd;
 }
 
-template __global__ void k_interp<0>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
-                                     int, int, double, double *, unsigned *);
-template __global__ void k_interp<1>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
-                                     int, int, double, double *, unsigned *);
-template __global__ void k_interp<2>(DT, size_t, const double *, const double *, const unsigned *, const unsigned *, int,
-                                     int, int, double, double *, unsigned *);
+template __global__ void k_interp<0>(DT, size_t, const double *, int, const double *, int, const unsigned *,
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
+template __global__ void k_interp<1>(DT, size_t, const double *, int, const double *, int, const unsigned *,
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
+template __global__ void k_interp<2>(DT, size_t, const double *, int, const double *, int, const unsigned *,
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
 
 /* ------------------------------------------------------------------ tensor entries */
 /* Per-pair helper vectors of the standard path (shared by many tensor entries), each evaluated with
@@ -1322,9 +1389,21 @@ DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, co
   return s;
 }
 
+/* FP64 sum over the 64 lanes of a wavefront (every lane active), result in every lane */
+DEV double wave_sum(double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+/* Output modes: layout 0 writes out[q][384], layout 1 out[384][ntot], layout 2 (contraction)
+ * writes the dot product out[q] = sum_c L_c Tc[q][c] with the FP32 site tensor Tc.  With Wc
+ * (layout 2 and a single standard L0 slot) the dot takes the linear shortcut
+ * sum_c L_c T_c = sum_j u_j W_j, u the 200 tensor entries and W = A^T T precomputed per site. */
 __global__ void __launch_bounds__(64 * KQED_ASM_WAVES)
 k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, double *__restrict__ out,
-           int layout, int nslots)
+           int layout, int nslots, size_t ntot, const float *__restrict__ Tc, const double *__restrict__ Wc)
 {
   __shared__ unsigned short sE[KQED_WM_MAX];
   __shared__ unsigned short sRow[3 * 384];
@@ -1341,6 +1420,9 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
     if (lane == 0) Tv[200] = 0.0;
     for (int i = lane; i < 384; i += 64) O[i] = 0.0;
     wave_sync();
+    const size_t q = (size_t)R[KQED_REC_P];
+    double dot = 0.0;
+    bool have_dot = false;
     for (int slot = 0; slot < nslots; slot++) {
       const unsigned kd = (kall >> (3 * slot)) & 7u;
       if (kd == KIND_NONE) continue;
@@ -1359,10 +1441,19 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         for (int c = lane; c < 192; c += 64) Tv[c] = yeq0_entry(pl, H, c);
       }
       wave_sync();
-      const int q = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
+      if (Wc && kd == KIND_STD) {
+        /* contraction shortcut: only slot 0 exists (nslots == 1) */
+        const double *w = Wc + q * 200;
+        double part = 0.0;
+        for (int c = lane; c < 200; c += 64) part = fma(Tv[c], w[c], part);
+        dot = wave_sum(part);
+        have_dot = true;
+        continue;
+      }
+      const int mq = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
       double v[6];
 #pragma unroll
-      for (int m = 0; m < 6; m++) v[m] = wm_dot(sE + T.wmOff[q][m], T.wmK[q][m], lane, Tv);
+      for (int m = 0; m < 6; m++) v[m] = wm_dot(sE + T.wmOff[mq][m], T.wmK[mq][m], lane, Tv);
       if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
         /* reference kernels.c:292-323: the alpha sums take other rows of L0(0,y) or L0(x,0),
          * recomputed from the tensors with the row-ordered map */
@@ -1371,7 +1462,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         const double *vv = kd == KIND_XEQ0_M ? R + KQED_REC_X : R + KQED_REC_Y;
 #pragma unroll
         for (int m = 0; m < 6; m++) {
-          const int r = sRow[q * 384 + 64 * m + lane];
+          const int r = sRow[mq * 384 + 64 * m + lane];
           const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
           double k4[4];
           if (kd == KIND_XEQ0_M) {
@@ -1386,18 +1477,98 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         const double cf = R[KQED_REC_COEF + slot];
 #pragma unroll
         for (int m = 0; m < 6; m++) {
-          const int r = sRow[q * 384 + 64 * m + lane];
+          const int r = sRow[mq * 384 + 64 * m + lane];
           O[r] = slot == 0 ? v[m] : fma(v[m], cf, O[r]); /* atomic_kernel_Saxpy, kernels.c:52-78 */
         }
       }
       wave_sync();
     }
     if (layout == 0) {
-      double *o = out + p * 384;
+      double *o = out + q * 384;
       for (int i = lane; i < 384; i += 64) o[i] = O[i];
+    } else if (layout == 1) {
+      for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = O[i];
     } else {
-      for (int i = lane; i < 384; i += 64) out[(size_t)i * n + p] = O[i];
+      if (!have_dot) {
+        const float *tq = Tc + q * 384;
+        double part = 0.0;
+        for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
+        dot = wave_sum(part);
+      }
+      if (lane == 0) out[q] = dot;
     }
     wave_sync();
+  }
+}
+
+/* ------------------------------------------------------------------ contraction helpers */
+/* W[i][c] = sum over the entries (r, v) of column c of the standard assembly map of v * T[i][r]:
+ * the site tensor pulled back through the linear assembly (columns in a fixed order, so the sums
+ * are deterministic).  One thread per (site, column). */
+__global__ void k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restrict__ cp,
+                                   const short *__restrict__ rows, const double *__restrict__ vals, double *__restrict__ W)
+{
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= nx * 200) return;
+  const size_t i = id / 200;
+  const int c = (int)(id % 200);
+  const float *t = Tc + i * 384;
+  double s = 0.0;
+  for (int e = cp[c]; e < cp[c + 1]; e++) s = fma(vals[e], (double)t[rows[e]], s);
+  W[id] = s;
+}
+
+/* Deterministic sum of in[0, n): block b reduces the contiguous range [b*per, (b+1)*per) in a fixed
+ * order into out[b] (launch with 256 threads). */
+__global__ void k_block_sum(size_t n, size_t per, const double *__restrict__ in, double *__restrict__ out)
+{
+  __shared__ double part[256];
+  const size_t lo = (size_t)blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+  double s = 0.0;
+  for (size_t i = lo + threadIdx.x; i < hi; i += 256) s += in[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+/* Column sums of the site tensors, FP64: block b sums rows [b*per, (b+1)*per) of column
+ * threadIdx.x into part[b][c] (384 threads). */
+__global__ void k_colsum(size_t nx, size_t per, const float *__restrict__ Tc, double *__restrict__ part)
+{
+  const int c = threadIdx.x;
+  const size_t lo = (size_t)blockIdx.x * per, hi = lo + per < nx ? lo + per : nx;
+  double s = 0.0;
+  for (size_t i = lo; i < hi; i++) s += (double)Tc[i * 384 + c];
+  part[(size_t)blockIdx.x * 384 + c] = s;
+}
+
+__global__ void k_colsum_final(int nb, const double *__restrict__ part, double *__restrict__ sum)
+{
+  const int c = threadIdx.x;
+  double s = 0.0;
+  for (int b = 0; b < nb; b++) s += part[(size_t)b * 384 + c];
+  sum[c] = s;
+}
+
+/* s[j] = sigma[j] + ca * (La[j] . sumT) + cc0 * consts[0] + cc1 * consts[1]  (one wavefront per j) */
+__global__ void k_combine(int ny, const double *__restrict__ sigma, const double *__restrict__ La, double ca,
+                          const double *__restrict__ sumT, const double *__restrict__ consts, double cc0, double cc1,
+                          double *__restrict__ s)
+{
+  const int j = blockIdx.x, lane = threadIdx.x;
+  if (j >= ny) return;
+  double part = 0.0;
+  if (La)
+    for (int c = lane; c < 384; c += 64) part = fma(La[(size_t)j * 384 + c], sumT[c], part);
+  const double dot = wave_sum(part);
+  if (lane == 0) {
+    double v = sigma[j];
+    if (La) v = fma(ca, dot, v);
+    if (consts) { v = fma(cc0, consts[0], v); v = fma(cc1, consts[1], v); }
+    s[j] = v;
   }
 }

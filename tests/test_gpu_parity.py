@@ -70,3 +70,25 @@ def test_soa_layout(gpu_small):
     a = gpu_small.batch_host(1, x, y, layout=kqed_amd.KQED_AOS)
     s = gpu_small.batch_host(1, x, y, layout=kqed_amd.KQED_SOA)
     assert np.array_equal(a, s.T)
+
+
+def test_soa_layout_sorted(gpu_full):
+    """SoA output of a cell-sorted batch: every pair lands in its own column."""
+    x, y = log_uniform(12, 5000)
+    a = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_AOS)
+    s = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_SOA)
+    assert np.array_equal(a, s.T)
+
+
+def test_multi_pass_batch(gpu_full, oracle_full):
+    """A batch longer than one pass (2^20 pairs): sorted once, walked in passes; results equal the
+    unsorted small-batch path pair by pair and the oracle on a sample."""
+    n = (1 << 20) + 3001
+    x, y = log_uniform(9, n)
+    got = gpu_full.batch_host(0, x, y)
+    rng = np.random.default_rng(4)
+    idx = np.concatenate([rng.choice(n, 1500, replace=False), np.arange((1 << 20) - 8, (1 << 20) + 8), [n - 1]])
+    small = gpu_full.batch_host(0, x[idx], y[idx])
+    assert normwise(got[idx], small).max() <= 1e-15
+    ref = oracle_full.batch(0, x[idx[:300]], y[idx[:300]])
+    assert normwise(got[idx[:300]], ref).max() <= TOL
