@@ -51,6 +51,27 @@ def cpu_baseline(table, x, y, budget_s=12.0):
             "sample": f"first {done} pairs of the workload, L0, {dt:.1f} s"}, flops, sample_out
 
 
+def contraction_leg(t, dev, stream, L):
+    """sum_x L2(x,y) . T(x) over an L^4 lattice for the 16 fixed y sites (FP32 T, FP64 accumulation)."""
+    import torch
+    from kqed_amd import workloads as W
+    x = W.lattice_sites(L)
+    T = W.site_tensors(x, seed=3)
+    y = W.y_sites()
+    xd = torch.from_numpy(x).to(dev)
+    Td = torch.from_numpy(T).to(dev)
+    del T
+    t.contract_device(2, len(x), xd.data_ptr(), Td.data_ptr(), y[:1], stream=stream)  # warm-up (workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = t.contract_device(2, len(x), xd.data_ptr(), Td.data_ptr(), y, stream=stream)
+    dt = time.perf_counter() - t0
+    del xd, Td
+    return {"workload": f"sum_x L2(x,y).T(x), {L}^4 lattice, a=0.0536, 16 y sites, FP32 T, FP64 accumulation",
+            "l2_evals": len(x) * len(y), "seconds": dt, "evals_per_s": len(x) * len(y) / dt,
+            "s_first": float(s[0])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,7 +80,8 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 24)
     ap.add_argument("--table", default="full")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates")
+    ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates and the contraction")
+    ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus})", file=sys.stderr)
@@ -114,6 +136,9 @@ def main():
             torch.cuda.synchronize()
             rates[name] = a.n / (time.perf_counter() - t1)
         extra["evals_per_s_other_kernels"] = rates
+
+    if not a.no_extra:
+        extra["contraction"] = contraction_leg(t, dev, stream, a.lattice)
 
     ms_i, cnt_i = stages["interp"]
     ms_a, cnt_a = stages["assemble"]

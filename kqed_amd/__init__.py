@@ -68,6 +68,8 @@ def lib():
     L.kqed_batch.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
     L.kqed_batch_host.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, arr, arr, arr, ctypes.c_int]
+    L.kqed_contract.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_size_t, arr, arr, ctypes.c_void_p]
     L.kqed_batch_workspace.argtypes = [ctypes.c_size_t]
     L.kqed_batch_workspace.restype = ctypes.c_size_t
     L.kqed_last_error.restype = ctypes.c_char_p
@@ -205,6 +207,25 @@ class QEDKernelTemps:
         lib().kqed_profile(ctypes.byref(self._t), int(bool(enable)), ms, cnt)
         names = ("sort", "interp", "assemble")
         return {n: (ms[i], cnt[i]) for i, n in enumerate(names)}
+
+    def contract_device(self, which, nx, x_ptr, T_ptr, y, M=0.0, stream=None):
+        """s[j] = sum_i L(x_i, y_j) . T_i for device x[nx][4] (FP64) and T[nx][384] (FP32); y on the host."""
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 4)
+        s = np.zeros(y.shape[0])
+        rc = lib().kqed_contract(ctypes.byref(self._t), int(which), float(M), int(nx), ctypes.c_void_p(x_ptr),
+                                 ctypes.c_void_p(T_ptr), y.shape[0], y.reshape(-1), s, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_contract failed: " + lib().kqed_last_error().decode())
+        return s
+
+    def contract(self, which, x, T, y, M=0.0):
+        """Host arrays in, host result out (the device copies are made with torch)."""
+        import torch
+        xd = torch.from_numpy(np.ascontiguousarray(x, np.float64)).to("cuda")
+        Td = torch.from_numpy(np.ascontiguousarray(T, np.float32)).to("cuda")
+        s = self.contract_device(which, xd.shape[0], xd.data_ptr(), Td.data_ptr(), y, M=M,
+                                 stream=torch.cuda.current_stream().cuda_stream)
+        return s
 
     def batch_device(self, which, n, x_ptr, y_ptr, out_ptr, M=0.0, layout=KQED_AOS, stream=None):
         """n pairs on device pointers (ints); enqueued on `stream` (an int hipStream_t or None)."""

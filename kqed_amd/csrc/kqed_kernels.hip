@@ -1443,9 +1443,9 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       wave_sync();
       if (Wc && kd == KIND_STD) {
         /* contraction shortcut: only slot 0 exists (nslots == 1) */
-        const double *w = Wc + q * 200;
+        const double *wq = Wc + q * 200;
         double part = 0.0;
-        for (int c = lane; c < 200; c += 64) part = fma(Tv[c], w[c], part);
+        for (int c = lane; c < 200; c += 64) part = fma(Tv[c], wq[c], part);
         dot = wave_sum(part);
         have_dot = true;
         continue;

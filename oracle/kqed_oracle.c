@@ -1493,3 +1493,50 @@ void oq_batch(const oq_tables *T, int which, double M, size_t n, const double *x
   }
   (void)nthreads;
 }
+
+/* Contraction of L(x_i, y_j) with per-site tensors Tc[i][384] (FP32), the pattern of reference
+ * amu_forlattice.c:99-107 (tmp += pihat[idx] * kerv[idx] in index order) summed over the sites:
+ *   s[j] = sum_i sum_c L(x_i,y_j)_c Tc[i][c],   a[j] = sum_i sum_c |L(x_i,y_j)_c Tc[i][c]|
+ * (a is the scale of the rounding error of any summation order).  Per-site dots are formed in
+ * parallel and summed in site order. */
+void oq_contract(const oq_tables *T, int which, double M, size_t nx, const double *x, const float *Tc, size_t ny,
+                 const double *y, double *s, double *a, int nthreads)
+{
+  maps_init();
+  double *dot = (double *)malloc(2 * nx * sizeof(double));
+  if (!dot) return;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#endif
+  for (size_t j = 0; j < ny; j++) {
+    const double *yj = y + 4 * j;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+#endif
+    for (size_t i = 0; i < nx; i++) {
+      double k[384];
+      const double *xi = x + 4 * i;
+      switch (which) {
+      case 0: oq_L0(T, xi, yj, k); break;
+      case 1: oq_L1(T, xi, yj, k); break;
+      case 2: oq_L2(T, xi, yj, k); break;
+      case 3: oq_L3(T, xi, yj, k); break;
+      default: oq_ML2(T, M, xi, yj, k); break;
+      }
+      double t = 0.0, u = 0.0;
+      for (int c = 0; c < 384; c++) {
+        const double v = k[c] * (double)Tc[384 * i + c];
+        t += v;
+        u += fabs(v);
+      }
+      dot[2 * i] = t;
+      dot[2 * i + 1] = u;
+    }
+    double st = 0.0, sa = 0.0;
+    for (size_t i = 0; i < nx; i++) { st += dot[2 * i]; sa += dot[2 * i + 1]; }
+    s[j] = st;
+    if (a) a[j] = sa;
+  }
+  free(dot);
+  (void)nthreads;
+}

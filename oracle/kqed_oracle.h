@@ -54,6 +54,9 @@ void oq_ML2(const oq_tables *T, double M, const double x[4], const double y[4], 
 /* Batched driver (OpenMP over pairs) for the CPU baseline: which = 0..3 for L0..L3, 4 = M-L2. */
 void oq_batch(const oq_tables *T, int which, double M, size_t n, const double *x, const double *y,
               double *out, int nthreads);
+/* contraction with per-site FP32 tensors (pattern of amu_forlattice.c:99-107), s[ny], a[ny] = sum |terms| */
+void oq_contract(const oq_tables *T, int which, double M, size_t nx, const double *x, const float *Tc, size_t ny,
+                 const double *y, double *s, double *a, int nthreads);
 
 /* Pieces exposed for unit tests. */
 double oq_chebU(int order, int nk, double x, const double *co);          /* cheby.c:13-91 */

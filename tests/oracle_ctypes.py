@@ -37,6 +37,9 @@ def _lib():
         getattr(L, n).argtypes = [T, v4, v4, v4]
     L.oq_ML2.argtypes = [T, ctypes.c_double, v4, v4, v4]
     L.oq_batch.argtypes = [T, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, v4, v4, v4, ctypes.c_int]
+    f4 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.oq_contract.argtypes = [T, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, v4, f4, ctypes.c_size_t, v4, v4, v4,
+                              ctypes.c_int]
     L.oq_chebU.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4]
     L.oq_chebU.restype = ctypes.c_double
     L.oq_chebU_avx.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4, T]
@@ -111,6 +114,18 @@ class Oracle:
         out = np.zeros((n, 384))
         self.L.oq_batch(ctypes.byref(self.t), int(which), float(M), n, x.ravel(), y.ravel(), out.ravel(), int(nthreads))
         return out
+
+    def contract(self, which, x, Tc, y, M=0.0, nthreads=0):
+        """s[j] = sum_i L(x_i, y_j) . Tc[i] and the error scale a[j] = sum_i sum_c |L_c Tc[i][c]|."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        Tc = np.ascontiguousarray(Tc, np.float32)
+        ny = y.shape[0]
+        s = np.zeros(ny)
+        a = np.zeros(ny)
+        self.L.oq_contract(ctypes.byref(self.t), int(which), float(M), x.shape[0], x.ravel(), Tc.ravel(), ny, y.ravel(),
+                           s, a, int(nthreads))
+        return s, a
 
     def stv(self, x, y):
         s = np.zeros(200)
