@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates and the contraction")
     ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus})", file=sys.stderr)
@@ -103,6 +105,13 @@ def main():
     def step(which=0, M=0.0, n=a.n):
         t.batch_device(which, n, xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=M, stream=stream)
 
+    if a.copy_calibration:
+        src = torch.empty(a.copy_calibration << 18, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(a.warmup):
         step()
     torch.cuda.synchronize()

@@ -68,7 +68,14 @@ void QED_Mkernel_L2(const double M, const double xv[4], const double yv[4],
 
 /* ---------------------------------------------------------------- batched entry points (new) */
 enum kqed_kernel { KQED_L0 = 0, KQED_L1 = 1, KQED_L2 = 2, KQED_L3 = 3, KQED_ML2 = 4 };
-enum kqed_layout { KQED_AOS = 0, /* out[n][384] */ KQED_SOA = 1 /* out[384][n] */ };
+/* Output layouts.  The FP32 layouts also run the sparse assembly maps in FP32 (interpolation, the
+ * Taylor branches and the L1..L3 / M-L2 subtractions stay FP64). */
+enum kqed_layout {
+  KQED_AOS = 0,     /* double out[n][384] */
+  KQED_SOA = 1,     /* double out[384][n] */
+  KQED_AOS_F32 = 2, /* float out[n][384] */
+  KQED_SOA_F32 = 3  /* float out[384][n] */
+};
 
 /* Initialise from an explicit table directory (initialise() with a path argument). */
 int kqed_initialise_dir(struct QED_kernel_temps *t, const char *table_dir);
@@ -80,9 +87,47 @@ int kqed_initialise_dir(struct QED_kernel_temps *t, const char *table_dir);
 int kqed_batch(const struct QED_kernel_temps *t, int which, double M, size_t n,
                const double *x_dev, const double *y_dev, double *out_dev, int layout, void *stream);
 
-/* Same on host buffers (copies in and out, synchronous). */
+/* Same on host buffers (copies in and out, synchronous); out is double or float per layout. */
 int kqed_batch_host(const struct QED_kernel_temps *t, int which, double M, size_t n,
-                    const double *x, const double *y, double *out, int layout);
+                    const double *x, const double *y, void *out, int layout);
+
+/* Ragged groups: group g pairs y_dev[g] with x_dev[offsets[g] .. offsets[g+1]) (offsets on the
+ * host, ngroups+1 entries, offsets[0] = 0); output row i belongs to pair i of the concatenation.
+ * Enqueued on `stream`; the offsets are copied before return. */
+int kqed_batch_ragged(const struct QED_kernel_temps *t, int which, double M, size_t ngroups,
+                      const size_t *offsets, const double *y_dev, const double *x_dev, void *out_dev,
+                      int layout, void *stream);
+
+/* The same ragged batch captured once into a HIP graph (own workspace): each kqed_graph_launch
+ * replays the whole pipeline as a single graph launch on `stream`.  The device buffers named at
+ * creation must stay valid; their contents may change between launches.  Launch parameters are
+ * those in force at creation (kqed_set_tune). */
+typedef struct kqed_graph kqed_graph;
+int kqed_graph_create_ragged(const struct QED_kernel_temps *t, int which, double M, size_t ngroups,
+                             const size_t *offsets, const double *y_dev, const double *x_dev,
+                             void *out_dev, int layout, kqed_graph **graph);
+int kqed_graph_launch(kqed_graph *graph, void *stream);
+void kqed_graph_destroy(kqed_graph *graph);
+
+/* Launch parameters of the pipeline (per initialised table set). */
+enum kqed_sort_mode { KQED_SORT_CELL = 0, /* by regime and (x node, y node) cell */
+                      KQED_SORT_XNODE = 1, /* by regime and x node */
+                      KQED_SORT_NONE = 2 };
+typedef struct {
+  size_t chunk;    /* pairs per pipeline pass (records of one pass stay in device memory) */
+  int asm_grid;    /* workgroups of the assembly kernel (each loops over its pairs) */
+  int sort_mode;   /* enum kqed_sort_mode */
+  size_t sort_min; /* batches below this size are not sorted */
+} kqed_tune;
+int kqed_get_tune(const struct QED_kernel_temps *t, kqed_tune *tune);
+int kqed_set_tune(const struct QED_kernel_temps *t, const kqed_tune *tune);
+
+/* Autotuner: times every candidate (pass length 2^17..2^21, assembly grid 4096..65536, sort by
+ * cell or x node) on the given batch, `reps` timed runs each, installs the fastest and returns it
+ * (and its time per run in ms). */
+int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
+                  const double *y_dev, void *out_dev, int layout, int reps, void *stream, kqed_tune *best,
+                  double *best_ms);
 
 /* Contraction with a per-site tensor: for j < ny,
  *   s[j] = sum_{i<nx} sum_{c<384} L(x_i, y_j)_c * T_i[c],   L = L0..L3 or M-L2 per `which`,

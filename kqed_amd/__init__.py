@@ -23,7 +23,8 @@ LIB_PATH = os.path.join(HERE, "libkqed_hip.so")
 GOLDEN_PRECOMP = os.path.join(ROOT, "tests", "golden", "PRECOMP")
 
 KQED_L0, KQED_L1, KQED_L2, KQED_L3, KQED_ML2 = range(5)
-KQED_AOS, KQED_SOA = 0, 1
+KQED_AOS, KQED_SOA, KQED_AOS_F32, KQED_SOA_F32 = 0, 1, 2, 3
+KQED_SORT_CELL, KQED_SORT_XNODE, KQED_SORT_NONE = 0, 1, 2
 
 _lib = None
 
@@ -39,6 +40,15 @@ class _Grid(ctypes.Structure):
 
 class _Temps(ctypes.Structure):
     _fields_ = [("Grid", _Grid), ("G8", ctypes.POINTER(ctypes.c_int)), ("dev", ctypes.c_void_p)]
+
+
+class Tune(ctypes.Structure):
+    """kqed_tune: pass length, assembly grid, sort granularity, sort threshold."""
+    _fields_ = [("chunk", ctypes.c_size_t), ("asm_grid", ctypes.c_int), ("sort_mode", ctypes.c_int),
+                ("sort_min", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def lib():
@@ -67,7 +77,22 @@ def lib():
     L.QED_Mkernel_L2.restype = None
     L.kqed_batch.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
-    L.kqed_batch_host.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, arr, arr, arr, ctypes.c_int]
+    L.kqed_batch_host.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, arr, arr, ctypes.c_void_p,
+                                  ctypes.c_int]
+    szp = np.ctypeslib.ndpointer(np.uintp, flags="C_CONTIGUOUS")
+    L.kqed_batch_ragged.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, szp, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    L.kqed_graph_create_ragged.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, szp, ctypes.c_void_p,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_void_p)]
+    L.kqed_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.kqed_graph_destroy.argtypes = [ctypes.c_void_p]
+    L.kqed_graph_destroy.restype = None
+    L.kqed_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
+    L.kqed_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
+    L.kqed_autotune.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
+                                ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(Tune),
+                                ctypes.POINTER(ctypes.c_double)]
     L.kqed_contract.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
                                 ctypes.c_size_t, arr, arr, ctypes.c_void_p]
     L.kqed_batch_workspace.argtypes = [ctypes.c_size_t]
@@ -189,16 +214,59 @@ class QEDKernelTemps:
         return self._one(None, x, y, M)
 
     def batch_host(self, which, x, y, M=0.0, layout=KQED_AOS):
-        """n pairs from host arrays x[n,4], y[n,4] -> (n,384) (AoS) or (384,n) (SoA) float64."""
+        """n pairs from host arrays x[n,4], y[n,4] -> (n,384) (AoS) or (384,n) (SoA), float64 or float32."""
         x = np.ascontiguousarray(x, np.float64)
         y = np.ascontiguousarray(y, np.float64)
         n = x.shape[0]
-        out = np.zeros((n, 384) if layout == KQED_AOS else (384, n))
+        out = np.zeros((n, 384) if layout in (KQED_AOS, KQED_AOS_F32) else (384, n),
+                       np.float32 if layout in (KQED_AOS_F32, KQED_SOA_F32) else np.float64)
         rc = lib().kqed_batch_host(ctypes.byref(self._t), int(which), float(M), n, x.reshape(-1), y.reshape(-1),
-                                   out.reshape(-1), int(layout))
+                                   out.ctypes.data, int(layout))
         if rc != 0:
             raise RuntimeError("kqed_batch_host failed: " + lib().kqed_last_error().decode())
         return out
+
+    def get_tune(self):
+        tu = Tune()
+        lib().kqed_get_tune(ctypes.byref(self._t), ctypes.byref(tu))
+        return tu
+
+    def set_tune(self, tune=None, **kw):
+        tu = self.get_tune() if tune is None else tune
+        for k, v in kw.items():
+            setattr(tu, k, v)
+        if lib().kqed_set_tune(ctypes.byref(self._t), ctypes.byref(tu)) != 0:
+            raise ValueError("kqed_set_tune: " + lib().kqed_last_error().decode())
+        return tu
+
+    def autotune(self, which, n, x_ptr, y_ptr, out_ptr, M=0.0, layout=KQED_AOS, reps=2, stream=None):
+        """Run the library's autotuner on a device batch; installs and returns (Tune, ms per run)."""
+        tu = Tune()
+        ms = ctypes.c_double()
+        rc = lib().kqed_autotune(ctypes.byref(self._t), int(which), float(M), int(n), ctypes.c_void_p(x_ptr),
+                                 ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr), int(layout), int(reps),
+                                 ctypes.c_void_p(stream or 0), ctypes.byref(tu), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError("kqed_autotune failed: " + lib().kqed_last_error().decode())
+        return tu, ms.value
+
+    def batch_ragged(self, which, offsets, y_ptr, x_ptr, out_ptr, M=0.0, layout=KQED_AOS, stream=None):
+        off = np.ascontiguousarray(offsets, np.uintp)
+        rc = lib().kqed_batch_ragged(ctypes.byref(self._t), int(which), float(M), len(off) - 1, off,
+                                     ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr),
+                                     int(layout), ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_batch_ragged failed: " + lib().kqed_last_error().decode())
+
+    def graph_ragged(self, which, offsets, y_ptr, x_ptr, out_ptr, M=0.0, layout=KQED_AOS):
+        off = np.ascontiguousarray(offsets, np.uintp)
+        g = ctypes.c_void_p()
+        rc = lib().kqed_graph_create_ragged(ctypes.byref(self._t), int(which), float(M), len(off) - 1, off,
+                                            ctypes.c_void_p(y_ptr), ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr),
+                                            int(layout), ctypes.byref(g))
+        if rc != 0:
+            raise RuntimeError("kqed_graph_create_ragged failed: " + lib().kqed_last_error().decode())
+        return RaggedGraph(g)
 
     def profile(self, enable):
         """Read-and-reset the per-stage kernel times (ms, launches) and switch timing on/off."""
@@ -234,3 +302,26 @@ class QEDKernelTemps:
                               ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError("kqed_batch failed: " + lib().kqed_last_error().decode())
+
+
+class RaggedGraph:
+    """A captured ragged batch (kqed_graph); launch() replays it on a stream."""
+
+    def __init__(self, g):
+        self._g = g
+
+    def launch(self, stream=None):
+        rc = lib().kqed_graph_launch(self._g, ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_graph_launch failed: " + lib().kqed_last_error().decode())
+
+    def destroy(self):
+        if self._g:
+            lib().kqed_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

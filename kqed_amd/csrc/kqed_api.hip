@@ -23,8 +23,8 @@
 /* kernels (kqed_kernels.hip) */
 __global__ void k_build_arrays(kqed_dev_tables T, const float *Fm, const float *Fp, double2 *P, double *PNC, double2 *S);
 __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
-__global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, unsigned *key,
-                           unsigned *rank, unsigned *count);
+__global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, int mode,
+                           unsigned *key, unsigned *rank, unsigned *count);
 __global__ void k_scan(unsigned *count, unsigned nb);
 __global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm);
 __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
@@ -32,8 +32,10 @@ template <int CLS>
 __global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys,
                          const unsigned *perm, const unsigned *bounds, int ilo, int ihi, size_t c0, size_t c1, int which,
                          double M, double *rec, unsigned *kinds);
-__global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out, int layout,
+template <typename R>
+__global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, void *out, int layout,
                            int nslots, size_t ntot, const float *Tc, const double *Wc);
+__global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
 __global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
                                    double *W);
 __global__ void k_block_sum(size_t n, size_t per, const double *in, double *out);
@@ -66,10 +68,8 @@ extern "C" const char *kqed_last_error(void) { return g_err; }
     }                                                                                 \
   } while (0)
 
-/* chunk of pairs processed per pipeline pass; bounds the per-pass workspace */
-static const size_t CHUNK = (size_t)1 << 20;
-/* below this many pairs the cell sort does not pay */
-static const size_t SORT_MIN = 2048;
+/* internal output mode of the assembly kernel: per-pair dot products with a site tensor */
+static const int LAYOUT_DOT = 16;
 
 struct Workspace {
   size_t cap = 0, ncap = 0, nbuckets = 0;
@@ -87,6 +87,9 @@ struct StageEvent { int stage; hipEvent_t a, b; };
 
 struct kqed_device {
   int device = 0;
+  /* launch parameters: pairs per pipeline pass (bounds the record workspace), assembly grid,
+   * sort key granularity and the batch size from which the sort is used */
+  kqed_tune tune = { (size_t)1 << 20, 16384, KQED_SORT_CELL, 2048 };
   kqed_dev_tables dt;
   void *allocs[32];
   int nalloc = 0;
@@ -364,7 +367,7 @@ extern "C" int initialise(struct QED_kernel_temps *t)
 
 extern "C" size_t kqed_batch_workspace(size_t n)
 {
-  const size_t c = n < CHUNK ? n : CHUNK;
+  const size_t c = n < ((size_t)1 << 20) ? n : ((size_t)1 << 20);
   return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (1 + 100 * 4096 + 4096);
 }
 
@@ -396,15 +399,19 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
   return 0;
 }
 
-/* The pipeline over n pairs (x[xs*i..], y[ys*i..]), lock held: one cell sort of the whole batch,
- * then passes of CHUNK pairs (interpolation per regime class, chain-rule algebra, assembly). */
+/* The pipeline over n pairs (x[xs*i..], y[ys*i..]), lock held: one sort of the whole batch by
+ * regime and interpolation cell, then passes of tune.chunk pairs (interpolation per regime class,
+ * chain-rule algebra, assembly).  Only kernel launches: it can be captured into a graph once the
+ * workspace is sized (ensure_ws with the same sizes beforehand). */
 static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, double M, size_t n, const double *x_dev,
-                     int xs, const double *y_dev, int ys, double *out_dev, int layout, const float *Tc,
+                     int xs, const double *y_dev, int ys, void *out_dev, int layout, const float *Tc,
                      const double *Wc)
 {
   if (n == 0) return 0;
-  const bool sort = n >= SORT_MIN;
-  if (ensure_ws(d, w, n < CHUNK ? n : CHUNK, sort ? n : 0)) return -2;
+  const kqed_tune &tu = d->tune;
+  const size_t CH = tu.chunk;
+  const bool sort = n >= tu.sort_min && tu.sort_mode != KQED_SORT_NONE;
+  if (ensure_ws(d, w, n < CH ? n : CH, sort ? n : 0)) return -2;
   const kqed_dev_tables &T = d->dt;
   const unsigned *perm = nullptr;
   StageEvent ev;
@@ -414,15 +421,16 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     const unsigned blocks = (unsigned)((n + 255) / 256);
     prof_begin(d, s, ev, ST_SORT);
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((w.nbuckets + 255) / 256)), dim3(256), 0, s, w.count, w.nbuckets, 0u);
-    hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, w.key, w.rank, w.count);
+    hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, tu.sort_mode, w.key, w.rank,
+                       w.count);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets);
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm);
     perm = w.perm;
     prof_end(d, s, ev);
   }
   static const int nslots[5] = { 1, 3, 3, 4, 3 };
-  for (size_t c0 = 0; c0 < n; c0 += CHUNK) {
-    const size_t c = n - c0 < CHUNK ? n - c0 : CHUNK, c1 = c0 + c;
+  for (size_t c0 = 0; c0 < n; c0 += CH) {
+    const size_t c = n - c0 < CH ? n - c0 : CH, c1 = c0 + c;
     const unsigned blocks = (unsigned)((c + 255) / 256);
     prof_begin(d, s, ev, ST_INTERP);
     if (perm) {
@@ -442,9 +450,14 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
     const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
+    const unsigned grid = (unsigned)(nblk < (size_t)tu.asm_grid ? nblk : (size_t)tu.asm_grid);
     prof_begin(d, s, ev, ST_ASSEMBLE);
-    hipLaunchKernelGGL(k_assemble, dim3((unsigned)(nblk < 16384 ? nblk : 16384)), dim3(64 * KQED_ASM_WAVES), 0, s, T, c,
-                       w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc, Wc);
+    if (layout == KQED_AOS_F32 || layout == KQED_SOA_F32)
+      hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
+                         nslots[which], n, Tc, Wc);
+    else
+      hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
+                         nslots[which], n, Tc, Wc);
     prof_end(d, s, ev);
   }
   const hipError_t e = hipGetLastError();
@@ -452,17 +465,244 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
   return 0;
 }
 
+static bool layout_ok(int layout)
+{
+  return layout == KQED_AOS || layout == KQED_SOA || layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
+}
+
 extern "C" int kqed_batch(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
                           const double *y_dev, double *out_dev, int layout, void *stream)
 {
   if (!t || !t->dev) { set_err("kqed_batch: tables not initialised"); return -1; }
-  if (which < 0 || which > 4 || (layout != KQED_AOS && layout != KQED_SOA)) { set_err("kqed_batch: bad argument"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout)) { set_err("kqed_batch: bad argument"); return -1; }
   if (n == 0) return 0;
   if (n > 0xFFFFFFFFull) { set_err("kqed_batch: at most 2^32-1 pairs per call"); return -1; }
   kqed_device *d = t->dev;
   std::lock_guard<std::mutex> lk(d->mu);
   return run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
                    nullptr);
+}
+
+extern "C" int kqed_get_tune(const struct QED_kernel_temps *t, kqed_tune *tune)
+{
+  if (!t || !t->dev || !tune) return -1;
+  std::lock_guard<std::mutex> lk(t->dev->mu);
+  *tune = t->dev->tune;
+  return 0;
+}
+
+extern "C" int kqed_set_tune(const struct QED_kernel_temps *t, const kqed_tune *tune)
+{
+  if (!t || !t->dev || !tune) return -1;
+  if (tune->chunk < 256 || tune->chunk > ((size_t)1 << 24) || tune->asm_grid < 1 || tune->asm_grid > (1 << 20) ||
+      tune->sort_mode < KQED_SORT_CELL || tune->sort_mode > KQED_SORT_NONE) {
+    set_err("kqed_set_tune: parameter out of range");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(t->dev->mu);
+  t->dev->tune = *tune;
+  return 0;
+}
+
+/* ---------------------------------------------------------------- ragged groups */
+/* Device copies of the group offsets and the per-pair y vectors of a ragged batch. */
+struct RaggedBufs {
+  size_t n = 0, ng = 0;
+  unsigned long long *off = nullptr;
+  double *ypair = nullptr;
+  void release()
+  {
+    (void)hipFree(off); (void)hipFree(ypair);
+    off = nullptr; ypair = nullptr; n = ng = 0;
+  }
+};
+
+static int ragged_prepare(RaggedBufs &rb, size_t ng, const size_t *offsets, hipStream_t s)
+{
+  const size_t n = offsets[ng];
+  for (size_t g = 0; g < ng; g++)
+    if (offsets[g + 1] < offsets[g]) { set_err("ragged offsets must be non-decreasing"); return -1; }
+  if (offsets[0] != 0) { set_err("ragged offsets must start at 0"); return -1; }
+  if (n > 0xFFFFFFFFull || ng > 0x7FFFFFFF) { set_err("ragged batch too large"); return -1; }
+  if (rb.ng < ng + 1 || rb.n < n) {
+    rb.release();
+    if (hipMalloc(&rb.off, 8 * (ng + 1)) != hipSuccess || (n && hipMalloc(&rb.ypair, 32 * n) != hipSuccess)) {
+      set_err("ragged workspace allocation failed");
+      return -2;
+    }
+    rb.ng = ng + 1;
+    rb.n = n;
+  }
+  std::vector<unsigned long long> o(offsets, offsets + ng + 1);
+  if (hipMemcpyAsync(rb.off, o.data(), 8 * (ng + 1), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_err("ragged offsets upload failed");
+    return -3;
+  }
+  return 0;
+}
+
+static int ragged_launch(kqed_device *d, Workspace &w, RaggedBufs &rb, hipStream_t s, int which, double M, size_t ng,
+                         size_t n, const double *y_dev, const double *x_dev, void *out_dev, int layout)
+{
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_ragged_expand, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, (int)ng, rb.off, y_dev,
+                     rb.ypair);
+  return run_pairs(d, w, s, which, M, n, x_dev, 4, rb.ypair, 4, out_dev, layout, nullptr, nullptr);
+}
+
+static thread_local RaggedBufs g_ragged;
+
+extern "C" int kqed_batch_ragged(const struct QED_kernel_temps *t, int which, double M, size_t ngroups,
+                                 const size_t *offsets, const double *y_dev, const double *x_dev, void *out_dev,
+                                 int layout, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_batch_ragged: tables not initialised"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout) || !offsets) { set_err("kqed_batch_ragged: bad argument"); return -1; }
+  if (ngroups == 0 || offsets[ngroups] == 0) return 0;
+  kqed_device *d = t->dev;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ragged_prepare(g_ragged, ngroups, offsets, s);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(d->mu);
+  return ragged_launch(d, d->ws[stream], g_ragged, s, which, M, ngroups, offsets[ngroups], y_dev, x_dev, out_dev, layout);
+}
+
+/* A ragged batch captured once into a HIP graph with its own workspace; every launch replays the
+ * whole pipeline (expansion, sort, all passes) as one graph launch. */
+struct kqed_graph {
+  kqed_device *d = nullptr;
+  Workspace w;
+  RaggedBufs rb;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  size_t n = 0;
+};
+
+extern "C" void kqed_graph_destroy(kqed_graph *g)
+{
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->cap) { (void)hipStreamSynchronize(g->cap); (void)hipStreamDestroy(g->cap); }
+  free_ws(g->w);
+  g->rb.release();
+  delete g;
+}
+
+extern "C" int kqed_graph_create_ragged(const struct QED_kernel_temps *t, int which, double M, size_t ngroups,
+                                        const size_t *offsets, const double *y_dev, const double *x_dev, void *out_dev,
+                                        int layout, kqed_graph **out)
+{
+  if (!out) return -1;
+  *out = nullptr;
+  if (!t || !t->dev) { set_err("kqed_graph_create_ragged: tables not initialised"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout) || !offsets || ngroups == 0) {
+    set_err("kqed_graph_create_ragged: bad argument");
+    return -1;
+  }
+  kqed_device *d = t->dev;
+  kqed_graph *g = new kqed_graph();
+  g->d = d;
+  g->n = offsets[ngroups];
+  int rc = 0;
+  if (hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) { set_err("stream creation failed"); rc = -2; }
+  if (!rc) rc = ragged_prepare(g->rb, ngroups, offsets, g->cap);
+  if (!rc && g->n) {
+    std::lock_guard<std::mutex> lk(d->mu);
+    const size_t CH = d->tune.chunk;
+    const bool sort = g->n >= d->tune.sort_min && d->tune.sort_mode != KQED_SORT_NONE;
+    rc = ensure_ws(d, g->w, g->n < CH ? g->n : CH, sort ? g->n : 0);
+    const bool prof = d->profiling;
+    d->profiling = false; /* no event records inside the graph */
+    if (!rc && hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      set_err("graph capture failed to start");
+      rc = -3;
+    } else if (!rc) {
+      rc = ragged_launch(d, g->w, g->rb, g->cap, which, M, ngroups, g->n, y_dev, x_dev, out_dev, layout);
+      const hipError_t e = hipStreamEndCapture(g->cap, &g->graph);
+      if (!rc && e != hipSuccess) { set_err("graph capture failed: %s", hipGetErrorString(e)); rc = -3; }
+      if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+        set_err("graph instantiation failed");
+        rc = -3;
+      }
+    }
+    d->profiling = prof;
+  }
+  if (rc) { kqed_graph_destroy(g); return rc; }
+  *out = g;
+  return 0;
+}
+
+extern "C" int kqed_graph_launch(kqed_graph *g, void *stream)
+{
+  if (!g) { set_err("kqed_graph_launch: null graph"); return -1; }
+  if (!g->exec) return 0; /* empty batch */
+  const hipError_t e = hipGraphLaunch(g->exec, (hipStream_t)stream);
+  if (e != hipSuccess) { set_err("graph launch failed: %s", hipGetErrorString(e)); return -3; }
+  return 0;
+}
+
+/* ---------------------------------------------------------------- autotuner */
+/* Times every candidate launch configuration on the caller's batch (device buffers, which/layout as
+ * given) and keeps the fastest: passes of 2^17..2^21 pairs, assembly grids of 4096..65536
+ * workgroups, sort by cell or by x node.  Each candidate runs once untimed, then `reps` times. */
+extern "C" int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
+                             const double *y_dev, void *out_dev, int layout, int reps, void *stream, kqed_tune *best,
+                             double *best_ms)
+{
+  if (!t || !t->dev || !best) { set_err("kqed_autotune: bad argument"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout) || n == 0) { set_err("kqed_autotune: bad argument"); return -1; }
+  kqed_device *d = t->dev;
+  hipStream_t s = (hipStream_t)stream;
+  reps = reps < 1 ? 1 : reps;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) { set_err("event creation failed"); return -2; }
+  kqed_tune orig;
+  {
+    std::lock_guard<std::mutex> lk(d->mu);
+    orig = d->tune;
+  }
+  kqed_tune bt = orig;
+  double bms = 1e300;
+  int rc = 0;
+  const size_t chunks[] = { (size_t)1 << 17, (size_t)1 << 18, (size_t)1 << 19, (size_t)1 << 20, (size_t)1 << 21 };
+  const int grids[] = { 4096, 16384, 65536 };
+  const int modes[] = { KQED_SORT_CELL, KQED_SORT_XNODE };
+  for (size_t ci = 0; ci < 5 && !rc; ci++) {
+    if (ci > 0 && chunks[ci - 1] >= n) break; /* longer passes than the batch change nothing */
+    for (int gi = 0; gi < 3 && !rc; gi++)
+      for (int mi = 0; mi < 2 && !rc; mi++) {
+        kqed_tune c = orig;
+        c.chunk = chunks[ci];
+        c.asm_grid = grids[gi];
+        c.sort_mode = modes[mi];
+        std::lock_guard<std::mutex> lk(d->mu);
+        d->tune = c;
+        Workspace &w = d->ws[stream];
+        rc = run_pairs(d, w, s, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr, nullptr);
+        if (rc) break;
+        (void)hipEventRecord(e0, s);
+        for (int r = 0; r < reps && !rc; r++)
+          rc = run_pairs(d, w, s, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr, nullptr);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) {
+          set_err("autotune timing failed");
+          rc = -3;
+        }
+        if (!rc && ms / reps < bms) { bms = ms / reps; bt = c; }
+      }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(d->mu);
+  d->tune = rc ? orig : bt;
+  if (rc) return rc;
+  *best = bt;
+  if (best_ms) *best_ms = bms;
+  return 0;
 }
 
 /* contraction buffers: dots [nx], W [nx][200] (shortcut only) and the reduction scratch `red`:
@@ -527,7 +767,7 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
     hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)((nx * 200 + 255) / 256)), dim3(256), 0, s, nx, T_dev, d->csc_cp,
                        d->csc_rows, d->csc_vals, w.W);
   for (size_t j = 0; j < ny && !rc; j++) {
-    rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, 2, T_dev,
+    rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, LAYOUT_DOT, T_dev,
                    shortcut ? w.W : nullptr);
     if (!rc) dev_sum(s, nx, w.dots, part, sigma + j);
   }
@@ -557,7 +797,7 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
     }
     for (int k = 0; k < 2 && !rc; k++) {
       if (!ax[k]) continue;
-      rc = run_pairs(d, w, s, KQED_L0, 0.0, nx, ax[k], as[k], bx[k], bs[k], w.dots, 2, T_dev, nullptr);
+      rc = run_pairs(d, w, s, KQED_L0, 0.0, nx, ax[k], as[k], bx[k], bs[k], w.dots, LAYOUT_DOT, T_dev, nullptr);
       if (!rc) dev_sum(s, nx, w.dots, part, consts + k);
     }
   }
@@ -583,8 +823,10 @@ struct HostStage {
 };
 static thread_local HostStage g_stage;
 
+static size_t out_bytes(int layout) { return layout == KQED_AOS_F32 || layout == KQED_SOA_F32 ? 1536 : 3072; }
+
 extern "C" int kqed_batch_host(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x,
-                               const double *y, double *out, int layout)
+                               const double *y, void *out, int layout)
 {
   if (!t || !t->dev) { set_err("kqed_batch_host: tables not initialised"); return -1; }
   if (n == 0) return 0;
@@ -608,7 +850,7 @@ extern "C" int kqed_batch_host(const struct QED_kernel_temps *t, int which, doub
   }
   int rc = kqed_batch(t, which, M, n, st.x, st.y, st.o, layout, st.s);
   if (rc) return rc;
-  if (hipMemcpyAsync(out, st.o, 3072 * n, hipMemcpyDeviceToHost, st.s) != hipSuccess ||
+  if (hipMemcpyAsync(out, st.o, out_bytes(layout) * n, hipMemcpyDeviceToHost, st.s) != hipSuccess ||
       hipStreamSynchronize(st.s) != hipSuccess) {
     set_err("device to host copy failed");
     return -3;

@@ -15,6 +15,7 @@
  */
 #include <stdint.h>
 
+#include "../../include/kqed.h"
 #include "kqed_device.h"
 
 #define DEV __device__ __forceinline__
@@ -1092,7 +1093,7 @@ DEV unsigned wave_atomic_add(unsigned *base, unsigned key)
 /* key 0: special points, out of grid, grid edges; 1 + ix*nstpy + iy: packed path;
  * 1 + nstpx*nstpy + iy: Taylor-x region */
 __global__ void k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
-                           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
+                           int mode, unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1101,8 +1102,11 @@ __global__ void k_classify(DT T, size_t n, const double *__restrict__ x, int xs,
   unsigned k = 0;
   if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
     const Inv I = set_inv(T, xv, yv);
-    if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (unsigned)I.INVy.idx;
-    else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1]) k = 1 + (unsigned)(I.INVx.idx * T.nstpy + I.INVy.idx);
+    /* mode KQED_SORT_XNODE keys the packed path by x node only and the Taylor region as one bucket */
+    const bool cell = mode == KQED_SORT_CELL;
+    if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (cell ? (unsigned)I.INVy.idx : 0u);
+    else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1])
+      k = 1 + (unsigned)(I.INVx.idx * T.nstpy + (cell ? I.INVy.idx : 0));
   }
   key[i] = k;
   rank[i] = wave_atomic_add(count, k); /* slot of the pair inside its bucket */
@@ -1368,24 +1372,26 @@ DEV void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-DEV double wm_dot(const unsigned short *__restrict__ e, int K, int lane, const double *Tv)
+template <typename R>
+DEV R wm_dot(const unsigned short *__restrict__ e, int K, int lane, const R *Tv)
 {
-  double s = 0.0;
+  R s = 0;
   for (int k = 0; k < K; k++) {
     const unsigned w = e[64 * k + lane];
-    const double c = ldexp((w & 0x400u) ? -1.0 : 1.0, (int)((w >> 8) & 3u) - 1);
+    const R c = ldexp((w & 0x400u) ? (R)-1 : (R)1, (int)((w >> 8) & 3u) - 1);
     s = fma(c, Tv[w & 0xFFu], s);
   }
   return s;
 }
 
 /* CSR row of map q (for the M-L2 alpha sums, which need rows other than the lane's own) */
+template <typename R>
 DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r,
-                   const double *Tv)
+                   const R *Tv)
 {
   double s = 0.0;
   const int e = rp[r + 1];
-  for (int k = rp[r]; k < e; k++) s = fma(val[k], Tv[col[k]], s);
+  for (int k = rp[r]; k < e; k++) s = fma(val[k], (double)Tv[col[k]], s);
   return s;
 }
 
@@ -1397,14 +1403,18 @@ DEV double wave_sum(double v)
   return v;
 }
 
-/* Output modes: layout 0 writes out[q][384], layout 1 out[384][ntot], layout 2 (contraction)
- * writes the dot product out[q] = sum_c L_c Tc[q][c] with the FP32 site tensor Tc.  With Wc
- * (layout 2 and a single standard L0 slot) the dot takes the linear shortcut
- * sum_c L_c T_c = sum_j u_j W_j, u the 200 tensor entries and W = A^T T precomputed per site. */
+/* R is the arithmetic type of the sparse assembly maps (the 200 tensor entries and their linear
+ * combinations); the slot combination of L1..L3 / M-L2 and every output sum stay FP64.
+ * Output modes: layout bit 0 selects out[384][ntot] (else out[q][384]), bit 1 FP32 outputs (else
+ * FP64); layout 16 (contraction) writes the FP64 dot product out[q] = sum_c L_c Tc[q][c] with the
+ * FP32 site tensor Tc.  With Wc (layout 16 and a single standard L0 slot) the dot takes the linear
+ * shortcut sum_c L_c T_c = sum_j u_j W_j, u the 200 tensor entries and W = A^T T per site. */
+template <typename MT>
 __global__ void __launch_bounds__(64 * KQED_ASM_WAVES)
-k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, double *__restrict__ out,
+k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, void *__restrict__ outv,
            int layout, int nslots, size_t ntot, const float *__restrict__ Tc, const double *__restrict__ Wc)
 {
+  typedef MT R_t;
   __shared__ unsigned short sE[KQED_WM_MAX];
   __shared__ unsigned short sRow[3 * 384];
   __shared__ double sW[KQED_ASM_WAVES][WREC + WH + 201 + WT];
@@ -1412,7 +1422,8 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
   for (int i = threadIdx.x; i < T.wmTotal; i += blockDim.x) sE[i] = T.wmE[i];
   for (int i = threadIdx.x; i < 3 * 384; i += blockDim.x) sRow[i] = T.wmRow[i];
   __syncthreads();
-  double *R = sW[w], *H = R + WREC, *Tv = H + WH, *O = Tv + 201;
+  double *R = sW[w], *H = R + WREC, *O = H + WH + 201;
+  R_t *Tv = (R_t *)(H + WH);
   const size_t stride = (size_t)gridDim.x * KQED_ASM_WAVES;
   for (size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w; p < n; p += stride) {
     for (int i = lane; i < WREC; i += 64) R[i] = rec[p * WREC + i];
@@ -1445,7 +1456,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         /* contraction shortcut: only slot 0 exists (nslots == 1) */
         const double *wq = Wc + q * 200;
         double part = 0.0;
-        for (int c = lane; c < 200; c += 64) part = fma(Tv[c], wq[c], part);
+        for (int c = lane; c < 200; c += 64) part = fma((double)Tv[c], wq[c], part);
         dot = wave_sum(part);
         have_dot = true;
         continue;
@@ -1483,19 +1494,22 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       wave_sync();
     }
-    if (layout == 0) {
-      double *o = out + q * 384;
-      for (int i = lane; i < 384; i += 64) o[i] = O[i];
-    } else if (layout == 1) {
-      for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = O[i];
-    } else {
+    if (layout == 16) {
       if (!have_dot) {
         const float *tq = Tc + q * 384;
         double part = 0.0;
         for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
         dot = wave_sum(part);
       }
-      if (lane == 0) out[q] = dot;
+      if (lane == 0) ((double *)outv)[q] = dot;
+    } else if (layout & 2) {
+      float *out = (float *)outv;
+      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = (float)O[i];
+      else for (int i = lane; i < 384; i += 64) out[q * 384 + i] = (float)O[i];
+    } else {
+      double *out = (double *)outv;
+      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = O[i];
+      else for (int i = lane; i < 384; i += 64) out[q * 384 + i] = O[i];
     }
     wave_sync();
   }
@@ -1571,4 +1585,23 @@ __global__ void k_combine(int ny, const double *__restrict__ sigma, const double
     if (consts) { v = fma(cc0, consts[0], v); v = fma(cc1, consts[1], v); }
     s[j] = v;
   }
+}
+
+template __global__ void k_assemble<double>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
+                                            const float *, const double *);
+template __global__ void k_assemble<float>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
+                                           const float *, const double *);
+
+/* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
+__global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,
+                                const double *__restrict__ y, double *__restrict__ ypair)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = 0, hi = ng; /* off[lo] <= i < off[hi] */
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid; else hi = mid;
+  }
+  for (int k = 0; k < 4; k++) ypair[4 * i + k] = y[4 * lo + k];
 }
