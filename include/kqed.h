@@ -10,8 +10,8 @@
  * Output layout everywhere: double kerv[6][4][4][4], flat index 64*rhosig + 16*mu + 4*nu + lambda,
  * rhosig enumerating (0,1),(0,2),(0,3),(1,2),(1,3),(2,3) (reference pi_pert.c:196-199).
  */
-#ifndef KQED_H
-#define KQED_H
+#ifndef KQED_AMD_H
+#define KQED_AMD_H
 
 #include <stddef.h>
 
@@ -149,6 +149,41 @@ int kqed_profile(const struct QED_kernel_temps *t, int enable, double *ms, long 
 
 /* Last error text of this thread ("" if none). */
 const char *kqed_last_error(void);
+
+/* ---------------------------------------------------------------- learned surrogate (new) */
+/* An MLP (3 -> 4 x 128 SiLU -> 34; BF16 weights, FP32 accumulation on the matrix cores) standing in
+ * for the table interpolation of the 34 weight functions of packed-path pairs (reference
+ * getff-new.c:214-312, simd.c:343-687, as consumed by chnr_dS.c, chnr_dT.c, chnr_dV.c).  It is
+ * trained on the GPU from the FP64 interpolator's own output on caller-given pairs, validated per
+ * (log|x|, log|y|) bin against the table path, and used only in bins that passed; the Taylor
+ * region, special points and failing bins always take the tables.  No reference counterpart. */
+typedef struct kqed_surrogate kqed_surrogate;
+int kqed_surrogate_create(const struct QED_kernel_temps *t, unsigned long long seed, kqed_surrogate **s);
+void kqed_surrogate_destroy(kqed_surrogate *s); /* before free_QED_temps of its tables */
+/* Interpolator samples on n device pairs, then `epochs` passes of Adam over batches (cosine step
+ * decay from lr); returns the last mean squared error of the normalised outputs in *loss. */
+int kqed_surrogate_train(kqed_surrogate *s, size_t n, const double *x_dev, const double *y_dev, int epochs,
+                         int batch, double lr, unsigned long long seed, double *loss);
+/* Error check on n device pairs: surrogate L0 vs table L0 per pair (normwise); a bin passes with
+ * at least min_count packed pairs all within tol.  stats[5] = {pairs, packed pairs, packed pairs in
+ * passing bins, max error, median error}. */
+int kqed_surrogate_validate(kqed_surrogate *s, size_t n, const double *x_dev, const double *y_dev,
+                            double tol, int min_count, double *stats);
+/* kqed_batch with the surrogate in the bins that passed (requires a validated surrogate). */
+int kqed_batch_surrogate(const struct QED_kernel_temps *t, const kqed_surrogate *s, int which, double M,
+                         size_t n, const double *x_dev, const double *y_dev, void *out_dev, int layout,
+                         void *stream);
+/* One forward/backward pass on a host batch (inputs X[B][3], targets and weights [B][34]): loss and
+ * the FP32 gradient of every parameter (kqed_surrogate_nparam() floats), for checking. */
+int kqed_surrogate_gradients(kqed_surrogate *s, int B, const float *X, const float *target,
+                             const float *weight, float *grad, double *loss);
+size_t kqed_surrogate_nparam(void);
+/* Parameters (FP32), log output scales [34][24][24], bin mask [24*24], input ranges [4]; any
+ * pointer may be NULL. */
+int kqed_surrogate_params(const kqed_surrogate *s, float *params, float *log_scales, unsigned char *mask,
+                          float *range);
+int kqed_surrogate_save(const kqed_surrogate *s, const char *path);
+int kqed_surrogate_load(kqed_surrogate *s, const char *path);
 
 /* ---------------------------------------------------------------- table utilities (new) */
 /* Write a synthetic form-factor table in the reference's file format (see kqed_tables.h). */

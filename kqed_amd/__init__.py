@@ -88,6 +88,20 @@ def lib():
     L.kqed_graph_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.kqed_graph_destroy.argtypes = [ctypes.c_void_p]
     L.kqed_graph_destroy.restype = None
+    vp, f32p = ctypes.c_void_p, np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+    L.kqed_surrogate_create.argtypes = [P, ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.kqed_surrogate_destroy.argtypes = [vp]
+    L.kqed_surrogate_destroy.restype = None
+    L.kqed_surrogate_train.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_double)]
+    L.kqed_surrogate_gradients.argtypes = [vp, ctypes.c_int, f32p, f32p, f32p, f32p, ctypes.POINTER(ctypes.c_double)]
+    L.kqed_surrogate_params.argtypes = [vp, vp, vp, vp, vp]
+    L.kqed_surrogate_nparam.restype = ctypes.c_size_t
+    L.kqed_surrogate_validate.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_double, ctypes.c_int, arr]
+    L.kqed_batch_surrogate.argtypes = [P, vp, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, vp, vp, vp, ctypes.c_int,
+                                       vp]
+    L.kqed_surrogate_save.argtypes = [vp, ctypes.c_char_p]
+    L.kqed_surrogate_load.argtypes = [vp, ctypes.c_char_p]
     L.kqed_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_autotune.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
@@ -325,3 +339,94 @@ class RaggedGraph:
             self.destroy()
         except Exception:
             pass
+
+
+# ----------------------------------------------------------------------------- surrogate
+SURR_NB = 24
+SURR_K = (16, 128, 128, 128, 128)
+SURR_N = (128, 128, 128, 128, 48)
+SURR_NOUT = 34
+
+
+def _err(name):
+    return RuntimeError(name + " failed: " + lib().kqed_last_error().decode())
+
+
+class Surrogate:
+    """The learned surrogate of the packed-path interpolation (kqed_surrogate_*)."""
+
+    def __init__(self, temps, seed=1):
+        self.temps = temps
+        self._g = ctypes.c_void_p()
+        if lib().kqed_surrogate_create(ctypes.byref(temps._t), int(seed), ctypes.byref(self._g)) != 0:
+            raise _err("kqed_surrogate_create")
+
+    def destroy(self):
+        if self._g:
+            lib().kqed_surrogate_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def train(self, n, x_ptr, y_ptr, epochs=4, batch=8192, lr=2e-3, seed=5):
+        loss = ctypes.c_double()
+        if lib().kqed_surrogate_train(self._g, int(n), x_ptr, y_ptr, int(epochs), int(batch), float(lr), int(seed),
+                                      ctypes.byref(loss)) != 0:
+            raise _err("kqed_surrogate_train")
+        return loss.value
+
+    def gradients(self, X3, tgt, wgt):
+        X3 = np.ascontiguousarray(X3, np.float32)
+        tgt = np.ascontiguousarray(tgt, np.float32)
+        wgt = np.ascontiguousarray(wgt, np.float32)
+        g = np.zeros(lib().kqed_surrogate_nparam(), np.float32)
+        loss = ctypes.c_double()
+        if lib().kqed_surrogate_gradients(self._g, X3.shape[0], X3.reshape(-1), tgt.reshape(-1), wgt.reshape(-1), g,
+                                          ctypes.byref(loss)) != 0:
+            raise _err("kqed_surrogate_gradients")
+        return loss.value, g
+
+    def params(self):
+        P = np.zeros(lib().kqed_surrogate_nparam(), np.float32)
+        lsc = np.zeros(SURR_NOUT * SURR_NB * SURR_NB, np.float32)
+        mask = np.zeros(SURR_NB * SURR_NB, np.uint8)
+        rng = np.zeros(4, np.float32)
+        if lib().kqed_surrogate_params(self._g, P.ctypes.data, lsc.ctypes.data, mask.ctypes.data, rng.ctypes.data) != 0:
+            raise _err("kqed_surrogate_params")
+        return P, lsc.reshape(SURR_NOUT, SURR_NB, SURR_NB), mask.reshape(SURR_NB, SURR_NB), rng
+
+    def validate(self, n, x_ptr, y_ptr, tol=1e-5, min_count=8):
+        st = np.zeros(5)
+        if lib().kqed_surrogate_validate(self._g, int(n), x_ptr, y_ptr, float(tol), int(min_count), st) != 0:
+            raise _err("kqed_surrogate_validate")
+        return {"pairs": int(st[0]), "packed": int(st[1]), "packed_in_passing_bins": int(st[2]),
+                "max_err": st[3], "median_err": st[4]}
+
+    def batch_device(self, which, n, x_ptr, y_ptr, out_ptr, M=0.0, layout=KQED_AOS, stream=None):
+        if lib().kqed_batch_surrogate(ctypes.byref(self.temps._t), self._g, int(which), float(M), int(n), x_ptr, y_ptr,
+                                      out_ptr, int(layout), stream or 0) != 0:
+            raise _err("kqed_batch_surrogate")
+
+    def save(self, path):
+        if lib().kqed_surrogate_save(self._g, path.encode()) != 0:
+            raise _err("kqed_surrogate_save")
+
+    def load(self, path):
+        if lib().kqed_surrogate_load(self._g, path.encode()) != 0:
+            raise _err("kqed_surrogate_load")
+
+
+def surrogate_layers(P):
+    """Split a flat parameter vector into [(W_l [K][N], b_l [N])] (FP32)."""
+    out, o = [], 0
+    for K, N in zip(SURR_K, SURR_N):
+        W = P[o:o + K * N].reshape(K, N)
+        o += K * N
+        b = P[o:o + N]
+        o += N
+        out.append((W, b))
+    return out

@@ -31,7 +31,32 @@ __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
 template <int CLS>
 __global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys,
                          const unsigned *perm, const unsigned *bounds, int ilo, int ihi, size_t c0, size_t c1, int which,
-                         double M, double *rec, unsigned *kinds);
+                         double M, double *rec, unsigned *kinds, const unsigned char *smask);
+__global__ void k_extract_scalars(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out);
+__global__ void k_surr_rowbin(kqed_dev_tables T, size_t n, const double *data, int *bin);
+/* surrogate kernels (kqed_surrogate.hip) */
+struct SurrRange { float lx0, lx1, ly0, ly1; };
+struct SurrNet {
+  const unsigned short *W[5];
+  const float *b[5];
+  const float *lsc;
+  SurrRange R;
+};
+__global__ void k_gemm_bf16(int M, int N, int K, int kper, const float *A, long sam, long sak, const float *B, long sbk,
+                            long sbn, float *C, long ldc, long zstride, const float *bias);
+__global__ void k_sum_splits(long n, int nz, long zstride, const float *part, float *out);
+__global__ void k_silu(long n, const float *Z, float *A);
+__global__ void k_silu_bwd(long n, const float *Z, const float *dA, float *dZ);
+__global__ void k_colsum_f(int B, int N, const float *dZ, float *db);
+__global__ void k_mse(int B, int NP, int NOUT, const float *pred, const float *tgt, const float *wgt, float norm,
+                      float *dpred, float *part);
+__global__ void k_adam(long n, float *p, const float *g, float *m, float *v, float lr_t, float b1, float b2, float eps);
+__global__ void k_surr_batch(int B, const unsigned *idx, const double *data, SurrRange R, const float *lsc, float *X,
+                             float *Tg, float *W);
+__global__ void k_surr_binmax(long n, const double *data, SurrRange R, unsigned *bmax, unsigned *bcount);
+__global__ void k_surr_records(size_t n, double *rec, const unsigned *kinds, SurrNet net);
+__global__ void k_surr_binerr(size_t n, const double *a, const double *b, const int *bin, unsigned *emax, unsigned *ecount,
+                              float *err);
 template <typename R>
 __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, void *out, int layout,
                            int nslots, size_t ntot, const float *Tc, const double *Wc);
@@ -68,8 +93,17 @@ extern "C" const char *kqed_last_error(void) { return g_err; }
     }                                                                                 \
   } while (0)
 
-/* internal output mode of the assembly kernel: per-pair dot products with a site tensor */
+/* internal output modes: per-pair dot products with a site tensor (assembly kernel), and the
+ * extracted interpolation results of the surrogate's training data (no algebra, no assembly) */
 static const int LAYOUT_DOT = 16;
+static const int LAYOUT_SCALARS = 32;
+
+/* The surrogate as the pipeline sees it: network parameters for the inference kernel and the
+ * per-bin mask of its error check (null = not validated: never used). */
+struct SurrUse {
+  SurrNet net;
+  const unsigned char *mask;
+};
 
 struct Workspace {
   size_t cap = 0, ncap = 0, nbuckets = 0;
@@ -405,7 +439,7 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
  * workspace is sized (ensure_ws with the same sizes beforehand). */
 static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, double M, size_t n, const double *x_dev,
                      int xs, const double *y_dev, int ys, void *out_dev, int layout, const float *Tc,
-                     const double *Wc)
+                     const double *Wc, const SurrUse *sur = nullptr)
 {
   if (n == 0) return 0;
   const kqed_tune &tu = d->tune;
@@ -437,15 +471,23 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       /* count[b] is the start of bucket b: special pairs [0, count[1]), packed path
        * [count[1], count[1 + nstpx*nstpy]), Taylor-x region [count[1 + nstpx*nstpy], n) */
       const int ip = 1 + T.nstpx * T.nstpy;
+      const unsigned char *smask = sur ? sur->mask : nullptr;
       hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
-                         which, M, w.rec, w.kinds);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
       hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
-                         which, M, w.rec, w.kinds);
+                         which, M, w.rec, w.kinds, smask);
       hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
-                         which, M, w.rec, w.kinds);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
+      if (smask)
+        hipLaunchKernelGGL(k_surr_records, dim3((unsigned)((c + 63) / 64)), dim3(256), 0, s, c, w.rec, w.kinds, sur->net);
     } else {
       hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
-                         which, M, w.rec, w.kinds);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
+    }
+    if (layout == LAYOUT_SCALARS) {
+      hipLaunchKernelGGL(k_extract_scalars, dim3(blocks), dim3(256), 0, s, T, c, w.rec, w.kinds, (double *)out_dev);
+      prof_end(d, s, ev);
+      continue;
     }
     hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
@@ -884,4 +926,514 @@ extern "C" int kqed_write_synthetic_ff(const char *path, int nstpx, int nstpy, i
   kqed_synth_params p;
   p.nstpx = nstpx; p.nstpy = nstpy; p.nf0 = nf0; p.nf1 = nf1; p.xh = xh; p.yh = yh; p.seed = seed;
   return kqed_synth_ff(path, &p);
+}
+
+/* ================================================================ learned surrogate */
+/* Parameter layout (FP32, contiguous): layer l = 0..4 holds W_l[K_l][N_l] then b_l[N_l], with
+ * K = {16, 128, 128, 128, 128} (rows 3..15 of W_0 stay zero) and N = {128, 128, 128, 128, 48}
+ * (columns 34..47 of W_4 and b_4 stay zero). */
+static const int SK[5] = { 16, 128, 128, 128, 128 };
+static const int SN[5] = { 128, 128, 128, 128, 48 };
+static size_t surr_off_W(int l)
+{
+  size_t o = 0;
+  for (int i = 0; i < l; i++) o += (size_t)SK[i] * SN[i] + SN[i];
+  return o;
+}
+static size_t surr_off_b(int l) { return surr_off_W(l) + (size_t)SK[l] * SN[l]; }
+static const size_t SURR_NPARAM = 2048 + 128 + 3 * (16384 + 128) + 6144 + 48;
+static const int NBB = KQED_SURR_NB * KQED_SURR_NB;
+
+struct kqed_surrogate {
+  kqed_device *d = nullptr;
+  hipStream_t s = nullptr;
+  float *P = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr; /* params, grads, Adam moments */
+  unsigned short *Wp = nullptr;                                    /* packed BF16 inference weights */
+  float *lsc = nullptr;                                            /* [34][NB][NB] log output scales */
+  unsigned char *mask = nullptr;                                   /* [NB*NB] error-check mask */
+  std::vector<float> h_lsc;
+  std::vector<unsigned char> h_mask;
+  SurrRange R;
+  bool validated = false;
+  long steps = 0;
+  /* training buffers for batches of up to bcap samples */
+  int bcap = 0;
+  float *X = nullptr, *Tg = nullptr, *Wt = nullptr, *Z[5] = {}, *A[5] = {}, *dA = nullptr, *dZ = nullptr,
+        *part = nullptr, *lpart = nullptr;
+  unsigned *idx = nullptr;
+};
+
+static size_t surr_wp_off(int l)
+{
+  size_t o = 0;
+  for (int i = 0; i < l; i++) o += (size_t)SK[i] * SN[i];
+  return o;
+}
+
+static SurrNet surr_net(const kqed_surrogate *g)
+{
+  SurrNet n;
+  for (int l = 0; l < 5; l++) {
+    n.W[l] = g->Wp + surr_wp_off(l);
+    n.b[l] = g->P + surr_off_b(l);
+  }
+  n.lsc = g->lsc;
+  n.R = g->R;
+  return n;
+}
+
+static void surr_free_train(kqed_surrogate *g)
+{
+  (void)hipFree(g->X); (void)hipFree(g->Tg); (void)hipFree(g->Wt); (void)hipFree(g->dA); (void)hipFree(g->dZ);
+  (void)hipFree(g->part); (void)hipFree(g->lpart); (void)hipFree(g->idx);
+  for (int l = 0; l < 5; l++) { (void)hipFree(g->Z[l]); (void)hipFree(g->A[l]); g->Z[l] = g->A[l] = nullptr; }
+  g->X = g->Tg = g->Wt = g->dA = g->dZ = g->part = g->lpart = nullptr;
+  g->idx = nullptr;
+  g->bcap = 0;
+}
+
+extern "C" void kqed_surrogate_destroy(kqed_surrogate *g)
+{
+  if (!g) return;
+  if (g->s) (void)hipStreamSynchronize(g->s);
+  surr_free_train(g);
+  (void)hipFree(g->P); (void)hipFree(g->G); (void)hipFree(g->Am); (void)hipFree(g->Av); (void)hipFree(g->Wp);
+  (void)hipFree(g->lsc); (void)hipFree(g->mask);
+  if (g->s) (void)hipStreamDestroy(g->s);
+  delete g;
+}
+
+/* splitmix64: seeded host initialisation */
+static unsigned long long smix(unsigned long long &s)
+{
+  unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+/* BF16 packing of the weights for the inference kernel (fragment order, see kqed_surrogate.hip) */
+static int surr_publish(kqed_surrogate *g)
+{
+  std::vector<float> P(SURR_NPARAM);
+  if (hipMemcpy(P.data(), g->P, 4 * SURR_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  std::vector<unsigned short> W(surr_wp_off(5));
+  for (int l = 0; l < 5; l++) {
+    const float *w = P.data() + surr_off_W(l);
+    unsigned short *o = W.data() + surr_wp_off(l);
+    for (int k = 0; k < SK[l]; k++)
+      for (int n = 0; n < SN[l]; n++) {
+        unsigned u;
+        memcpy(&u, &w[k * SN[l] + n], 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        o[((size_t)(k / 4) * SN[l] + n) * 4 + (k % 4)] = (unsigned short)(u >> 16);
+      }
+  }
+  return hipMemcpy(g->Wp, W.data(), 2 * W.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+}
+
+extern "C" int kqed_surrogate_create(const struct QED_kernel_temps *t, unsigned long long seed, kqed_surrogate **out)
+{
+  if (!t || !t->dev || !out) { set_err("kqed_surrogate_create: bad argument"); return -1; }
+  *out = nullptr;
+  kqed_surrogate *g = new kqed_surrogate();
+  g->d = t->dev;
+  const size_t np = SURR_NPARAM;
+  if (hipStreamCreateWithFlags(&g->s, hipStreamNonBlocking) != hipSuccess || hipMalloc(&g->P, 4 * np) != hipSuccess ||
+      hipMalloc(&g->G, 4 * np) != hipSuccess || hipMalloc(&g->Am, 4 * np) != hipSuccess ||
+      hipMalloc(&g->Av, 4 * np) != hipSuccess || hipMalloc(&g->Wp, 2 * surr_wp_off(5)) != hipSuccess ||
+      hipMalloc(&g->lsc, 4 * (size_t)KQED_NSURR * NBB) != hipSuccess || hipMalloc(&g->mask, NBB) != hipSuccess) {
+    set_err("surrogate allocation failed");
+    kqed_surrogate_destroy(g);
+    return -2;
+  }
+  /* uniform(-a, a) weights with a = sqrt(6 / fan_in) (fan_in 3 for the first layer), zero biases */
+  std::vector<float> P(np, 0.f);
+  unsigned long long st = seed;
+  for (int l = 0; l < 5; l++) {
+    const int fin = l == 0 ? 3 : SK[l], nout = l == 4 ? KQED_NSURR : SN[l];
+    const float a = sqrtf(6.f / fin) * (l == 4 ? 0.5f : 1.f);
+    float *w = P.data() + surr_off_W(l);
+    for (int k = 0; k < fin; k++)
+      for (int n = 0; n < nout; n++) w[k * SN[l] + n] = a * (2.f * (float)((smix(st) >> 11) * 0x1.0p-53) - 1.f);
+  }
+  const kqed_dev_tables &T = g->d->dt;
+  double xx0, xx1, yy1;
+  if (hipMemcpy(&xx0, T.XX, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&xx1, T.XX + T.nstpx - 1, 8, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(&yy1, T.YY + T.nstpy - 1, 8, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_err("surrogate: grid read failed");
+    kqed_surrogate_destroy(g);
+    return -3;
+  }
+  g->R = { (float)log(xx0), (float)log(xx1), (float)log(KQED_SURR_YMIN), (float)log(yy1) };
+  g->h_lsc.assign((size_t)KQED_NSURR * NBB, 0.f);
+  g->h_mask.assign(NBB, 0);
+  if (hipMemcpy(g->P, P.data(), 4 * np, hipMemcpyHostToDevice) != hipSuccess || hipMemset(g->Am, 0, 4 * np) != hipSuccess ||
+      hipMemset(g->Av, 0, 4 * np) != hipSuccess || hipMemset(g->G, 0, 4 * np) != hipSuccess ||
+      hipMemcpy(g->lsc, g->h_lsc.data(), 4 * g->h_lsc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(g->mask, 0, NBB) != hipSuccess || surr_publish(g)) {
+    set_err("surrogate initialisation failed");
+    kqed_surrogate_destroy(g);
+    return -3;
+  }
+  *out = g;
+  return 0;
+}
+
+static int surr_ensure_batch(kqed_surrogate *g, int B)
+{
+  if (g->bcap >= B) return 0;
+  surr_free_train(g);
+  bool ok = hipMalloc(&g->X, 4L * B * 16) == hipSuccess && hipMalloc(&g->Tg, 4L * B * KQED_NSURR) == hipSuccess &&
+            hipMalloc(&g->Wt, 4L * B * KQED_NSURR) == hipSuccess && hipMalloc(&g->dA, 4L * B * 128) == hipSuccess &&
+            hipMalloc(&g->dZ, 4L * B * 128) == hipSuccess && hipMalloc(&g->idx, 4L * B) == hipSuccess &&
+            hipMalloc(&g->lpart, 4L * ((B * 48 + 255) / 256)) == hipSuccess &&
+            hipMalloc(&g->part, 4L * 64 * 128 * 128) == hipSuccess;
+  for (int l = 0; l < 5 && ok; l++)
+    ok = hipMalloc(&g->Z[l], 4L * B * SN[l]) == hipSuccess && hipMalloc(&g->A[l], 4L * B * SN[l]) == hipSuccess;
+  if (!ok) { surr_free_train(g); set_err("surrogate training buffers failed"); return -2; }
+  g->bcap = B;
+  return 0;
+}
+
+static void gemm(hipStream_t s, int M, int N, int K, const float *A, long sam, long sak, const float *B, long sbk, long sbn,
+                 float *C, long ldc, const float *bias, float *part)
+{
+  /* split the reduction when the output grid alone cannot fill the device */
+  const int tiles = ((M + 31) / 32) * ((N + 31) / 32);
+  int splits = 1;
+  while (splits < 64 && tiles * splits < 512 && K / (splits * 2) >= 256) splits *= 2;
+  const int kper = ((K + splits - 1) / splits + 15) / 16 * 16;
+  splits = (K + kper - 1) / kper;
+  if (splits == 1) {
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((N + 31) / 32, (M + 31) / 32, 1), dim3(64), 0, s, M, N, K, kper, A, sam, sak, B, sbk,
+                       sbn, C, ldc, 0L, bias);
+    return;
+  }
+  hipLaunchKernelGGL(k_gemm_bf16, dim3((N + 31) / 32, (M + 31) / 32, splits), dim3(64), 0, s, M, N, K, kper, A, sam, sak, B,
+                     sbk, sbn, part, (long)N, (long)M * N, bias);
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)(((long)M * N + 255) / 256)), dim3(256), 0, s, (long)M * N, splits,
+                     (long)M * N, part, C); /* ldc == N here */
+}
+
+/* Forward and backward of one batch already in X/Tg/Wt: gradients into G, loss partials */
+static void surr_fwd_bwd(kqed_surrogate *g, int B)
+{
+  hipStream_t s = g->s;
+  const long n128 = (long)B * 128;
+  const float *in = g->X;
+  int ldin = 16;
+  for (int l = 0; l < 5; l++) {
+    gemm(s, B, SN[l], SK[l], in, ldin, 1, g->P + surr_off_W(l), SN[l], 1, g->Z[l], SN[l], g->P + surr_off_b(l), g->part);
+    if (l < 4) hipLaunchKernelGGL(k_silu, dim3((unsigned)((n128 + 255) / 256)), dim3(256), 0, s, n128, g->Z[l], g->A[l]);
+    in = g->A[l];
+    ldin = SN[l];
+  }
+  const long nout = (long)B * 48;
+  hipLaunchKernelGGL(k_mse, dim3((unsigned)((nout + 255) / 256)), dim3(256), 0, s, B, 48, KQED_NSURR, g->Z[4], g->Tg, g->Wt,
+                     (float)B * KQED_NSURR, g->dZ, g->lpart);
+  for (int l = 4; l >= 0; l--) {
+    /* dZ holds dL/dZ_l [B][N_l]; weight gradient A_{l-1}^T dZ_l, bias gradient column sums */
+    const float *Ain = l ? g->A[l - 1] : g->X;
+    const long ldA = l ? 128 : 16;
+    gemm(s, SK[l], SN[l], B, Ain, 1, ldA, g->dZ, SN[l], 1, g->G + surr_off_W(l), SN[l], nullptr, g->part);
+    hipLaunchKernelGGL(k_colsum_f, dim3((SN[l] + 127) / 128), dim3(128), 0, s, B, SN[l], g->dZ, g->G + surr_off_b(l));
+    if (l == 0) break;
+    /* dA_{l-1} = dZ_l W_l^T, then dZ_{l-1} = dA_{l-1} silu'(Z_{l-1}) */
+    gemm(s, B, SK[l], SN[l], g->dZ, SN[l], 1, g->P + surr_off_W(l), 1, SN[l], g->dA, SK[l], nullptr, g->part);
+    hipLaunchKernelGGL(k_silu_bwd, dim3((unsigned)((n128 + 255) / 256)), dim3(256), 0, s, n128, g->Z[l - 1], g->dA, g->dZ);
+  }
+}
+
+static double surr_loss(kqed_surrogate *g, int B)
+{
+  const int nb = (B * 48 + 255) / 256;
+  std::vector<float> lp(nb);
+  if (hipMemcpyAsync(lp.data(), g->lpart, 4L * nb, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+      hipStreamSynchronize(g->s) != hipSuccess)
+    return -1.0;
+  double l = 0.0;
+  for (float v : lp) l += v;
+  return l / ((double)B * KQED_NSURR);
+}
+
+/* Training data: the FP64 interpolator on the given pairs (device), rows of KQED_SURR_ROW doubles */
+static int surr_extract(kqed_surrogate *g, size_t n, const double *x_dev, const double *y_dev, double *data)
+{
+  kqed_device *d = g->d;
+  std::lock_guard<std::mutex> lk(d->mu);
+  return run_pairs(d, d->ws[(void *)g->s], g->s, KQED_L0, 0.0, n, x_dev, 4, y_dev, 4, data, LAYOUT_SCALARS, nullptr, nullptr);
+}
+
+/* per-bin log maxima of every weight function, empty bins filled from the nearest filled bin */
+static int surr_scales(kqed_surrogate *g, size_t n, const double *data)
+{
+  unsigned *bm = nullptr, *bc = nullptr;
+  const size_t nm = (size_t)KQED_NSURR * NBB;
+  if (hipMalloc(&bm, 4 * nm) != hipSuccess || hipMalloc(&bc, 4 * NBB) != hipSuccess) {
+    (void)hipFree(bm);
+    set_err("surrogate scale buffers failed");
+    return -2;
+  }
+  int rc = 0;
+  std::vector<unsigned> hm(nm), hc(NBB);
+  if (hipMemsetAsync(bm, 0, 4 * nm, g->s) != hipSuccess || hipMemsetAsync(bc, 0, 4 * NBB, g->s) != hipSuccess) rc = -3;
+  if (!rc) {
+    hipLaunchKernelGGL(k_surr_binmax, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->s, (long)n, data, g->R, bm, bc);
+    if (hipMemcpyAsync(hm.data(), bm, 4 * nm, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipMemcpyAsync(hc.data(), bc, 4 * NBB, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipStreamSynchronize(g->s) != hipSuccess)
+      rc = -3;
+  }
+  (void)hipFree(bm);
+  (void)hipFree(bc);
+  if (rc) { set_err("surrogate scale pass failed"); return rc; }
+  for (int k = 0; k < KQED_NSURR; k++)
+    for (int c = 0; c < NBB; c++) {
+      int best = -1, bd = 1 << 30;
+      for (int e = 0; e < NBB; e++) {
+        if (!hm[(size_t)k * NBB + e]) continue;
+        const int du = c / KQED_SURR_NB - e / KQED_SURR_NB, dv = c % KQED_SURR_NB - e % KQED_SURR_NB;
+        const int dd = du * du + dv * dv;
+        if (dd < bd) { bd = dd; best = e; }
+        if (!dd) break;
+      }
+      float v = 1.f;
+      if (best >= 0) memcpy(&v, &hm[(size_t)k * NBB + best], 4);
+      g->h_lsc[(size_t)k * NBB + c] = logf(v > 1e-30f ? v : 1e-30f);
+    }
+  return hipMemcpy(g->lsc, g->h_lsc.data(), 4 * nm, hipMemcpyHostToDevice) == hipSuccess ? 0 : -3;
+}
+
+extern "C" int kqed_surrogate_train(kqed_surrogate *g, size_t n, const double *x_dev, const double *y_dev, int epochs,
+                                    int batch, double lr, unsigned long long seed, double *loss_out)
+{
+  if (!g || !x_dev || !y_dev || n == 0 || epochs < 1 || batch < 64 || batch > (1 << 16)) {
+    set_err("kqed_surrogate_train: bad argument");
+    return -1;
+  }
+  double *data = nullptr;
+  if (hipMalloc(&data, 8 * KQED_SURR_ROW * n) != hipSuccess) { set_err("surrogate data allocation failed"); return -2; }
+  int rc = surr_extract(g, n, x_dev, y_dev, data);
+  if (!rc) rc = surr_scales(g, n, data);
+  if (!rc) rc = surr_ensure_batch(g, batch);
+  std::vector<unsigned> hidx(batch);
+  unsigned long long st = seed;
+  const long steps = (long)epochs * (long)((n + batch - 1) / batch);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double last = 0.0;
+  for (long it = 0; it < steps && !rc; it++) {
+    for (int b = 0; b < batch; b++) hidx[b] = (unsigned)(smix(st) % n);
+    if (hipMemcpyAsync(g->idx, hidx.data(), 4L * batch, hipMemcpyHostToDevice, g->s) != hipSuccess) { rc = -3; break; }
+    hipLaunchKernelGGL(k_surr_batch, dim3((batch + 255) / 256), dim3(256), 0, g->s, batch, g->idx, data, g->R, g->lsc, g->X,
+                       g->Tg, g->Wt);
+    surr_fwd_bwd(g, batch);
+    /* cosine decay of the step size over the run */
+    const double lr_it = lr * 0.5 * (1.0 + cos(M_PI * (double)it / (double)steps));
+    g->steps++;
+    const double bc1 = 1.0 - pow(b1, (double)g->steps), bc2 = 1.0 - pow(b2, (double)g->steps);
+    const float lr_t = (float)(lr_it * sqrt(bc2) / bc1);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((SURR_NPARAM + 255) / 256)), dim3(256), 0, g->s, (long)SURR_NPARAM, g->P, g->G,
+                       g->Am, g->Av, lr_t, b1, b2, eps);
+    if (it + 1 == steps || (it & 255) == 0) {
+      /* hipMemcpyAsync of the indices above needs the host buffer until the copy ran */
+      last = surr_loss(g, batch);
+    }
+    if (hipStreamSynchronize(g->s) != hipSuccess) rc = -3; /* the host index buffer is reused next step */
+  }
+  (void)hipFree(data);
+  if (!rc) rc = surr_publish(g);
+  if (!rc && loss_out) *loss_out = last;
+  g->validated = false;
+  if (rc && !*g_err) set_err("surrogate training failed");
+  return rc;
+}
+
+/* One forward/backward on a host batch (X[B][3] network inputs, targets and weights [B][34]):
+ * returns the loss and the FP32 gradients (SURR_NPARAM, layout above) for checking. */
+extern "C" int kqed_surrogate_gradients(kqed_surrogate *g, int B, const float *X3, const float *tgt, const float *wgt,
+                                        float *grad, double *loss)
+{
+  if (!g || B < 1 || B > (1 << 16) || !X3 || !tgt || !wgt || !grad) { set_err("kqed_surrogate_gradients: bad argument"); return -1; }
+  int rc = surr_ensure_batch(g, B);
+  if (rc) return rc;
+  std::vector<float> X((size_t)B * 16, 0.f);
+  for (int b = 0; b < B; b++)
+    for (int k = 0; k < 3; k++) X[(size_t)b * 16 + k] = X3[b * 3 + k];
+  if (hipMemcpy(g->X, X.data(), 4L * B * 16, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->Tg, tgt, 4L * B * KQED_NSURR, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->Wt, wgt, 4L * B * KQED_NSURR, hipMemcpyHostToDevice) != hipSuccess) {
+    set_err("surrogate batch upload failed");
+    return -3;
+  }
+  surr_fwd_bwd(g, B);
+  const double l = surr_loss(g, B);
+  if (l < 0 || hipMemcpy(grad, g->G, 4 * SURR_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) { set_err("surrogate gradient read failed"); return -3; }
+  if (loss) *loss = l;
+  return 0;
+}
+
+extern "C" int kqed_surrogate_params(const kqed_surrogate *g, float *P, float *lsc, unsigned char *mask, float *range)
+{
+  if (!g) return -1;
+  if (P && hipMemcpy(P, g->P, 4 * SURR_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+  if (lsc) memcpy(lsc, g->h_lsc.data(), 4 * g->h_lsc.size());
+  if (mask) memcpy(mask, g->h_mask.data(), NBB);
+  if (range) { range[0] = g->R.lx0; range[1] = g->R.lx1; range[2] = g->R.ly0; range[3] = g->R.ly1; }
+  return 0;
+}
+
+extern "C" size_t kqed_surrogate_nparam(void) { return SURR_NPARAM; }
+
+/* n pairs through either path, FP64 assembly: surrogate (every packed pair, no mask) or table */
+static int surr_eval(kqed_surrogate *g, size_t n, const double *x_dev, const double *y_dev, double *out, bool use,
+                     unsigned char *allmask)
+{
+  kqed_device *d = g->d;
+  SurrUse su = { surr_net(g), allmask };
+  std::lock_guard<std::mutex> lk(d->mu);
+  kqed_tune keep = d->tune;
+  d->tune.sort_min = 0; /* the surrogate runs inside the sorted pipeline only */
+  if (d->tune.sort_mode == KQED_SORT_NONE) d->tune.sort_mode = KQED_SORT_CELL;
+  const int rc = run_pairs(d, d->ws[(void *)g->s], g->s, KQED_L0, 0.0, n, x_dev, 4, y_dev, 4, out, KQED_AOS, nullptr, nullptr,
+                           use ? &su : nullptr);
+  d->tune = keep;
+  return rc;
+}
+
+/* Error check: on validation pairs, the surrogate L0 against the table L0 (both FP64 assembly);
+ * a bin passes when it holds at least min_count packed-path pairs and its largest normwise error
+ * is at most tol.  stats = {pairs, packed pairs, packed pairs in passing bins, max error, median
+ * error over packed pairs}. */
+extern "C" int kqed_surrogate_validate(kqed_surrogate *g, size_t n, const double *x_dev, const double *y_dev, double tol,
+                                       int min_count, double *stats)
+{
+  if (!g || !x_dev || !y_dev || n == 0) { set_err("kqed_surrogate_validate: bad argument"); return -1; }
+  double *a = nullptr, *b = nullptr, *data = nullptr;
+  int *bin = nullptr;
+  unsigned *emax = nullptr, *ecnt = nullptr;
+  float *err = nullptr;
+  unsigned char *all = nullptr;
+  int rc = 0;
+  if (hipMalloc(&a, 3072 * n) != hipSuccess || hipMalloc(&b, 3072 * n) != hipSuccess ||
+      hipMalloc(&data, 8 * KQED_SURR_ROW * n) != hipSuccess || hipMalloc(&bin, 4 * n) != hipSuccess ||
+      hipMalloc(&emax, 4 * NBB) != hipSuccess || hipMalloc(&ecnt, 4 * NBB) != hipSuccess ||
+      hipMalloc(&err, 4 * n) != hipSuccess || hipMalloc(&all, NBB) != hipSuccess) {
+    set_err("surrogate validation buffers failed");
+    rc = -2;
+  }
+  std::vector<unsigned> hm(NBB), hc(NBB);
+  std::vector<float> he(n);
+  std::vector<int> hb(n);
+  if (!rc && (hipMemsetAsync(all, 1, NBB, g->s) != hipSuccess || hipMemsetAsync(emax, 0, 4 * NBB, g->s) != hipSuccess ||
+              hipMemsetAsync(ecnt, 0, 4 * NBB, g->s) != hipSuccess))
+    rc = -3;
+  if (!rc) rc = surr_eval(g, n, x_dev, y_dev, a, true, all);
+  if (!rc) rc = surr_eval(g, n, x_dev, y_dev, b, false, nullptr);
+  if (!rc) rc = surr_extract(g, n, x_dev, y_dev, data);
+  if (!rc) {
+    const kqed_dev_tables &T = g->d->dt;
+    hipLaunchKernelGGL(k_surr_rowbin, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->s, T, n, data, bin);
+    hipLaunchKernelGGL(k_surr_binerr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->s, n, a, b, bin, emax, ecnt, err);
+    if (hipMemcpyAsync(hm.data(), emax, 4 * NBB, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipMemcpyAsync(hc.data(), ecnt, 4 * NBB, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipMemcpyAsync(he.data(), err, 4 * n, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipMemcpyAsync(hb.data(), bin, 4 * n, hipMemcpyDeviceToHost, g->s) != hipSuccess ||
+        hipStreamSynchronize(g->s) != hipSuccess)
+      rc = -3;
+  }
+  (void)hipFree(a); (void)hipFree(b); (void)hipFree(data); (void)hipFree(bin); (void)hipFree(emax); (void)hipFree(ecnt);
+  (void)hipFree(err); (void)hipFree(all);
+  if (rc) { if (!*g_err) set_err("surrogate validation failed"); return rc; }
+  size_t packed = 0, pass = 0;
+  std::vector<float> pe;
+  for (int c = 0; c < NBB; c++) {
+    float m;
+    memcpy(&m, &hm[c], 4);
+    g->h_mask[c] = hc[c] >= (unsigned)(min_count < 1 ? 1 : min_count) && m <= tol;
+  }
+  for (size_t p = 0; p < n; p++) {
+    if (hb[p] < 0) continue;
+    packed++;
+    pe.push_back(he[p]);
+    if (g->h_mask[hb[p]]) pass++;
+  }
+  std::sort(pe.begin(), pe.end());
+  if (hipMemcpy(g->mask, g->h_mask.data(), NBB, hipMemcpyHostToDevice) != hipSuccess) { set_err("mask upload failed"); return -3; }
+  g->validated = true;
+  if (stats) {
+    stats[0] = (double)n;
+    stats[1] = (double)packed;
+    stats[2] = (double)pass;
+    stats[3] = pe.empty() ? 0.0 : pe.back();
+    stats[4] = pe.empty() ? 0.0 : pe[pe.size() / 2];
+  }
+  return 0;
+}
+
+/* kqed_batch with the surrogate: packed-path pairs in bins that passed the error check take the
+ * network, every other pair (Taylor region, special points, failing bins) the table path. */
+extern "C" int kqed_batch_surrogate(const struct QED_kernel_temps *t, const kqed_surrogate *g, int which, double M,
+                                    size_t n, const double *x_dev, const double *y_dev, void *out_dev, int layout,
+                                    void *stream)
+{
+  if (!t || !t->dev || !g || g->d != t->dev) { set_err("kqed_batch_surrogate: bad argument"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout)) { set_err("kqed_batch_surrogate: bad argument"); return -1; }
+  if (!g->validated) { set_err("kqed_batch_surrogate: surrogate not validated (kqed_surrogate_validate)"); return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull) { set_err("kqed_batch_surrogate: at most 2^32-1 pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  SurrUse su = { surr_net(g), g->mask };
+  std::lock_guard<std::mutex> lk(d->mu);
+  kqed_tune keep = d->tune;
+  d->tune.sort_min = 0;
+  if (d->tune.sort_mode == KQED_SORT_NONE) d->tune.sort_mode = KQED_SORT_CELL;
+  const int rc = run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
+                           nullptr, &su);
+  d->tune = keep;
+  return rc;
+}
+
+/* Save / load: magic, parameter count, parameters, scales, mask, input ranges, validated flag. */
+extern "C" int kqed_surrogate_save(const kqed_surrogate *g, const char *path)
+{
+  if (!g || !path) return -1;
+  std::vector<float> P(SURR_NPARAM);
+  if (hipMemcpy(P.data(), g->P, 4 * SURR_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) { set_err("surrogate read failed"); return -3; }
+  FILE *f = fopen(path, "wb");
+  if (!f) { set_err("cannot write %s", path); return -1; }
+  const unsigned magic = 0x4B514553u, np = (unsigned)SURR_NPARAM, val = g->validated;
+  bool ok = fwrite(&magic, 4, 1, f) == 1 && fwrite(&np, 4, 1, f) == 1 && fwrite(P.data(), 4, P.size(), f) == P.size() &&
+            fwrite(g->h_lsc.data(), 4, g->h_lsc.size(), f) == g->h_lsc.size() &&
+            fwrite(g->h_mask.data(), 1, g->h_mask.size(), f) == g->h_mask.size() && fwrite(&g->R, sizeof g->R, 1, f) == 1 &&
+            fwrite(&val, 4, 1, f) == 1;
+  ok = fclose(f) == 0 && ok;
+  if (!ok) { set_err("write error on %s", path); return -1; }
+  return 0;
+}
+
+extern "C" int kqed_surrogate_load(kqed_surrogate *g, const char *path)
+{
+  if (!g || !path) return -1;
+  FILE *f = fopen(path, "rb");
+  if (!f) { set_err("cannot read %s", path); return -1; }
+  unsigned magic = 0, np = 0, val = 0;
+  std::vector<float> P(SURR_NPARAM);
+  SurrRange R;
+  bool ok = fread(&magic, 4, 1, f) == 1 && fread(&np, 4, 1, f) == 1 && magic == 0x4B514553u && np == SURR_NPARAM &&
+            fread(P.data(), 4, P.size(), f) == P.size() && fread(g->h_lsc.data(), 4, g->h_lsc.size(), f) == g->h_lsc.size() &&
+            fread(g->h_mask.data(), 1, g->h_mask.size(), f) == g->h_mask.size() && fread(&R, sizeof R, 1, f) == 1 &&
+            fread(&val, 4, 1, f) == 1;
+  fclose(f);
+  if (!ok) { set_err("%s is not a surrogate file of this build", path); return -1; }
+  g->R = R;
+  g->validated = val != 0;
+  if (hipMemcpy(g->P, P.data(), 4 * SURR_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->lsc, g->h_lsc.data(), 4 * g->h_lsc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->mask, g->h_mask.data(), g->h_mask.size(), hipMemcpyHostToDevice) != hipSuccess || surr_publish(g)) {
+    set_err("surrogate upload failed");
+    return -3;
+  }
+  return 0;
 }

@@ -35,6 +35,20 @@
 #define KQED_REC_AUX 80   /* slots 1..3 payloads [20] each */
 #define KQED_AUX_LEN 20
 
+/* Learned surrogate of the packed-path interpolation: the interpolated weight functions it stands
+ * in for, as double offsets into the stage-1 record (checked against the struct in the kernels):
+ * dS 3, dT 12, dV 17, and the two swap-regime second y derivatives. */
+#define KQED_NSURR 34
+#define KQED_SURR_SLOTS                                                                     \
+  { 14, 12, 13, 23, 24, 16, 25, 33, 32, 19, 34, 20, 21, 18, 22, 35, 40,                     \
+    41, 48, 49, 45, 39, 46, 47, 50, 42, 51, 52, 44, 53, 54, 43, 61, 60 }
+#define KQED_S1_FLAG2 10    /* stage-1 record: swap flag */
+#define KQED_S1_NEEDALG 11  /* stage-1 record: chain-rule algebra still to run */
+#define KQED_SURR_BIT 0x80000000u /* kinds bit: slot 0 interpolated by the surrogate */
+#define KQED_SURR_NB 24           /* (log x, log y) bins of the output scales and of the error check */
+#define KQED_SURR_YMIN 1e-3       /* lower end of the y range of the bins */
+#define KQED_SURR_ROW 40          /* doubles per row of extracted training data (5 + 34, padded) */
+
 #define KQED_WM_MAX 7168    /* u16 entries of the three wave-dealt assembly maps (7104 used) */
 #define KQED_ASM_WAVES 8    /* wavefronts per assembly block */
 

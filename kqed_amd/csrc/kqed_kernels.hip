@@ -809,6 +809,40 @@ struct Stage1 {
   Vt V;
 };
 
+/* the surrogate's slot list names these fields */
+static_assert(offsetof(Stage1, dg0dcb) / 8 == 14 && offsetof(Stage1, dg0dx) / 8 == 12 && offsetof(Stage1, dg0dy) / 8 == 13,
+              "stage-1 layout");
+static_assert(offsetof(Stage1, T) / 8 + offsetof(Tt, dell2dx) / 8 == 23 && offsetof(Stage1, T) / 8 + offsetof(Tt, ell2) / 8 == 16 &&
+              offsetof(Stage1, T) / 8 + offsetof(Tt, dell4dcb) / 8 == 34 && offsetof(Stage1, T) / 8 + offsetof(Tt, v1) / 8 == 18,
+              "stage-1 layout");
+static_assert(offsetof(Stage1, V) / 8 == 35 && offsetof(Stage1, V) / 8 + offsetof(Vt, dg3dy) / 8 == 43 &&
+              offsetof(Stage1, V) / 8 + offsetof(Vt, ddg2dydy) / 8 == 61 &&
+              offsetof(Stage1, V) / 8 + offsetof(Vt, ddg1dydy) / 8 == 60 &&
+              offsetof(Stage1, V) / 8 + offsetof(Vt, ddg3dcbdcb) / 8 == 54 && offsetof(Stage1, flag2) / 8 == KQED_S1_FLAG2 &&
+              offsetof(Stage1, need_alg) / 8 == KQED_S1_NEEDALG && sizeof(Stage1) <= 64 * 8,
+              "stage-1 layout");
+
+/* header of the stage-1 record (invariants) */
+DEV void write_header(Stage1 *st, const Inv &I)
+{
+  st->x = I.x; st->cb = I.cb; st->y = I.y; st->xsq = I.xsq; st->ysq = I.ysq; st->xdoty = I.xdoty;
+  st->cborig = I.cborig; st->yorig = I.yorig; st->xmysq = I.xmysq; st->xmy = I.xmy;
+  st->flag2 = I.flag2 ? 1.0 : 0.0;
+}
+
+/* Surrogate bins: NB x NB over (log x, log y) of the post-swap invariants, x in [XX[0], XX[last]],
+ * y in [KQED_SURR_YMIN, YY[last]] (clamped). */
+DEV int surr_bin(const DT &T, double x, double y)
+{
+  const double lx0 = log(T.XX[0]), lx1 = log(T.XX[T.nstpx - 1]);
+  const double ly0 = log(KQED_SURR_YMIN), ly1 = log(T.YY[T.nstpy - 1]);
+  double u = (log(x) - lx0) / (lx1 - lx0), v = (log(y) - ly0) / (ly1 - ly0);
+  int bu = (int)(u * KQED_SURR_NB), bv = (int)(v * KQED_SURR_NB);
+  bu = bu < 0 ? 0 : bu >= KQED_SURR_NB ? KQED_SURR_NB - 1 : bu;
+  bv = bv < 0 ? 0 : bv >= KQED_SURR_NB ? KQED_SURR_NB - 1 : bv;
+  return bu * KQED_SURR_NB + bv;
+}
+
 /* Interpolation of one pair into st.  CLS: 0 any pair, 1 packed-path pairs only, 2 Taylor-x pairs
  * only (the sort sends each class to its own kernel, which then carries only its own code path).
  * Returns 1 when the pair is outside the grid (the reference then leaves an all-zero kernel). */
@@ -816,9 +850,7 @@ template <int CLS>
 DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st)
 {
   const Inv I = set_inv(T, xv, yv);
-  st->x = I.x; st->cb = I.cb; st->y = I.y; st->xsq = I.xsq; st->ysq = I.ysq; st->xdoty = I.xdoty;
-  st->cborig = I.cborig; st->yorig = I.yorig; st->xmysq = I.xmysq; st->xmy = I.xmy;
-  st->flag2 = I.flag2 ? 1.0 : 0.0;
+  write_header(st, I);
   if (CLS != 1 && (CLS == 2 || I.x < T.XX[0])) {
     /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0); the Taylor
      * branches already apply the algebra (chnr_dT.c:16-143, chnr_dV.c:19-174) */
@@ -1161,7 +1193,8 @@ DEV void plan_tabd(const DT &T, int kind, const double *v, double *payload, unsi
 
 /* L0(a,b) into slot `slot`: regime dispatch of reference kernels.c:82-115 */
 template <int CLS>
-DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot)
+DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot,
+                 const unsigned char *smask)
 {
   const bool az = is_zero(a), bz = is_zero(b);
   double *pl = slot == 0 ? rec + KQED_REC_MAIN : rec + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
@@ -1170,6 +1203,18 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
   if (bz) { plan_tabd(T, KIND_YEQ0, a, pl, kinds, slot); return; }
   if (x_is_y(a, b)) { plan_tabd(T, KIND_YEQ0_MULAM, a, pl, kinds, slot); return; }
   if (slot != 0) return; /* only the main slot is ever a standard evaluation */
+  if (CLS == 1 && smask) {
+    /* packed-path pair in a bin where the surrogate passed its error check: only the invariants
+     * here, the weight functions come from the network */
+    const Inv I = set_inv(T, a, b);
+    if (smask[surr_bin(T, I.x, I.y)]) {
+      Stage1 *st = (Stage1 *)pl;
+      write_header(st, I);
+      st->need_alg = 1.0;
+      kinds |= (unsigned)KIND_STD | KQED_SURR_BIT;
+      return;
+    }
+  }
   if (interp_eval<CLS>(T, a, b, (Stage1 *)pl) == 0) kinds |= (unsigned)KIND_STD;
 }
 
@@ -1186,7 +1231,7 @@ template <int CLS>
 __global__ void __launch_bounds__(256, InterpWaves<CLS>::v)
 k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
          const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi, size_t c0, size_t c1,
-         int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds)
+         int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds, const unsigned char *smask)
 {
   size_t lo = ilo < 0 ? 0 : bounds[ilo], hi = ihi < 0 ? n : bounds[ihi];
   lo = lo > c0 ? lo : c0;
@@ -1202,14 +1247,15 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   unsigned kd = 0;
   const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
   double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  plan_L0<CLS>(T, xv, yv, r, kd, 0);
+  plan_L0<CLS>(T, xv, yv, r, kd, 0, smask);
   switch (which) {
   case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
-    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, yv, yv, r, kd, 2); s1 = -0.5; s2 = -0.5; break;
+    plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, yv, yv, r, kd, 2, nullptr); s1 = -0.5; s2 = -0.5; break;
   case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
-    plan_L0<CLS>(T, z, yv, r, kd, 1); plan_L0<CLS>(T, xv, z, r, kd, 2); s1 = -1.0; s2 = -1.0; break;
+    plan_L0<CLS>(T, z, yv, r, kd, 1, nullptr); plan_L0<CLS>(T, xv, z, r, kd, 2, nullptr); s1 = -1.0; s2 = -1.0; break;
   case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
-    plan_L0<CLS>(T, xv, xv, r, kd, 1); plan_L0<CLS>(T, z, xv, r, kd, 2); plan_L0<CLS>(T, z, yv, r, kd, 3);
+    plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, z, xv, r, kd, 2, nullptr);
+    plan_L0<CLS>(T, z, yv, r, kd, 3, nullptr);
     s1 = -1.0; s2 = 1.0; s3 = -1.0; break;
   case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
     if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
@@ -1229,11 +1275,14 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
 }
 
 template __global__ void k_interp<0>(DT, size_t, const double *, int, const double *, int, const unsigned *,
-                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
+                                     const unsigned char *);
 template __global__ void k_interp<1>(DT, size_t, const double *, int, const double *, int, const unsigned *,
-                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
+                                     const unsigned char *);
 template __global__ void k_interp<2>(DT, size_t, const double *, int, const double *, int, const unsigned *,
-                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *);
+                                     const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
+                                     const unsigned char *);
 
 /* ------------------------------------------------------------------ tensor entries */
 /* Per-pair helper vectors of the standard path (shared by many tensor entries), each evaluated with
@@ -1604,4 +1653,34 @@ __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__re
     if (off[mid] <= i) lo = mid; else hi = mid;
   }
   for (int k = 0; k < 4; k++) ypair[4 * i + k] = y[4 * lo + k];
+}
+
+/* Training data of the surrogate: for every packed-path standard pair of a pass (records before the
+ * chain-rule algebra), out[p] = {x, y, cb (post-swap invariants), flag2, 1, the 34 interpolated
+ * weight functions}; other pairs get out[p][4] = 0. */
+__global__ void k_extract_scalars(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds,
+                                  double *__restrict__ out)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *r = rec + i * KQED_REC_LEN;
+  const double *m = r + KQED_REC_MAIN;
+  const size_t p = (size_t)r[KQED_REC_P];
+  double *o = out + p * KQED_SURR_ROW;
+  const Stage1 *st = (const Stage1 *)m;
+  const bool packed = (kinds[i] & 7u) == (unsigned)KIND_STD && st->need_alg != 0.0 && st->x >= T.XX[0] &&
+                      st->x < T.XX[T.nstpx - 1] && st->y < T.YY[T.nstpy - 1];
+  o[0] = st->x; o[1] = st->y; o[2] = st->cb; o[3] = st->flag2; o[4] = packed ? 1.0 : 0.0;
+  const int slots[KQED_NSURR] = KQED_SURR_SLOTS;
+  for (int k = 0; k < KQED_NSURR; k++) o[5 + k] = packed ? m[slots[k]] : 0.0;
+  if (packed && st->flag2 == 0.0) { o[5 + 32] = 0.0; o[5 + 33] = 0.0; }
+}
+
+/* bin of the error check for each extracted row (-1 unless a packed-path pair) */
+__global__ void k_surr_rowbin(DT T, size_t n, const double *__restrict__ data, int *__restrict__ bin)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double *row = data + i * KQED_SURR_ROW;
+  bin[i] = row[4] != 0.0 ? surr_bin(T, row[0], row[1]) : -1;
 }

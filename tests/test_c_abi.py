@@ -30,3 +30,24 @@ def test_initialise_reports_missing_tables(tmp_path):
     assert rc == 1
     assert b"FFxy_single_cksum.bin" in kqed_amd.lib().kqed_last_error()
     kqed_amd.lib().free_QED_temps(ctypes.byref(t))  # safe on a failed struct
+
+
+def test_c_caller_compiles_and_links(tmp_path):
+    """A C99 caller written against the reference's KQED.h compiles against include/ and links."""
+    import subprocess
+    src = tmp_path / "caller.c"
+    src.write_text(
+        '#include "KQED.h"\n'
+        "int main(void) {\n"
+        "  struct QED_kernel_temps t; double kerv[6][4][4][4] KQED_ALIGN;\n"
+        "  const double x[4] = {0, 0, 0.1, 0.2}, y[4] = {0, 0, 0, 0.3};\n"
+        "  if (initialise(&t)) return 1;\n"
+        "  QED_kernel_L0(x, y, t, kerv); QED_kernel_L2(x, y, t, kerv); QED_Mkernel_L2(0.5, x, y, t, kerv);\n"
+        "  printf(\"%g %g\\n\", kerv[0][0][0][0], AlfQED);\n"
+        "  free_QED_temps(&t); return 0;\n}\n")
+    inc = os.path.join(kqed_amd.ROOT, "include")
+    libdir = os.path.dirname(kqed_amd.LIB_PATH)
+    exe = tmp_path / "caller"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", inc, str(src), "-L", libdir, "-lkqed_hip",
+                    "-Wl,-rpath," + libdir, "-lm", "-o", str(exe)], check=True)
+    assert exe.exists()
