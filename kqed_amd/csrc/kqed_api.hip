@@ -224,6 +224,7 @@ static int build_wave_maps(kqed_device *d)
     std::stable_sort(order, order + 384, [&](int a, int b) {
       return m.rowptr[a + 1] - m.rowptr[a] > m.rowptr[b + 1] - m.rowptr[b];
     });
+    static const int KB[6] = KQED_WM_K;
     for (int sl = 0; sl < 6; sl++) {
       T.wmOff[q][sl] = (int)E.size();
       int K = 0;
@@ -232,6 +233,9 @@ static int build_wave_maps(kqed_device *d)
         K = std::max(K, m.rowptr[r + 1] - m.rowptr[r]);
         R[q * 384 + 64 * sl + l] = (unsigned short)r;
       }
+      /* the kernel unrolls each slot to its compile-time length; shorter rows take zero entries */
+      if (K > KB[sl]) { set_err("assembly map row longer than the kernel's unrolled length"); return 1; }
+      K = KB[sl];
       T.wmK[q][sl] = K;
       for (int k = 0; k < K; k++)
         for (int l = 0; l < 64; l++) {
@@ -248,6 +252,8 @@ static int build_wave_maps(kqed_device *d)
         }
     }
     T.wmOff[q][6] = (int)E.size();
+    for (int sl = 0, cum = 0; sl <= 6; cum += sl < 6 ? KB[sl] : 0, sl++)
+      if (T.wmOff[q][sl] != 64 * (KQED_WM_PER_LANE * q + cum)) { set_err("wave map layout mismatch"); return 1; }
   }
   T.wmTotal = (int)E.size();
   if (T.wmTotal > KQED_WM_MAX) { set_err("assembly maps too large for LDS"); return 1; }

@@ -49,7 +49,11 @@
 #define KQED_SURR_YMIN 1e-3       /* lower end of the y range of the bins */
 #define KQED_SURR_ROW 40          /* doubles per row of extracted training data (5 + 34, padded) */
 
-#define KQED_WM_MAX 7168    /* u16 entries of the three wave-dealt assembly maps (7104 used) */
+/* Entries per lane of the six row slots of every wave-dealt assembly map (rows sorted by length;
+ * the standard map needs exactly these, the Tabd maps fewer and are padded with the zero slot) */
+#define KQED_WM_K { 13, 9, 6, 6, 4, 3 }
+#define KQED_WM_PER_LANE 41
+#define KQED_WM_MAX (3 * 64 * KQED_WM_PER_LANE) /* u16 entries of the three maps */
 #define KQED_ASM_WAVES 8    /* wavefronts per assembly block */
 
 enum {

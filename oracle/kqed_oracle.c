@@ -27,10 +27,13 @@
 #endif
 
 /* ------------------------------------------------------------------ flop accounting */
-static __thread double g_flops = 0.0;
+static __thread double g_flops = 0.0, g_flops_build = 0.0;
 #define FL(n) (g_flops += (double)(n))
 double oq_flops_get(void) { return g_flops; }
-void oq_flops_reset(void) { g_flops = 0.0; }
+/* the part of oq_flops_get spent building per-pair coefficient arrays (getff6/getff7 y powers,
+ * simd.c:343-613): work that depends only on the grid node */
+double oq_flops_build_get(void) { return g_flops_build; }
+void oq_flops_reset(void) { g_flops = 0.0; g_flops_build = 0.0; }
 
 /* ------------------------------------------------------------------ CRC-32C (crc32c.c:160-172) */
 static uint32_t crc_tab[256];
@@ -502,6 +505,7 @@ static int build_arrays(const oq_tables *T, const oq_pc *PC, int nm, int ix, con
     MSHP = MSHP + 1.0;
   }
   FL(4.0 * nf_max * 14);
+  g_flops_build += 4.0 * nf_max * 14;
   for (int k = 0; k < 4 * mm; k++) { fval[k] = fvald[k] = fvaldx[k] = fvaldxd[k] = 0.0; }
   *mm_out = mm;
   *nk_out = nf_max + mm;

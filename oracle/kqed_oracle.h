@@ -80,6 +80,7 @@ int oq_regime(const oq_tables *T, const double x[4], const double y[4]);
 
 /* Algorithmic FP64 flop count of the last oq_L0 calls on this thread (reference algorithm). */
 double oq_flops_get(void);
+double oq_flops_build_get(void); /* part spent on per-pair coefficient arrays (node-only work) */
 void oq_flops_reset(void);
 
 #ifdef __cplusplus

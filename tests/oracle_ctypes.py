@@ -51,6 +51,7 @@ def _lib():
     L.oq_interp_scalars.argtypes = [T, v4, v4, v4]
     L.oq_regime.argtypes = [T, v4, v4]
     L.oq_flops_get.restype = ctypes.c_double
+    L.oq_flops_build_get.restype = ctypes.c_double
     L.oq_crc32c.argtypes = [ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     L.oq_crc32c.restype = ctypes.c_uint32
     return L
@@ -160,6 +161,9 @@ class Oracle:
 
     def flops(self):
         return self.L.oq_flops_get()
+
+    def flops_build(self):
+        return self.L.oq_flops_build_get()
 
     def flops_reset(self):
         self.L.oq_flops_reset()
