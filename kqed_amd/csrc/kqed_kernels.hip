@@ -362,6 +362,51 @@ DEV double interpol3(double y, double y1, double y2, double dy, double f1, doubl
           (dy * dy * dy));
 }
 
+/* Scalar-path Clenshaw sums (reference cheby.c:13-91) of order D over the value and d/dy arrays of
+ * one or two grid nodes in one pass: r = {a.x, a.y, b.x, b.y}.  Order 0 is the reference's
+ * expression; for D >= 1 the step factors (j+1)/j, (j+3)/(j+1), ... are read from the ratio table
+ * instead of dividing at every step (the two forms agree to rounding). */
+template <int D>
+DEV void cheb_r(int nk, double x, const double2 *__restrict__ a, const double2 *__restrict__ b, const double *LUTg,
+                double r[4])
+{
+  const cdouble *LUT = lut_ptr(LUTg);
+  const int nc = b ? 4 : 2;
+  double ya[4] = { 0.0, 0.0, 0.0, 0.0 }, yb[4] = { 0.0, 0.0, 0.0, 0.0 };
+  const double twox = 2.0 * x;
+  const int jmin = D == 0 ? 1 : D;
+  for (int j = nk - 1; j >= jmin; j--) {
+    const double2 ca = a[j], cc = b ? b[j] : make_double2(0.0, 0.0);
+    const double c[4] = { ca.x, ca.y, cc.x, cc.y };
+    if (D == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (i >= nc) break;
+        const double t = yb[i];
+        yb[i] = twox * yb[i] - ya[i] + c[i];
+        ya[i] = t;
+      }
+    } else {
+      const double PA = LUT[8 * j + 2 * D - 1], PB = LUT[8 * j + 2 * D];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (i >= nc) break;
+        const double t = yb[i];
+        yb[i] = twox * yb[i] * PA - ya[i] * PB + c[i];
+        ya[i] = t;
+      }
+    }
+  }
+  if (D == 0) {
+    const double2 ca = a[0], cc = b ? b[0] : make_double2(0.0, 0.0);
+    const double c[4] = { ca.x, ca.y, cc.x, cc.y };
+    for (int i = 0; i < 4; i++) r[i] = -ya[i] + twox * yb[i] + c[i];
+  } else {
+    const double f = D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0;
+    for (int i = 0; i < 4; i++) r[i] = f * yb[i];
+  }
+}
+
 /* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays */
 DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm, bool ndy, int ndcb,
                      double cb, double y)
@@ -370,13 +415,17 @@ DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int i
   const int nk = T.nfx[ix] + mm;
   const int ord = mm + ndcb;
   const double2 *a = T.S + node(T, nm, ix, iy);
-  const double r11 = cheb_s(ord, nk, cb, a, 1);
-  const double r10 = ndy ? r11 : cheb_s(ord, nk, cb, a, 0);
+  const double2 *b = flag_hy ? nullptr : T.S + node(T, nm, ix, iy + 1);
+  double r[4];
+  switch (ord) {
+  case 0: cheb_r<0>(nk, cb, a, b, T.LUT, r); break;
+  case 1: cheb_r<1>(nk, cb, a, b, T.LUT, r); break;
+  case 2: cheb_r<2>(nk, cb, a, b, T.LUT, r); break;
+  default: cheb_r<3>(nk, cb, a, b, T.LUT, r); break;
+  }
+  const double r10 = ndy ? r[1] : r[0], r11 = r[1], r20 = ndy ? r[3] : r[2], r21 = r[3];
   if (!flag_hy) {
     const double y1 = T.YY[iy], y2 = T.YY[iy + 1];
-    const double2 *b = T.S + node(T, nm, ix, iy + 1);
-    const double r21 = cheb_s(ord, nk, cb, b, 1);
-    const double r20 = ndy ? r21 : cheb_s(ord, nk, cb, b, 0);
     if (use_y_derivs) return interpol3(y, y1, y2, y1 - y2, r10, r20, r11, r21);
     return lerp((y2 - y) / (y2 - y1), r10, r20);
   }
@@ -1421,14 +1470,22 @@ DEV void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename R>
-DEV R wm_dot(const unsigned short *__restrict__ e, int K, int lane, const R *Tv)
+/* One row of a lane: KB entries (column | exponent << 8 | sign << 10, coefficient
+ * (-1)^sign 2^(exponent-1)), unrolled so that every LDS read of the row is in flight at once. */
+template <int KB, typename R>
+DEV R wm_dot(const unsigned short *__restrict__ e, int lane, const R *Tv)
 {
+  unsigned w[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
+  R t[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) t[k] = Tv[w[k] & 0xFFu];
   R s = 0;
-  for (int k = 0; k < K; k++) {
-    const unsigned w = e[64 * k + lane];
-    const R c = ldexp((w & 0x400u) ? (R)-1 : (R)1, (int)((w >> 8) & 3u) - 1);
-    s = fma(c, Tv[w & 0xFFu], s);
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const R c = ldexp((w[k] & 0x400u) ? (R)-1 : (R)1, (int)((w[k] >> 8) & 3u) - 1);
+    s = fma(c, t[k], s);
   }
   return s;
 }
@@ -1512,8 +1569,16 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       const int mq = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
       double v[6];
-#pragma unroll
-      for (int m = 0; m < 6; m++) v[m] = wm_dot(sE + T.wmOff[mq][m], T.wmK[mq][m], lane, Tv);
+      {
+        /* slot m of map mq starts at 64 * (41 mq + 13 + 9 + ...) entries (checked on the host) */
+        const unsigned short *e = sE + mq * 64 * KQED_WM_PER_LANE;
+        v[0] = wm_dot<13>(e, lane, Tv);
+        v[1] = wm_dot<9>(e + 64 * 13, lane, Tv);
+        v[2] = wm_dot<6>(e + 64 * 22, lane, Tv);
+        v[3] = wm_dot<6>(e + 64 * 28, lane, Tv);
+        v[4] = wm_dot<4>(e + 64 * 34, lane, Tv);
+        v[5] = wm_dot<3>(e + 64 * 38, lane, Tv);
+      }
       if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
         /* reference kernels.c:292-323: the alpha sums take other rows of L0(0,y) or L0(x,0),
          * recomputed from the tensors with the row-ordered map */
