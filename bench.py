@@ -123,6 +123,56 @@ def ragged_leg(t, dev, stream, reps=3):
             "evals_per_s_eager": n / dt_eager, "tune": tune.as_dict(), "tune_ms_per_2^20": tune_ms}
 
 
+def surrogate_leg(t, dev, stream, n_train, epochs, n_infer, n_plain_rate):
+    """Surrogate + FP32 form: train the network on interpolator samples of n_train pairs, validate at
+    1e-5 (per-bin error check), then evaluate n_infer pairs of the benchmark law with FP32 assembly
+    (network in passing bins, tables elsewhere); accuracy against the plain FP64 path on a sample."""
+    import torch
+    import kqed_amd
+    from kqed_amd import workloads as W
+    from pairs import normwise
+    x, y = W.pairs_log_uniform(31, n_train)
+    xd, yd = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    s = kqed_amd.Surrogate(t, seed=7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = s.train(n_train, xd.data_ptr(), yd.data_ptr(), epochs=epochs, batch=8192, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del xd, yd
+    xv, yv = W.pairs_log_uniform(32, 1 << 20)
+    xvd, yvd = torch.from_numpy(xv).to(dev), torch.from_numpy(yv).to(dev)
+    val = {f"{tol:g}": s.validate(len(xv), xvd.data_ptr(), yvd.data_ptr(), tol=tol, min_count=8)
+           for tol in (1e-2, 1e-3, 1e-5)}  # the last one installs the 1e-5 mask
+    del xvd, yvd
+    chunk = min(n_infer, 1 << 24)
+    xi, yi = W.pairs_log_uniform(2, chunk)
+    xid, yid = torch.from_numpy(xi).to(dev), torch.from_numpy(yi).to(dev)
+    out = torch.empty((chunk, 384), dtype=torch.float32, device=dev)
+    s.batch_device(0, chunk, xid.data_ptr(), yid.data_ptr(), out.data_ptr(), layout=kqed_amd.KQED_AOS_F32,
+                   stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < n_infer:
+        s.batch_device(0, chunk, xid.data_ptr(), yid.data_ptr(), out.data_ptr(), layout=kqed_amd.KQED_AOS_F32,
+                       stream=stream)
+        done += chunk
+    torch.cuda.synchronize()
+    rate = done / (time.perf_counter() - t0)
+    m = 1 << 16
+    ref = np.empty((m, 384))
+    ref[:] = t.batch_host(0, xi[:m], yi[:m])
+    err = normwise(out[:m].double().cpu().numpy(), ref)
+    s.destroy()
+    del out, xid, yid
+    return {"workload": f"train on {n_train} interpolator samples x {epochs} epochs; {n_infer} pairs of the "
+                        "benchmark law, L0, FP32 assembly/output, network in bins passing a 1e-5 check",
+            "train_seconds": t_train, "final_loss": loss, "validation": val, "evals_per_s": rate,
+            "speedup_vs_plain_fp64": rate / n_plain_rate,
+            "max_rel_err_vs_fp64": float(err.max()), "median_rel_err_vs_fp64": float(np.median(err))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,6 +183,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates and the contraction")
     ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
+    ap.add_argument("--surrogate", type=int, default=1, help="run the surrogate + FP32 leg (0 = skip)")
+    ap.add_argument("--surrogate-train", type=int, default=1 << 24, help="training pairs of the surrogate leg")
+    ap.add_argument("--surrogate-epochs", type=int, default=1)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()
@@ -200,6 +253,9 @@ def main():
     if not a.no_extra:
         extra["contraction"] = contraction_leg(t, dev, stream, a.lattice)
         extra["ragged"] = ragged_leg(t, dev, stream)
+        if a.surrogate:
+            del out
+            extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
 
     ms_i, cnt_i = stages["interp"]
     ms_a, cnt_a = stages["assemble"]

@@ -1149,48 +1149,63 @@ DEV int yeq0_prims(const DT &T, const double *xv, double *o)
 }
 
 /* ------------------------------------------------------------------ classification + sort */
-/* Atomic add on base[key] shared by the lanes of a wave that hold the same key: one atomic per
- * distinct key; returns the old value plus this lane's rank among those lanes. */
-DEV unsigned wave_atomic_add(unsigned *base, unsigned key)
-{
-  const unsigned lane = __lane_id();
-  unsigned res = 0;
-  for (;;) {
-    const unsigned lead = __builtin_amdgcn_readfirstlane(key);
-    const unsigned long long same = __ballot(key == lead);
-    if (key == lead) {
-      const int first = __ffsll((long long)same) - 1;
-      const unsigned rank = (unsigned)__popcll(same & ((1ull << lane) - 1ull));
-      unsigned old = 0;
-      if ((int)lane == first) old = atomicAdd(&base[lead], (unsigned)__popcll(same));
-      old = __builtin_amdgcn_readlane(old, first);
-      res = old + rank;
-      break;
-    }
-  }
-  return res;
-}
 
 /* key 0: special points, out of grid, grid edges; 1 + ix*nstpy + iy: packed path;
  * 1 + nstpx*nstpy + iy: Taylor-x region */
-__global__ void k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
-                           int mode, unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
+/* Keys of 256 pairs per block, and each pair's slot inside its bucket.  The block sorts its keys in
+ * LDS (bitonic, key then position) and takes one atomic per run of equal keys, so that buckets
+ * holding a large share of the batch (small |y| cells, the Taylor region) see one atomic per block
+ * instead of one per pair.  Launch with 256 threads. */
+__global__ void __launch_bounds__(256)
+k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys, int mode,
+           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
 {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double xv[4], yv[4];
-  for (int k = 0; k < 4; k++) { xv[k] = x[xs * i + k]; yv[k] = y[ys * i + k]; }
-  unsigned k = 0;
-  if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
-    const Inv I = set_inv(T, xv, yv);
-    /* mode KQED_SORT_XNODE keys the packed path by x node only and the Taylor region as one bucket */
-    const bool cell = mode == KQED_SORT_CELL;
-    if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (cell ? (unsigned)I.INVy.idx : 0u);
-    else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1])
-      k = 1 + (unsigned)(I.INVx.idx * T.nstpy + (cell ? I.INVy.idx : 0));
+  __shared__ unsigned sk[256], sp[256], sbase[256];
+  const int t = threadIdx.x;
+  const size_t i = (size_t)blockIdx.x * 256 + t;
+  unsigned k = 0xFFFFFFFFu; /* past the end of the batch: sorts last, never counted */
+  if (i < n) {
+    double xv[4], yv[4];
+    for (int c = 0; c < 4; c++) { xv[c] = x[xs * i + c]; yv[c] = y[ys * i + c]; }
+    k = 0;
+    if (!is_zero(xv) && !is_zero(yv) && !x_is_y(xv, yv)) {
+      const Inv I = set_inv(T, xv, yv);
+      /* mode KQED_SORT_XNODE keys the packed path by x node only and the Taylor region as one bucket */
+      const bool cell = mode == KQED_SORT_CELL;
+      if (I.x < T.XX[0]) k = 1 + (unsigned)(T.nstpx * T.nstpy) + (cell ? (unsigned)I.INVy.idx : 0u);
+      else if (I.x < T.XX[T.nstpx - 1] && I.y < T.YY[T.nstpy - 1])
+        k = 1 + (unsigned)(I.INVx.idx * T.nstpy + (cell ? I.INVy.idx : 0));
+    }
+    key[i] = k;
   }
-  key[i] = k;
-  rank[i] = wave_atomic_add(count, k); /* slot of the pair inside its bucket */
+  sk[t] = k;
+  sp[t] = t;
+  __syncthreads();
+  for (int size = 2; size <= 256; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      const int o = t ^ stride;
+      if (o > t) {
+        const bool up = (t & size) == 0;
+        const unsigned ka = sk[t], kb = sk[o], pa = sp[t], pb = sp[o];
+        const bool gt = ka > kb || (ka == kb && pa > pb);
+        if (gt == up) { sk[t] = kb; sk[o] = ka; sp[t] = pb; sp[o] = pa; }
+      }
+      __syncthreads();
+    }
+  /* run starts: one atomic per run of equal keys for the run's length */
+  const unsigned kt = sk[t];
+  const bool start = kt != 0xFFFFFFFFu && (t == 0 || sk[t - 1] != kt);
+  if (start) {
+    int e = t + 1;
+    while (e < 256 && sk[e] == kt) e++;
+    sbase[t] = atomicAdd(&count[kt], (unsigned)(e - t));
+  }
+  __syncthreads();
+  if (kt != 0xFFFFFFFFu) {
+    int s0 = t;
+    while (s0 > 0 && sk[s0 - 1] == kt) s0--;
+    rank[(size_t)blockIdx.x * 256 + sp[t]] = sbase[s0] + (unsigned)(t - s0);
+  }
 }
 
 __global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
@@ -1340,14 +1355,17 @@ template __global__ void k_interp<2>(DT, size_t, const double *, int, const doub
  *   44 astuff[alpha], 48 bstuff[beta], 52/56/60 T_x coefficient vectors, 64/68/72 T_y ones,
  *   76 c1[alpha][beta] of get_dv */
 #define NH 92
-DEV double std_helper(const double *P, const double *xv, const double *yv, int j)
+/* rx = 1/x, ry = 1/y: the helpers multiply by the reciprocals where the reference divides by x
+ * and y (the results agree to rounding) */
+DEV double std_helper(const double *P, const double *xv, const double *yv, int j, double rx, double ry)
 {
   const double x = P[15], y = P[16], cb = P[17];
+  (void)x; (void)y;
   const int i = j & 3;
-  const double xh = xv[i] / x, yh = yv[i] / y;
+  const double xh = xv[i] * rx, yh = yv[i] * ry;
   if (j < 4) return xh;
   if (j < 8) return yh;
-  const double c2 = (xh - cb * yh) / y, c4 = (yh - cb * xh) / x;
+  const double c2 = (xh - cb * yh) * ry, c4 = (yh - cb * xh) * rx;
   if (j < 12) return c2;
   if (j < 16) return c4;
   if (j < 20) return xv[i] * P[26] + yv[i] * P[27];
@@ -1355,10 +1373,10 @@ DEV double std_helper(const double *P, const double *xv, const double *yv, int j
   if (j < 28) return xv[i] * P[30] + yv[i] * P[31];
   if (j < 32) return xv[i] * P[32] + yv[i] * P[33];
   if (j < 36) return xv[i] * P[34] + yv[i] * P[35];
-  if (j < 40) return (xv[i] * P[21] + yv[i] * P[22]) / x;
-  if (j < 44) return (xv[i] * P[23] + yv[i] * P[24]) / x;
+  if (j < 40) return (xv[i] * P[21] + yv[i] * P[22]) * rx;
+  if (j < 44) return (xv[i] * P[23] + yv[i] * P[24]) * rx;
   if (j < 48) return (P[21] + P[22]) * xh + (P[23] + P[24]) * c4;
-  if (j < 52) return (P[21] * xh + P[25] * yh + P[23] * (xh / y + yh / x - cb * (xh / x + yh / y)));
+  if (j < 52) return (P[21] * xh + P[25] * yh + P[23] * (xh * ry + yh * rx - cb * (xh * rx + yh * ry)));
   if (j < 56) return xh * P[6] + c4 * P[8];
   if (j < 60) return xh * P[9] + c4 * P[11];
   if (j < 64) return xh * P[12] + c4 * P[14];
@@ -1366,10 +1384,10 @@ DEV double std_helper(const double *P, const double *xv, const double *yv, int j
   if (j < 72) return yh * P[10] + c2 * P[11];
   if (j < 76) return yh * P[13] + c2 * P[14];
   const int a = (j - 76) >> 2, b = (j - 76) & 3;
-  const double xa = xv[a] / x, xb = xv[b] / x, ya = yv[a] / y, yb = yv[b] / y;
+  const double xa = xv[a] * rx, xb = xv[b] * rx, ya = yv[a] * ry, yb = yv[b] * ry;
   const int d_ab = (a == b);
   const double xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
-  return ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) / x + (d_ab + xayb * cb - xaxb - yayb) / y);
+  return ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) * rx + (d_ab + xayb * cb - xaxb - yayb) * ry);
 }
 
 /* S/T/V entry c of the standard path: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
@@ -1545,7 +1563,8 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       if (kd == KIND_NONE) continue;
       const double *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
       if (kd == KIND_STD) {
-        for (int j = lane; j < WH; j += 64) H[j] = std_helper(pl, R + KQED_REC_X, R + KQED_REC_Y, j);
+        const double rx = 1.0 / pl[15], ry = 1.0 / pl[16];
+        for (int j = lane; j < WH; j += 64) H[j] = std_helper(pl, R + KQED_REC_X, R + KQED_REC_Y, j, rx, ry);
       } else if (lane < 4) {
         H[lane] = pl[lane] / pl[4];
       }
