@@ -268,7 +268,7 @@ def main():
         # times the pairs of one pipeline pass, over the measured duration of that pass's kernels
         # (interpolation classes + chain-rule algebra + assembly; HIP events on the launch stream).
         per_eval = flops_per_eval["total"] - flops_per_eval["node_only"]
-        pairs_per_pass = a.n / max(cnt_i, 1)
+        pairs_per_pass = a.n * a.steps / max(cnt_i, 1)
         avg = (ms_i + ms_a) / max(cnt_i, 1) / 1e3
         achieved = per_eval * pairs_per_pass / avg
         roof = {"bound": "mfma", "resource": "FP64 vector ALU (peak equal to the FP64 matrix peak)",

@@ -246,8 +246,8 @@ DEV void clen_step(double &ya, double &yb, double c, double twox, double PA, dou
  * are zero past each node's own length (up to ks), and zero coefficients leave every chain at exact
  * zero, so a node's sums do not depend on how far above its length the chain starts.  A block's
  * U coefficient loads are issued together, so one memory latency covers U steps. */
-template <int O0, int NO, int U>
-DEV void clen_node2(const double2 *__restrict__ c1, const double2 *__restrict__ c2, int nk, double twox,
+template <int O0, int NO, int U, typename P1>
+DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
                     const double *__restrict__ LUTg, double r1[NO][2], double r2[NO][2])
 {
   static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
@@ -262,7 +262,11 @@ DEV void clen_node2(const double2 *__restrict__ c1, const double2 *__restrict__ 
   for (int kb = kt; kb >= O0; kb -= U) {
     double2 cc[U][2];
 #pragma unroll
-    for (int u = 0; u < U; u++) { cc[u][0] = c1[kb - u]; cc[u][1] = c2[kb - u]; }
+    for (int u = 0; u < U; u++) {
+      /* c1, c2 point at double2 arrays read as pairs of doubles (value, d/dy) */
+      cc[u][0] = make_double2(c1[2 * (kb - u)], c1[2 * (kb - u) + 1]);
+      cc[u][1] = make_double2(c2[2 * (kb - u)], c2[2 * (kb - u) + 1]);
+    }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = kb - u;
@@ -297,8 +301,9 @@ DEV void clen_node2(const double2 *__restrict__ c1, const double2 *__restrict__ 
 }
 
 /* two packed chains of one order over plain double arrays (same wave-uniform length, blocks of U) */
-template <int U>
-DEV void clen_two(const double *__restrict__ q1, const double *__restrict__ q2, int nk, int d, double twox,
+
+template <int U, typename P1>
+DEV void clen_two(P1 __restrict__ q1, P1 __restrict__ q2, int nk, int d, double twox,
                   const double *__restrict__ LUTg, double &o1, double &o2)
 {
   static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
@@ -366,18 +371,17 @@ DEV double interpol3(double y, double y1, double y2, double dy, double f1, doubl
  * one or two grid nodes in one pass: r = {a.x, a.y, b.x, b.y}.  Order 0 is the reference's
  * expression; for D >= 1 the step factors (j+1)/j, (j+3)/(j+1), ... are read from the ratio table
  * instead of dividing at every step (the two forms agree to rounding). */
-template <int D>
-DEV void cheb_r(int nk, double x, const double2 *__restrict__ a, const double2 *__restrict__ b, const double *LUTg,
-                double r[4])
+template <int D, typename P1>
+DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const double *LUTg, double r[4])
 {
+  /* a, b: double2 arrays (value, d/dy) read as pairs of doubles */
   const cdouble *LUT = lut_ptr(LUTg);
   const int nc = b ? 4 : 2;
   double ya[4] = { 0.0, 0.0, 0.0, 0.0 }, yb[4] = { 0.0, 0.0, 0.0, 0.0 };
   const double twox = 2.0 * x;
   const int jmin = D == 0 ? 1 : D;
   for (int j = nk - 1; j >= jmin; j--) {
-    const double2 ca = a[j], cc = b ? b[j] : make_double2(0.0, 0.0);
-    const double c[4] = { ca.x, ca.y, cc.x, cc.y };
+    const double c[4] = { a[2 * j], a[2 * j + 1], b ? b[2 * j] : 0.0, b ? b[2 * j + 1] : 0.0 };
     if (D == 0) {
 #pragma unroll
       for (int i = 0; i < 4; i++) {
@@ -398,8 +402,7 @@ DEV void cheb_r(int nk, double x, const double2 *__restrict__ a, const double2 *
     }
   }
   if (D == 0) {
-    const double2 ca = a[0], cc = b ? b[0] : make_double2(0.0, 0.0);
-    const double c[4] = { ca.x, ca.y, cc.x, cc.y };
+    const double c[4] = { a[0], a[1], b ? b[0] : 0.0, b ? b[1] : 0.0 };
     for (int i = 0; i < 4; i++) r[i] = -ya[i] + twox * yb[i] + c[i];
   } else {
     const double f = D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0;
@@ -414,8 +417,8 @@ DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int i
   const int mm = mm_of(nm);
   const int nk = T.nfx[ix] + mm;
   const int ord = mm + ndcb;
-  const double2 *a = T.S + node(T, nm, ix, iy);
-  const double2 *b = flag_hy ? nullptr : T.S + node(T, nm, ix, iy + 1);
+  const double *a = (const double *)(T.S + node(T, nm, ix, iy));
+  const double *b = flag_hy ? nullptr : (const double *)(T.S + node(T, nm, ix, iy + 1));
   double r[4];
   switch (ord) {
   case 0: cheb_r<0>(nk, cb, a, b, T.LUT, r); break;
@@ -496,13 +499,20 @@ DEV void interpol5(double F[4], const Ip &P, const double f[12])
 
 /* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of one table at NO orders.
  * nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
-template <int O0, int NO>
+template <int O0, int NO, bool UNI>
 DEV void table_xnode(const DT &T, int nm, int ixn, int nfw, const Inv &I, double twox, double H[NO], double Lv[NO])
 {
   const int nk = nfw + mm_of(nm);
   double r1[NO][2], r2[NO][2];
   const double2 *c1 = T.P + node(T, nm, ixn, I.INVy.idx);
-  clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(c1, c1 + T.ks, nk, twox, T.LUT, r1, r2);
+  if (UNI) {
+    /* one grid cell for every active lane: the coefficient arrays are read by scalar loads */
+    const cdouble *u1 = (const cdouble *)c1;
+    clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(u1, u1 + 2 * T.ks, nk, twox, T.LUT, r1, r2);
+  } else {
+    const double *g1 = (const double *)c1;
+    clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(g1, g1 + 2 * T.ks, nk, twox, T.LUT, r1, r2);
+  }
 #pragma unroll
   for (int i = 0; i < NO; i++) {
     H[i] = hy(I.INVy, r1[i][0], r2[i][0], r1[i][1], r2[i][1]);
@@ -511,6 +521,7 @@ DEV void table_xnode(const DT &T, int nm, int ixn, int nfw, const Inv &I, double
 }
 
 /* derivative-only variant (accessv6 with ndy, simd.c:616-650) + x step: extractff(nm, ndy=true) */
+template <bool UNI>
 DEV double ndy_packed(const DT &T, int nci_nm, int nci_dm, int nm, const Inv &I, const int nfw[2], double twox, int d)
 {
   double f[4];
@@ -521,7 +532,12 @@ DEV double ndy_packed(const DT &T, int nci_nm, int nci_dm, int nm, const Inv &I,
       const int nci = t == 0 ? nci_nm : nci_dm;
       const double *q1 = T.PNC + (((size_t)nci * T.nstpx + ixn) * T.nstpy + I.INVy.idx) * T.ks;
       double a, b;
-      clen_two<8>(q1, q1 + T.ks, nk, d, twox, T.LUT, a, b);
+      if (UNI) {
+        const cdouble *u1 = (const cdouble *)q1;
+        clen_two<8>(u1, u1 + T.ks, nk, d, twox, T.LUT, a, b);
+      } else {
+        clen_two<8>(q1, q1 + T.ks, nk, d, twox, T.LUT, a, b);
+      }
       f[2 * t + xn] = ly(I.INVy, a, b);
     }
   }
@@ -540,15 +556,15 @@ DEV void request(const Ip &INVx, const double Hn[2][2], const double Ln[2], cons
 }
 
 /* Simple groups: one request (nm, nm+6) at orders d, d+1 (d+1 < 4 always here). */
-template <int D>
+template <int D, bool UNI>
 DEV void simple_group(const DT &T, int nm, const Inv &I, const int nfw[2], double twox, double F[4])
 {
   double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
   for (int xn = 0; xn < 2; xn++) {
     double H[2], L[2];
-    table_xnode<D, 2>(T, nm, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
+    table_xnode<D, 2, UNI>(T, nm, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
     Hn[xn][0] = H[0]; Hn[xn][1] = H[1]; Ln[xn] = L[0];
-    table_xnode<D, 2>(T, nm + 6, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
+    table_xnode<D, 2, UNI>(T, nm + 6, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
     Hd[xn][0] = H[0]; Hd[xn][1] = H[1]; Ld[xn] = L[0];
   }
   request(I.INVx, Hn, Ln, Hd, Ld, F);
@@ -892,6 +908,108 @@ DEV int surr_bin(const DT &T, double x, double y)
   return bu * KQED_SURR_NB + bv;
 }
 
+/* Scalar-load variant of the packed path (one cell per pass of a waterfall loop): switched off,
+ * it returned wrong, run-to-run varying results for about 2 % of packed-path pairs and the cause is
+ * not found yet. */
+#define KQED_PACKED_SCALAR_LOADS 0
+
+/* The packed path (simd.c getff6/getff7 + accessv6/7 + extractff/extractff2, FMA build) for one
+ * pair: every weight function of chnr_dS/dT/dV into st.  UNI: I's cell (INVx.idx, INVy.idx) and
+ * nfw are the same in every active lane, so table addresses are scalar. */
+template <int CLS, bool UNI>
+DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, const int nfw[2])
+{
+  Tt &Q = st->T;
+  Vt &D = st->V;
+  double F[4];
+  const double twox = 2. * I.cb;
+  /* chnr_dS: QG0 */
+  simple_group<0, UNI>(T, QG0, I, nfw, twox, F);
+  st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
+  /* chnr_dT: QL2, QL4, QG1 */
+  simple_group<2, UNI>(T, QL2, I, nfw, twox, F);
+  Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
+  simple_group<1, UNI>(T, QL4, I, nfw, twox, F);
+  Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
+  simple_group<0, UNI>(T, QG1, I, nfw, twox, F);
+  Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
+  /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
+  {
+    double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
+    for (int xn = 0; xn < 2; xn++) {
+      table_xnode<1, 3, UNI>(T, QG2, I.INVx.idx + xn, nfw[xn], I, twox, H2[xn], L2[xn]);
+      table_xnode<1, 3, UNI>(T, dxQG2, I.INVx.idx + xn, nfw[xn], I, twox, H8[xn], L8[xn]);
+      table_xnode<1, 2, UNI>(T, d2xQG2, I.INVx.idx + xn, nfw[xn], I, twox, H12[xn], L12[xn]);
+    }
+    double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+    /* extractff2(QG2, d0cb): d = 1 */
+    for (int xn = 0; xn < 2; xn++) {
+      Hn[xn][0] = H2[xn][0]; Hn[xn][1] = H2[xn][1]; Ln[xn] = L2[xn][0];
+      Hd[xn][0] = H8[xn][0]; Hd[xn][1] = H8[xn][1]; Ld[xn] = L8[xn][0];
+    }
+    request(I.INVx, Hn, Ln, Hd, Ld, F);
+    D.g2 = F[3]; D.dg2dy = F[2];
+    /* extractff2(QG2, d1cb): d = 2 */
+    for (int xn = 0; xn < 2; xn++) {
+      Hn[xn][0] = H2[xn][1]; Hn[xn][1] = H2[xn][2]; Ln[xn] = L2[xn][1];
+      Hd[xn][0] = H8[xn][1]; Hd[xn][1] = H8[xn][2]; Ld[xn] = L8[xn][1];
+    }
+    request(I.INVx, Hn, Ln, Hd, Ld, F);
+    D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
+    /* extractff2(dxQG2, d0cb): d = 1 over (dxQG2, d2xQG2) */
+    for (int xn = 0; xn < 2; xn++) {
+      Hn[xn][0] = H8[xn][0]; Hn[xn][1] = H8[xn][1]; Ln[xn] = L8[xn][0];
+      Hd[xn][0] = H12[xn][0]; Hd[xn][1] = H12[xn][1]; Ld[xn] = L12[xn][0];
+    }
+    request(I.INVx, Hn, Ln, Hd, Ld, F);
+    D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
+  }
+  /* chnr_dV, QG3 family: QG3 (orders 1-2), dxQG3 (0-2), d2xQG3 (0-1) */
+  {
+    double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
+    for (int xn = 0; xn < 2; xn++) {
+      table_xnode<1, 2, UNI>(T, QG3, I.INVx.idx + xn, nfw[xn], I, twox, H3[xn], L3[xn]);
+      table_xnode<0, 3, UNI>(T, dxQG3, I.INVx.idx + xn, nfw[xn], I, twox, H9[xn], L9[xn]);
+      table_xnode<0, 2, UNI>(T, d2xQG3, I.INVx.idx + xn, nfw[xn], I, twox, H13[xn], L13[xn]);
+    }
+    double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+    /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
+    for (int xn = 0; xn < 2; xn++) {
+      Hn[xn][0] = H9[xn][0]; Hn[xn][1] = H9[xn][1]; Ln[xn] = L9[xn][0];
+      Hd[xn][0] = H13[xn][0]; Hd[xn][1] = H13[xn][1]; Ld[xn] = L13[xn][0];
+    }
+    request(I.INVx, Hn, Ln, Hd, Ld, F);
+    D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
+    /* extractff2(QG3, d1cb): d = 1 over (QG3, dxQG3) */
+    for (int xn = 0; xn < 2; xn++) {
+      Hn[xn][0] = H3[xn][0]; Hn[xn][1] = H3[xn][1]; Ln[xn] = L3[xn][0];
+      Hd[xn][0] = H9[xn][1]; Hd[xn][1] = H9[xn][2]; Ld[xn] = L9[xn][1];
+    }
+    request(I.INVx, Hn, Ln, Hd, Ld, F);
+    D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
+  }
+  /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
+  const double dg3dy = ndy_packed<UNI>(T, 2, 3, QG3, I, nfw, twox, 0);
+  D.dg3dy = dg3dy;
+  if (I.flag2) {
+    /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
+     * so they coincide unless the upper end sits on the top y node */
+    const int iy1 = I.INVy.idx;
+    const double fq1 = ndy_packed<UNI>(T, 0, 1, QG2, I, nfw, twox, 1);
+    double fq2 = fq1, gq2 = dg3dy;
+    if (CLS == 0 && iy1 + 1 == T.nstpy - 1) {
+      Inv I2 = I;
+      I2.y = T.YY[iy1 + 1];
+      fq2 = extractff_s(T, QG2, true, 0, I2);
+      gq2 = extractff_s(T, QG3, true, 0, I2);
+    }
+    const double ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
+    D.ddg2dydy = ddg2dydy;
+    const double ddg3dydy = (gq2 - dg3dy) / (T.YY[1] - T.YY[0]);
+    D.ddg1dydy = ddg3dydy - (I.y * ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
+  }
+}
+
 /* Interpolation of one pair into st.  CLS: 0 any pair, 1 packed-path pairs only, 2 Taylor-x pairs
  * only (the sort sends each class to its own kernel, which then carries only its own code path).
  * Returns 1 when the pair is outside the grid (the reference then leaves an all-zero kernel). */
@@ -925,96 +1043,27 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
   const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
   const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
   double F[4];
-  if (CLS == 1 || (!flag_hx && !flag_hy)) {
-    const double twox = 2. * I.cb;
+  if (CLS == 1 && KQED_PACKED_SCALAR_LOADS) {
+    /* one grid cell at a time per wavefront (cell order makes this one pass almost always): the
+     * cell's indices and Clenshaw lengths are uniform, so every table read is a scalar load */
+    const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
+    for (;;) {
+      const int cu = __builtin_amdgcn_readfirstlane(cell);
+      if (cell == cu) {
+        Inv Iu = I;
+        Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx); /* equal in every active lane */
+        Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
+        const int nfw[2] = { __builtin_amdgcn_readfirstlane(T.nfx[Iu.INVx.idx]),
+                             __builtin_amdgcn_readfirstlane(T.nfx[Iu.INVx.idx + 1]) };
+        packed_section<CLS, true>(T, Iu, st, nfw);
+        break;
+      }
+    }
+  } else if (CLS == 1 || (!flag_hx && !flag_hy)) {
     /* Clenshaw lengths are made wave-uniform (zero padding keeps each node's sums exact), so the
      * loops and ratio-table reads are scalar */
     const int nfw[2] = { wave_max_active(T.nfx[I.INVx.idx]), wave_max_active(T.nfx[I.INVx.idx + 1]) };
-    /* chnr_dS: QG0 */
-    simple_group<0>(T, QG0, I, nfw, twox, F);
-    st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
-    /* chnr_dT: QL2, QL4, QG1 */
-    simple_group<2>(T, QL2, I, nfw, twox, F);
-    Q.dell2dx = F[1]; Q.dell2dy = F[2]; Q.ell2 = F[3]; Q.dell2dcb = F[0];
-    simple_group<1>(T, QL4, I, nfw, twox, F);
-    Q.dell4dx = F[1]; Q.dell4dy = F[2]; Q.ell4 = F[3]; Q.dell4dcb = F[0];
-    simple_group<0>(T, QG1, I, nfw, twox, F);
-    Q.dv1dx = F[1]; Q.dv1dy = F[2]; Q.v1 = F[3]; Q.dv1dcb = F[0];
-    /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
-    {
-      double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
-      for (int xn = 0; xn < 2; xn++) {
-        table_xnode<1, 3>(T, QG2, I.INVx.idx + xn, nfw[xn], I, twox, H2[xn], L2[xn]);
-        table_xnode<1, 3>(T, dxQG2, I.INVx.idx + xn, nfw[xn], I, twox, H8[xn], L8[xn]);
-        table_xnode<1, 2>(T, d2xQG2, I.INVx.idx + xn, nfw[xn], I, twox, H12[xn], L12[xn]);
-      }
-      double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
-      /* extractff2(QG2, d0cb): d = 1 */
-      for (int xn = 0; xn < 2; xn++) {
-        Hn[xn][0] = H2[xn][0]; Hn[xn][1] = H2[xn][1]; Ln[xn] = L2[xn][0];
-        Hd[xn][0] = H8[xn][0]; Hd[xn][1] = H8[xn][1]; Ld[xn] = L8[xn][0];
-      }
-      request(I.INVx, Hn, Ln, Hd, Ld, F);
-      D.g2 = F[3]; D.dg2dy = F[2];
-      /* extractff2(QG2, d1cb): d = 2 */
-      for (int xn = 0; xn < 2; xn++) {
-        Hn[xn][0] = H2[xn][1]; Hn[xn][1] = H2[xn][2]; Ln[xn] = L2[xn][1];
-        Hd[xn][0] = H8[xn][1]; Hd[xn][1] = H8[xn][2]; Ld[xn] = L8[xn][1];
-      }
-      request(I.INVx, Hn, Ln, Hd, Ld, F);
-      D.dg2dcb = F[3]; D.ddg2dydcb = F[2]; D.ddg2dcbdcb = F[0];
-      /* extractff2(dxQG2, d0cb): d = 1 over (dxQG2, d2xQG2) */
-      for (int xn = 0; xn < 2; xn++) {
-        Hn[xn][0] = H8[xn][0]; Hn[xn][1] = H8[xn][1]; Ln[xn] = L8[xn][0];
-        Hd[xn][0] = H12[xn][0]; Hd[xn][1] = H12[xn][1]; Ld[xn] = L12[xn][0];
-      }
-      request(I.INVx, Hn, Ln, Hd, Ld, F);
-      D.ddg2dxdx = F[1]; D.dg2dx = F[3]; D.ddg2dxdy = F[2]; D.ddg2dxdcb = F[0];
-    }
-    /* chnr_dV, QG3 family: QG3 (orders 1-2), dxQG3 (0-2), d2xQG3 (0-1) */
-    {
-      double H3[2][2], L3[2][2], H9[2][3], L9[2][3], H13[2][2], L13[2][2];
-      for (int xn = 0; xn < 2; xn++) {
-        table_xnode<1, 2>(T, QG3, I.INVx.idx + xn, nfw[xn], I, twox, H3[xn], L3[xn]);
-        table_xnode<0, 3>(T, dxQG3, I.INVx.idx + xn, nfw[xn], I, twox, H9[xn], L9[xn]);
-        table_xnode<0, 2>(T, d2xQG3, I.INVx.idx + xn, nfw[xn], I, twox, H13[xn], L13[xn]);
-      }
-      double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
-      /* extractff2(dxQG3, d0cb): d = 0 over (dxQG3, d2xQG3) */
-      for (int xn = 0; xn < 2; xn++) {
-        Hn[xn][0] = H9[xn][0]; Hn[xn][1] = H9[xn][1]; Ln[xn] = L9[xn][0];
-        Hd[xn][0] = H13[xn][0]; Hd[xn][1] = H13[xn][1]; Ld[xn] = L13[xn][0];
-      }
-      request(I.INVx, Hn, Ln, Hd, Ld, F);
-      D.ddg3dxdx = F[1]; D.dg3dx = F[3]; D.ddg3dxdy = F[2]; D.ddg3dxdcb = F[0];
-      /* extractff2(QG3, d1cb): d = 1 over (QG3, dxQG3) */
-      for (int xn = 0; xn < 2; xn++) {
-        Hn[xn][0] = H3[xn][0]; Hn[xn][1] = H3[xn][1]; Ln[xn] = L3[xn][0];
-        Hd[xn][0] = H9[xn][1]; Hd[xn][1] = H9[xn][2]; Ld[xn] = L9[xn][1];
-      }
-      request(I.INVx, Hn, Ln, Hd, Ld, F);
-      D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
-    }
-    /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
-    const double dg3dy = ndy_packed(T, 2, 3, QG3, I, nfw, twox, 0);
-    D.dg3dy = dg3dy;
-    if (I.flag2) {
-      /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
-       * so they coincide unless the upper end sits on the top y node */
-      const int iy1 = I.INVy.idx;
-      const double fq1 = ndy_packed(T, 0, 1, QG2, I, nfw, twox, 1);
-      double fq2 = fq1, gq2 = dg3dy;
-      if (CLS == 0 && iy1 + 1 == T.nstpy - 1) {
-        Inv I2 = I;
-        I2.y = T.YY[iy1 + 1];
-        fq2 = extractff_s(T, QG2, true, 0, I2);
-        gq2 = extractff_s(T, QG3, true, 0, I2);
-      }
-      const double ddg2dydy = (fq2 - fq1) / (T.YY[1] - T.YY[0]);
-      D.ddg2dydy = ddg2dydy;
-      const double ddg3dydy = (gq2 - dg3dy) / (T.YY[1] - T.YY[0]);
-      D.ddg1dydy = ddg3dydy - (I.y * ddg2dydy + 2.0 * D.dg2dy) * I.cb / I.x;
-    }
+    packed_section<CLS, false>(T, I, st, nfw);
   } else if (CLS == 0) {
     /* grid-edge fallback: every extractff2 on the scalar path */
     extractff2_s(T, QG0, 0, I, F); st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
