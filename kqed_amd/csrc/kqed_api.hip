@@ -216,7 +216,8 @@ extern "C" void free_QED_temps(struct QED_kernel_temps *t)
 static int build_wave_maps(kqed_device *d)
 {
   kqed_dev_tables &T = d->dt;
-  std::vector<unsigned short> E, R(3 * 384);
+  std::vector<unsigned> E;
+  std::vector<unsigned short> R(3 * 384);
   for (int q = 0; q < 3; q++) {
     const kqed_map &m = d->maps[q];
     int order[384];
@@ -240,15 +241,20 @@ static int build_wave_maps(kqed_device *d)
       for (int k = 0; k < K; k++)
         for (int l = 0; l < 64; l++) {
           const int r = order[64 * sl + l], b = m.rowptr[r], len = m.rowptr[r + 1] - b;
-          unsigned short e = 200 | (1u << 8); /* zero slot, coefficient +1 */
+          /* entry: byte offset of the column in an FP64 tensor row (col * 8) in the low half, the
+           * upper 16 bits of the coefficient's FP64 bit pattern (+-0.5, +-1, +-2: low 48 bits are
+           * zero) in the high half; padding takes the zero slot (col 200) */
+          double v = 1.0;
+          int col = 200;
           if (k < len) {
-            const double v = m.val[b + k];
+            v = m.val[b + k];
+            col = m.col[b + k];
             const double a = fabs(v);
-            const int ex = a == 0.5 ? 0 : a == 1.0 ? 1 : a == 2.0 ? 2 : -1;
-            if (ex < 0 || m.col[b + k] >= 200) { set_err("assembly coefficient %g not encodable", v); return 1; }
-            e = (unsigned short)(m.col[b + k] | (ex << 8) | ((v < 0) << 10));
+            if ((a != 0.5 && a != 1.0 && a != 2.0) || col >= 200) { set_err("assembly coefficient %g not encodable", v); return 1; }
           }
-          E.push_back(e);
+          unsigned long long bits;
+          memcpy(&bits, &v, 8);
+          E.push_back((unsigned)(bits >> 48) << 16 | (unsigned)(col * 8));
         }
     }
     T.wmOff[q][6] = (int)E.size();
@@ -257,9 +263,9 @@ static int build_wave_maps(kqed_device *d)
   }
   T.wmTotal = (int)E.size();
   if (T.wmTotal > KQED_WM_MAX) { set_err("assembly maps too large for LDS"); return 1; }
-  unsigned short *e = (unsigned short *)dev_alloc(d, 2 * E.size());
+  unsigned *e = (unsigned *)dev_alloc(d, 4 * E.size());
   unsigned short *r = (unsigned short *)dev_alloc(d, 2 * R.size());
-  if (!e || !r || hipMemcpy(e, E.data(), 2 * E.size(), hipMemcpyHostToDevice) != hipSuccess ||
+  if (!e || !r || hipMemcpy(e, E.data(), 4 * E.size(), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(r, R.data(), 2 * R.size(), hipMemcpyHostToDevice) != hipSuccess) {
     set_err("wave map upload failed");
     return 1;

@@ -53,8 +53,8 @@
  * the standard map needs exactly these, the Tabd maps fewer and are padded with the zero slot) */
 #define KQED_WM_K { 13, 9, 6, 6, 4, 3 }
 #define KQED_WM_PER_LANE 41
-#define KQED_WM_MAX (3 * 64 * KQED_WM_PER_LANE) /* u16 entries of the three maps */
-#define KQED_ASM_WAVES 8    /* wavefronts per assembly block */
+#define KQED_WM_MAX (3 * 64 * KQED_WM_PER_LANE) /* u32 entries of the three maps */
+#define KQED_ASM_WAVES 16   /* wavefronts per assembly block: one block (LDS 136 KB) fills a CU */
 
 enum {
   KIND_NONE = 0,
@@ -83,7 +83,7 @@ struct kqed_dev_tables {
   /* the same maps dealt to the 64 lanes of a wavefront: rows sorted by length, row 64m + l of the
    * sorted order goes to lane l, slot m; entries [m][k][lane] as u16 = col | exp << 8 | sign << 10
    * with coefficient (sign ? -1 : 1) * 2^(exp-1); padded entries read the zero slot (col 200) */
-  const unsigned short *wmE;   /* all three maps, concatenated */
+  const unsigned *wmE;         /* all three maps, concatenated */
   const unsigned short *wmRow; /* [3][6][64] output row of (map, slot, lane) */
   int wmOff[3][7];             /* start of slot m of map q in wmE; wmOff[q][6] = end */
   int wmK[3][6];               /* entries per lane in slot m of map q */

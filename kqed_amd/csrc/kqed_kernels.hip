@@ -1537,22 +1537,24 @@ DEV void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* One row of a lane: KB entries (column | exponent << 8 | sign << 10, coefficient
- * (-1)^sign 2^(exponent-1)), unrolled so that every LDS read of the row is in flight at once. */
+/* One row of a lane: KB entries (byte offset col*8 of the column | upper 16 bits of the FP64
+ * coefficient << 16), unrolled so that every LDS read of the row is in flight at once.  The FP64
+ * coefficient is the entry with its low half cleared, as the high word of a double. */
 template <int KB, typename R>
-DEV R wm_dot(const unsigned short *__restrict__ e, int lane, const R *Tv)
+DEV R wm_dot(const unsigned *__restrict__ e, int lane, const R *Tv)
 {
   unsigned w[KB];
 #pragma unroll
   for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
   R t[KB];
 #pragma unroll
-  for (int k = 0; k < KB; k++) t[k] = Tv[w[k] & 0xFFu];
+  for (int k = 0; k < KB; k++)
+    t[k] = *(const R *)((const char *)Tv + (sizeof(R) == 8 ? (w[k] & 0xFFFFu) : (w[k] & 0xFFFFu) >> 1));
   R s = 0;
 #pragma unroll
   for (int k = 0; k < KB; k++) {
-    const R c = ldexp((w[k] & 0x400u) ? (R)-1 : (R)1, (int)((w[k] >> 8) & 3u) - 1);
-    s = fma(c, t[k], s);
+    const double c = __hiloint2double((int)(w[k] & 0xFFFF0000u), 0);
+    s = fma((R)c, t[k], s);
   }
   return s;
 }
@@ -1588,7 +1590,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
            int layout, int nslots, size_t ntot, const float *__restrict__ Tc, const double *__restrict__ Wc)
 {
   typedef MT R_t;
-  __shared__ unsigned short sE[KQED_WM_MAX];
+  __shared__ unsigned sE[KQED_WM_MAX];
   __shared__ unsigned short sRow[3 * 384];
   __shared__ double sW[KQED_ASM_WAVES][WREC + WH + 201 + WT];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1598,9 +1600,31 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
   double *R = sW[w], *H = R + WREC, *O = H + WH + 201;
   R_t *Tv = (R_t *)(H + WH);
   const size_t stride = (size_t)gridDim.x * KQED_ASM_WAVES;
-  for (size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w; p < n; p += stride) {
-    for (int i = lane; i < WREC; i += 64) R[i] = rec[p * WREC + i];
-    const unsigned kall = __builtin_amdgcn_readfirstlane(kinds[p]);
+  /* the next pair's record and kinds are loaded into registers while the current pair is assembled,
+   * so the wavefront never waits on a record load from HBM */
+  static_assert(WREC <= 192, "record prefetch covers three doubles per lane");
+  size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w;
+  double nx0 = 0.0, nx1 = 0.0, nx2 = 0.0;
+  unsigned nk = 0;
+  if (p < n) {
+    const double *rp = rec + p * WREC;
+    nx0 = rp[lane];
+    nx1 = rp[lane + 64];
+    if (lane + 128 < WREC) nx2 = rp[lane + 128];
+    nk = kinds[p];
+  }
+  for (; p < n; p += stride) {
+    R[lane] = nx0;
+    R[lane + 64] = nx1;
+    if (lane + 128 < WREC) R[lane + 128] = nx2;
+    const unsigned kall = __builtin_amdgcn_readfirstlane(nk);
+    if (p + stride < n) {
+      const double *rp = rec + (p + stride) * WREC;
+      nx0 = rp[lane];
+      nx1 = rp[lane + 64];
+      if (lane + 128 < WREC) nx2 = rp[lane + 128];
+      nk = kinds[p + stride];
+    }
     if (lane == 0) Tv[200] = 0.0;
     for (int i = lane; i < 384; i += 64) O[i] = 0.0;
     wave_sync();
@@ -1639,7 +1663,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       double v[6];
       {
         /* slot m of map mq starts at 64 * (41 mq + 13 + 9 + ...) entries (checked on the host) */
-        const unsigned short *e = sE + mq * 64 * KQED_WM_PER_LANE;
+        const unsigned *e = sE + mq * 64 * KQED_WM_PER_LANE;
         v[0] = wm_dot<13>(e, lane, Tv);
         v[1] = wm_dot<9>(e + 64 * 13, lane, Tv);
         v[2] = wm_dot<6>(e + 64 * 22, lane, Tv);
