@@ -908,9 +908,10 @@ DEV int surr_bin(const DT &T, double x, double y)
   return bu * KQED_SURR_NB + bv;
 }
 
-/* Scalar-load variant of the packed path (one cell per pass of a waterfall loop): switched off,
- * it returned wrong, run-to-run varying results for about 2 % of packed-path pairs and the cause is
- * not found yet. */
+/* Scalar-load variant of the packed path (one cell per pass of a waterfall loop, every table read
+ * a scalar load).  Correct, but measured 3x slower than the vector-load form on the benchmark
+ * workload (passes over several cells per wavefront, scalar-load latency per Clenshaw block), so
+ * it is off. */
 #define KQED_PACKED_SCALAR_LOADS 0
 
 /* The packed path (simd.c getff6/getff7 + accessv6/7 + extractff/extractff2, FMA build) for one
@@ -1046,18 +1047,27 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
   if (CLS == 1 && KQED_PACKED_SCALAR_LOADS) {
     /* one grid cell at a time per wavefront (cell order makes this one pass almost always): the
      * cell's indices and Clenshaw lengths are uniform, so every table read is a scalar load */
+    /* Waterfall over the wavefront's cells with an explicit mask of lanes still to serve: each pass
+     * takes the first remaining lane's cell (readlane: uniform) and serves the lanes of that cell.
+     * The mask is loop-carried on purpose: a loop on readfirstlane of a loop-invariant value alone
+     * lets the compiler hoist the readfirstlane and drop the loop. */
     const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
-    for (;;) {
-      const int cu = __builtin_amdgcn_readfirstlane(cell);
+    unsigned long long todo = __ballot(1);
+    while (todo) {
+      const int first = (int)__builtin_ctzll(todo);
+      const int cu = __builtin_amdgcn_readlane(cell, first);
+      const unsigned long long same = __ballot(cell == cu);
       if (cell == cu) {
+        /* the first lane's indices (read straight into scalar registers: indices computed with
+         * vector instructions would turn the table reads back into vector loads) */
         Inv Iu = I;
-        Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx); /* equal in every active lane */
-        Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
-        const int nfw[2] = { __builtin_amdgcn_readfirstlane(T.nfx[Iu.INVx.idx]),
-                             __builtin_amdgcn_readfirstlane(T.nfx[Iu.INVx.idx + 1]) };
+        Iu.INVx.idx = __builtin_amdgcn_readlane(I.INVx.idx, first);
+        Iu.INVy.idx = __builtin_amdgcn_readlane(I.INVy.idx, first);
+        const int nfw[2] = { __builtin_amdgcn_readlane(T.nfx[Iu.INVx.idx], first),
+                             __builtin_amdgcn_readlane(T.nfx[Iu.INVx.idx + 1], first) };
         packed_section<CLS, true>(T, Iu, st, nfw);
-        break;
       }
+      todo &= ~same;
     }
   } else if (CLS == 1 || (!flag_hx && !flag_hy)) {
     /* Clenshaw lengths are made wave-uniform (zero padding keeps each node's sums exact), so the
@@ -1331,10 +1341,11 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
   if (interp_eval<CLS>(T, a, b, (Stage1 *)pl) == 0) kinds |= (unsigned)KIND_STD;
 }
 
-/* occupancy: the packed-path kernel runs at two waves per SIMD (it then spills a few bytes), the
- * scalar-path kernels keep all their registers at one */
+/* occupancy: the packed-path and Taylor-region kernels run at two waves per SIMD (they then spill
+ * a little: measured faster than one wave with every register), the generic kernel at one */
 template <int CLS> struct InterpWaves { static constexpr int v = 1; };
 template <> struct InterpWaves<1> { static constexpr int v = 2; };
+template <> struct InterpWaves<2> { static constexpr int v = 2; };
 
 /* Sorted position i (input position when perm is null) in the window [c0, c1) of this pass and in
  * the class segment [lo, hi), lo = bounds[ilo] (0 when ilo < 0), hi = bounds[ihi] (n when ihi < 0).
