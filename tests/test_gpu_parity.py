@@ -92,3 +92,15 @@ def test_multi_pass_batch(gpu_full, oracle_full):
     assert normwise(got[idx], small).max() <= 1e-15
     ref = oracle_full.batch(0, x[idx[:300]], y[idx[:300]])
     assert normwise(got[idx[:300]], ref).max() <= TOL
+
+
+def test_deterministic_and_order_independent(gpu_full):
+    """Each pair's result is independent of the batch around it: the same batch twice, and the batch
+    in a shuffled order, give identical per-pair results."""
+    x, y = log_uniform(13, 30000)
+    a = gpu_full.batch_host(0, x, y)
+    b = gpu_full.batch_host(0, x, y)
+    assert np.array_equal(a, b)
+    perm = np.random.default_rng(5).permutation(len(x))
+    c = gpu_full.batch_host(0, x[perm], y[perm])
+    assert np.array_equal(c, a[perm])
