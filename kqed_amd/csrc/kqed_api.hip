@@ -341,16 +341,6 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     if (!d->maps || kqed_build_maps(&d->maps[0], &d->maps[1], &d->maps[2])) { set_err("assembly map build failed"); goto fail; }
     if (d->maps[0].kmax > 14 || d->maps[1].kmax > 10 || d->maps[2].kmax > 10) { set_err("assembly map rows too long"); goto fail; }
     if (build_wave_maps(d)) goto fail;
-    for (int m = 0; m < 3; m++) {
-      int *rp = (int *)dev_alloc(d, 4 * 385);
-      short *cl = (short *)dev_alloc(d, 2 * (size_t)d->maps[m].nnz);
-      double *vl = (double *)dev_alloc(d, 8 * (size_t)d->maps[m].nnz);
-      if (!rp || !cl || !vl) { set_err("device allocation failed"); goto fail; }
-      HIPCHK(hipMemcpy(rp, d->maps[m].rowptr, 4 * 385, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(cl, d->maps[m].col, 2 * (size_t)d->maps[m].nnz, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(vl, d->maps[m].val, 8 * (size_t)d->maps[m].nnz, hipMemcpyHostToDevice));
-      T.rowptr[m] = rp; T.col[m] = cl; T.val[m] = vl;
-    }
     {
       /* standard map by column: entries of column c in row order */
       const kqed_map &m0 = d->maps[0];

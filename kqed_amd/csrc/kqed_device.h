@@ -76,13 +76,10 @@ struct kqed_dev_tables {
   const double2 *P;
   const double *PNC;
   const double2 *S;
-  /* assembly maps: 0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign (CSR) */
-  const int *rowptr[3];
-  const short *col[3];
-  const double *val[3];
-  /* the same maps dealt to the 64 lanes of a wavefront: rows sorted by length, row 64m + l of the
-   * sorted order goes to lane l, slot m; entries [m][k][lane] as u16 = col | exp << 8 | sign << 10
-   * with coefficient (sign ? -1 : 1) * 2^(exp-1); padded entries read the zero slot (col 200) */
+  /* assembly maps (0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign) dealt to the 64 lanes of
+   * a wavefront: rows sorted by length, row 64m + l of the sorted order goes to lane l, slot m;
+   * entries [m][k][lane] as u32 = col * 8 | (upper 16 bits of the FP64 coefficient) << 16 with
+   * coefficient +-0.5, +-1 or +-2; padded entries read the zero slot (col 200) */
   const unsigned *wmE;         /* all three maps, concatenated */
   const unsigned short *wmRow; /* [3][6][64] output row of (map, slot, lane) */
   int wmOff[3][7];             /* start of slot m of map q in wmE; wmOff[q][6] = end */

@@ -1570,16 +1570,6 @@ DEV R wm_dot(const unsigned *__restrict__ e, int lane, const R *Tv)
   return s;
 }
 
-/* CSR row of map q (for the M-L2 alpha sums, which need rows other than the lane's own) */
-template <typename R>
-DEV double row_dot(const int *__restrict__ rp, const short *__restrict__ col, const double *__restrict__ val, int r,
-                   const R *Tv)
-{
-  double s = 0.0;
-  const int e = rp[r + 1];
-  for (int k = rp[r]; k < e; k++) s = fma(val[k], (double)Tv[col[k]], s);
-  return s;
-}
 
 /* FP64 sum over the 64 lanes of a wavefront (every lane active), result in every lane */
 DEV double wave_sum(double v)
@@ -1683,23 +1673,37 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         v[5] = wm_dot<3>(e + 64 * 38, lane, Tv);
       }
       if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
-        /* reference kernels.c:292-323: the alpha sums take other rows of L0(0,y) or L0(x,0),
-         * recomputed from the tensors with the row-ordered map */
+        /* reference kernels.c:292-323: row r = (rs, mu, nu, la) of the slot takes the alpha sum
+         * S = sum_a v_a K(rs, a, nu, la) (x = 0 term; K(rs, mu, a, la) for the y = 0 term) over other
+         * rows of the same L0(0,y) / L0(x,0) kernel.  The lanes hold those rows, so the 96 sums are
+         * gathered in LDS (in the tensor region, free once the rows are computed) in four rounds, one
+         * per a: every sum gets one term per round, in the order a = 0..3. */
+        const bool xm = kd == KIND_XEQ0_M;
         const double Mv = R[KQED_REC_M];
-        const double g = kd == KIND_XEQ0_M ? R[KQED_REC_GX] : R[KQED_REC_GY];
-        const double *vv = kd == KIND_XEQ0_M ? R + KQED_REC_X : R + KQED_REC_Y;
+        const double g = xm ? R[KQED_REC_GX] : R[KQED_REC_GY];
+        const double *vv = xm ? R + KQED_REC_X : R + KQED_REC_Y;
+        double *S = (double *)Tv;
+        wave_sync(); /* every lane is done reading the tensors */
+        for (int i = lane; i < 96; i += 64) S[i] = 0.0;
+        wave_sync();
+        for (int a = 0; a < 4; a++) {
+#pragma unroll
+          for (int m = 0; m < 6; m++) {
+            const int r = sRow[mq * 384 + 64 * m + lane];
+            const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
+            if ((xm ? mu : nu) == a) {
+              double *sp = S + 16 * rs + 4 * (xm ? nu : mu) + la;
+              *sp = *sp + vv[a] * v[m];
+            }
+          }
+          wave_sync();
+        }
 #pragma unroll
         for (int m = 0; m < 6; m++) {
           const int r = sRow[mq * 384 + 64 * m + lane];
           const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
-          double k4[4];
-          if (kd == KIND_XEQ0_M) {
-            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * a + 4 * nu + la, Tv);
-            O[r] -= g * (v[m] - Mv * vv[mu] * (vv[0] * k4[0] + vv[1] * k4[1] + vv[2] * k4[2] + vv[3] * k4[3]));
-          } else {
-            for (int a = 0; a < 4; a++) k4[a] = row_dot(T.rowptr[1], T.col[1], T.val[1], 64 * rs + 16 * mu + 4 * a + la, Tv);
-            O[r] -= g * (v[m] - Mv * vv[nu] * (vv[0] * k4[0] + vv[1] * k4[1] + vv[2] * k4[2] + vv[3] * k4[3]));
-          }
+          const double sum = S[16 * rs + 4 * (xm ? nu : mu) + la];
+          O[r] -= g * (v[m] - Mv * vv[xm ? mu : nu] * sum);
         }
       } else {
         const double cf = R[KQED_REC_COEF + slot];
