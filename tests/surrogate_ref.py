@@ -13,6 +13,11 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def exact(a):
+    """No operand rounding (for checking the reference itself against finite differences)."""
+    return np.asarray(a, np.float64)
+
+
 def silu(z):
     return z / (1.0 + np.exp(-z))
 
@@ -22,18 +27,19 @@ def dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def forward_backward(layers, X3, tgt, wgt):
-    """layers: [(W [K][N], b [N])] FP32; X3 [B][3]; tgt, wgt [B][34].  Returns loss, [(dW, db)]."""
+def forward_backward(layers, X3, tgt, wgt, rnd=bf16):
+    """layers: [(W [K][N], b [N])]; X3 [B][3]; tgt, wgt [B][34].  Returns loss, [(dW, db)].
+    rnd: operand rounding of the GEMMs (bf16 as on the GPU, or exact)."""
     B = X3.shape[0]
     X = np.zeros((B, 16))
     X[:, :3] = X3
     A = [X]
     Z = []
     for l, (W, b) in enumerate(layers):
-        z = bf16(A[-1]) @ bf16(W) + b.astype(np.float64)
+        z = rnd(A[-1]) @ rnd(W) + np.asarray(b, np.float64)
         Z.append(z)
         if l < 4:
-            A.append(silu(np.float32(z).astype(np.float64)))
+            A.append(silu(z if rnd is exact else np.float32(z).astype(np.float64)))
     nout = tgt.shape[1]
     d = Z[4][:, :nout] - tgt
     loss = float((wgt * d * d).sum() / (B * nout))
@@ -42,11 +48,11 @@ def forward_backward(layers, X3, tgt, wgt):
     grads = [None] * 5
     for l in range(4, -1, -1):
         W, _ = layers[l]
-        dW = bf16(A[l]).T @ bf16(dZ)
+        dW = rnd(A[l]).T @ rnd(dZ)
         db = dZ.sum(axis=0)
         grads[l] = (dW, db)
         if l == 0:
             break
-        dA = bf16(dZ) @ bf16(W).T
+        dA = rnd(dZ) @ rnd(W).T
         dZ = dA * dsilu(Z[l - 1])
     return loss, grads
