@@ -56,7 +56,8 @@ def pass_traffic(pairs_per_pass):
     calibration factor + WRITE_SIZE, summed over the pass's kernels), scaled to this pass length."""
     f = os.path.join(ROOT, "profiles", "traffic_l0_pass.txt")
     try:
-        d = json.load(open(f))
+        last = [ln for ln in open(f).read().splitlines() if ln.strip()][-1]  # the summary's final JSON line
+        d = json.loads(last)
         return d["bytes_per_pass"] * pairs_per_pass / d["pairs_per_pass"]
     except (OSError, KeyError, ValueError):
         return None
