@@ -66,6 +66,63 @@ void QED_kernel_L3(const double xv[4], const double yv[4], const struct QED_kern
 void QED_Mkernel_L2(const double M, const double xv[4], const double yv[4],
                     const struct QED_kernel_temps t, double kerv[6][4][4][4]);
 
+/* Kernel containers of the combination routines (reference KQED.h:105-131), same layout:
+ * 384 doubles per kernel, kernels in declaration order. */
+struct Kernel {
+  double xy[6][4][4][4];
+  double yx[6][4][4][4];
+};
+struct Kernels {
+  double L0[6][4][4][4];
+  double L1[6][4][4][4];
+  double L2[6][4][4][4];
+  double L3[6][4][4][4];
+};
+struct QED_Kernels {
+  struct Kernel L0;
+  struct Kernel L1;
+  struct Kernel L2;
+  struct Kernel L3;
+};
+struct QED_Con_Kernels {
+  struct Kernels A; /* L^i(x,y) */
+  struct Kernels B; /* L^i(x,x-y) */
+  struct Kernels C; /* L^i(x,y) + L^i_{mu<>nu}(y,x) - L^i(x,x-y) */
+};
+
+/* Kernel combinations, one (x,y) pair each, synchronous, host pointers; computed on the GPU
+ * through kqed_combine.  Replace reference all_kernels.h:4-33, con_kernel.h:4-27,
+ * sub_kernel.h:4-9, SYMXY.h:4-16, SYMXY0.h:4-16 (all_kernels.c, con_kernel.c, sub_kernel.c,
+ * SYMXY.c, SYMXY0.c).  On a device error they print the message and leave zeros. */
+void compute_all_kernels(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                         struct QED_Kernels *K);
+void compute_all_Mkernels(const double M[4], const double xv[4], const double yv[4],
+                          const struct QED_kernel_temps t, struct QED_Kernels *K);
+void compute_all_Mkernels_v2(const double M[4], const double xv[4], const double yv[4],
+                             const struct QED_kernel_temps t, struct QED_Kernels *K);
+void compute_con_kernels(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                         struct QED_Kernels *K);
+void compute_con_kernels_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                            struct Kernels *K /* [3] */);
+void compute_con_kernelsM_L2(const double M[4], const double xv[4], const double yv[4],
+                             const struct QED_kernel_temps t, struct Kernels *K /* [3] */);
+void compute_sub_kernelsM_L2(const double M[4], const double xv[4], const double yv[4],
+                             const struct QED_kernel_temps t, struct Kernels *K /* [3] */);
+void compute_all_kernels_SYMXY(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                               struct QED_Kernels *K);
+void compute_all_kernels_SYMXY_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                  struct Kernels *K);
+void compute_all_kernels_SYMXY0(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                struct QED_Kernels *K);
+void compute_all_kernels_SYMXY0_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                   struct Kernels *K);
+/* Host-side helpers on the containers: K = (K + W) / 2, and mu <-> nu of every .yx kernel.
+ * Replace all_kernels.c:18-82 and :372-406, con_kernel.c:13-59. */
+void average_all_kernels(struct Kernels *K, const struct Kernels W);
+void average_all_QED_kernels(struct QED_Kernels *K, const struct QED_Kernels W);
+void average_con_kernels(struct Kernels *K /* [3] */, const struct Kernels *W /* [3] */);
+void swap_munu_Lyx(struct QED_Kernels *K);
+
 /* ---------------------------------------------------------------- batched entry points (new) */
 enum kqed_kernel { KQED_L0 = 0, KQED_L1 = 1, KQED_L2 = 2, KQED_L3 = 3, KQED_ML2 = 4 };
 /* Output layouts.  The FP32 layouts also run the sparse assembly maps in FP32 (interpolation, the
@@ -189,6 +246,34 @@ int kqed_surrogate_load(kqed_surrogate *s, const char *path);
 /* Write a synthetic form-factor table in the reference's file format (see kqed_tables.h). */
 int kqed_write_synthetic_ff(const char *path, int nstpx, int nstpy, int nf0, int nf1, double xh,
                             double yh, unsigned long long seed);
+
+/* Batched kernel combinations: n pairs, device pointers x[n][4], y[n][4]; out[n][len] doubles with
+ * len = 384 * kqed_comb_kernels(comb), per pair the reference's struct layout (QED_Kernels,
+ * Kernels or Kernels[3]).  M: host pointer to 4 doubles for the M variants (ignored otherwise,
+ * NULL allowed).  Every L0 the combination needs runs through the batched pipeline; the
+ * workspace holds one pass of tune.chunk L0 tensors (3 KiB each).  Enqueued on `stream`. */
+enum kqed_comb {
+  KQED_COMB_ALL = 0,      /* compute_all_kernels           -> struct QED_Kernels */
+  KQED_COMB_ALL_M = 1,    /* compute_all_Mkernels          -> struct QED_Kernels */
+  KQED_COMB_ALL_M_V2 = 2, /* compute_all_Mkernels_v2       -> struct QED_Kernels */
+  KQED_COMB_CON = 3,      /* compute_con_kernels           -> struct QED_Kernels */
+  KQED_COMB_CON_V2 = 4,   /* compute_con_kernels_v2        -> struct Kernels[3] */
+  KQED_COMB_CON_M_L2 = 5, /* compute_con_kernelsM_L2       -> struct Kernels[3] */
+  KQED_COMB_SUB_M_L2 = 6, /* compute_sub_kernelsM_L2       -> struct Kernels[3] */
+  KQED_COMB_SYMXY = 7,    /* compute_all_kernels_SYMXY     -> struct QED_Kernels */
+  KQED_COMB_SYMXY_V2 = 8, /* compute_all_kernels_SYMXY_v2  -> struct Kernels */
+  KQED_COMB_SYMXY0 = 9,   /* compute_all_kernels_SYMXY0    -> struct QED_Kernels */
+  KQED_COMB_SYMXY0_V2 = 10, /* compute_all_kernels_SYMXY0_v2 -> struct Kernels */
+  KQED_COMB_N = 11
+};
+/* kernels of 384 doubles per pair in the output of `comb`: 8 (QED_Kernels), 12 (Kernels[3]) or
+ * 4 (Kernels); 0 for an unknown comb */
+int kqed_comb_kernels(int comb);
+int kqed_combine(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
+                 const double *x_dev, const double *y_dev, double *out_dev, void *stream);
+/* Same on host buffers (copies in and out, synchronous). */
+int kqed_combine_host(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
+                      const double *x, const double *y, double *out);
 
 #ifdef __cplusplus
 }

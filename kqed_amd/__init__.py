@@ -25,6 +25,9 @@ GOLDEN_PRECOMP = os.path.join(ROOT, "tests", "golden", "PRECOMP")
 KQED_L0, KQED_L1, KQED_L2, KQED_L3, KQED_ML2 = range(5)
 KQED_AOS, KQED_SOA, KQED_AOS_F32, KQED_SOA_F32 = 0, 1, 2, 3
 KQED_SORT_CELL, KQED_SORT_XNODE, KQED_SORT_NONE = 0, 1, 2
+# kernel combinations (reference all_kernels.c, con_kernel.c, sub_kernel.c, SYMXY.c, SYMXY0.c)
+(KQED_COMB_ALL, KQED_COMB_ALL_M, KQED_COMB_ALL_M_V2, KQED_COMB_CON, KQED_COMB_CON_V2, KQED_COMB_CON_M_L2,
+ KQED_COMB_SUB_M_L2, KQED_COMB_SYMXY, KQED_COMB_SYMXY_V2, KQED_COMB_SYMXY0, KQED_COMB_SYMXY0_V2) = range(11)
 
 _lib = None
 
@@ -102,6 +105,9 @@ def lib():
                                        vp]
     L.kqed_surrogate_save.argtypes = [vp, ctypes.c_char_p]
     L.kqed_surrogate_load.argtypes = [vp, ctypes.c_char_p]
+    L.kqed_comb_kernels.argtypes = [ctypes.c_int]
+    L.kqed_combine.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, vp, vp]
+    L.kqed_combine_host.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, arr, arr, vp]
     L.kqed_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_autotune.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
@@ -239,6 +245,33 @@ class QEDKernelTemps:
         if rc != 0:
             raise RuntimeError("kqed_batch_host failed: " + lib().kqed_last_error().decode())
         return out
+
+    def combine(self, comb, x, y, M=None):
+        """A kernel combination (KQED_COMB_*) for n pairs of host arrays x[n,4], y[n,4] ->
+        (n, nk, 384) float64, nk kernels per pair in the reference's struct order; M: 4 masses for
+        the M variants."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 4)
+        nk = lib().kqed_comb_kernels(int(comb))
+        if nk == 0:
+            raise ValueError(f"unknown combination {comb}")
+        n = x.shape[0]
+        out = np.zeros((n, nk, 384))
+        M4 = None if M is None else np.ascontiguousarray(M, np.float64).reshape(4)
+        rc = lib().kqed_combine_host(ctypes.byref(self._t), int(comb), None if M4 is None else M4.ctypes.data, n,
+                                     x.reshape(-1), y.reshape(-1), out.ctypes.data)
+        if rc != 0:
+            raise RuntimeError("kqed_combine_host failed: " + lib().kqed_last_error().decode())
+        return out
+
+    def combine_device(self, comb, n, x_ptr, y_ptr, out_ptr, M=None, stream=None):
+        """A kernel combination on device pointers (ints); out holds n * 384 * kqed_comb_kernels(comb) doubles."""
+        M4 = None if M is None else np.ascontiguousarray(M, np.float64).reshape(4)
+        rc = lib().kqed_combine(ctypes.byref(self._t), int(comb), None if M4 is None else M4.ctypes.data, int(n),
+                                ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr),
+                                ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_combine failed: " + lib().kqed_last_error().decode())
 
     def get_tune(self):
         tu = Tune()

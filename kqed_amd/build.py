@@ -37,7 +37,7 @@ def build(verbose=True):
         o = os.path.join(OUT, c + ".o")
         _run(["gcc", *C_FLAGS, "-c", os.path.join(SRC, c), "-o", o])
         objs.append(o)
-    for h in ("kqed_kernels.hip", "kqed_surrogate.hip", "kqed_api.hip"):
+    for h in ("kqed_kernels.hip", "kqed_surrogate.hip", "kqed_combine.hip", "kqed_api.hip"):
         o = os.path.join(OUT, h + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, h), "-o", o])
         objs.append(o)

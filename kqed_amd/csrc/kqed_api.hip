@@ -17,6 +17,7 @@
 
 #include "../../include/kqed.h"
 #include "kqed_assembly.h"
+#include "kqed_combine.h"
 #include "kqed_device.h"
 #include "kqed_tables.h"
 
@@ -113,6 +114,9 @@ struct Workspace {
    * reduction buffers (RED_LEN doubles + 4 * ycap * 384 + ...) */
   size_t xcap = 0, ycap = 0;
   double *dots = nullptr, *W = nullptr, *red = nullptr;
+  /* kernel combinations: derived argument pairs [2][ccap][4] and their L0 tensors [ccap][384] */
+  size_t ccap = 0;
+  double *cargs = nullptr, *cL = nullptr;
 };
 
 /* optional per-stage timing: events recorded on the launch stream around every kernel */
@@ -190,6 +194,7 @@ static void free_ws(Workspace &w)
 {
   (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
+  (void)hipFree(w.cargs); (void)hipFree(w.cL);
   w = Workspace();
 }
 
@@ -900,6 +905,165 @@ extern "C" int kqed_batch_host(const struct QED_kernel_temps *t, int which, doub
     return -3;
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ kernel combinations */
+
+/* derived argument pairs and L0 tensors of one combination pass (ne evaluations) */
+static int ensure_comb_ws(Workspace &w, size_t ne)
+{
+  if (w.ccap >= ne) return 0;
+  (void)hipFree(w.cargs); (void)hipFree(w.cL);
+  w.cargs = w.cL = nullptr; w.ccap = 0;
+  if (hipMalloc(&w.cargs, 64 * ne) != hipSuccess || hipMalloc(&w.cL, 3072 * ne) != hipSuccess) {
+    set_err("combination workspace allocation of %zu evaluations failed", ne);
+    return -2;
+  }
+  w.ccap = ne;
+  return 0;
+}
+
+/* Passes of cp pairs (E * cp <= tune.chunk L0 evaluations): derived arguments, the L0 pipeline
+ * over all of them (one sort per pass: the special x=0 / y=0 evaluations group together), then
+ * the per-pair combination into the caller's output. */
+extern "C" int kqed_combine(const struct QED_kernel_temps *t, int comb, const double *M, size_t n, const double *x_dev,
+                            const double *y_dev, double *out_dev, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_combine: tables not initialised"); return -1; }
+  const int E = kqed_comb_evals(comb);
+  if (!E) { set_err("kqed_combine: unknown combination %d", comb); return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull) { set_err("kqed_combine: at most 2^32-1 pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  Workspace &w = d->ws[stream];
+  hipStream_t s = (hipStream_t)stream;
+  size_t cp = d->tune.chunk / (size_t)E;
+  if (cp < 1) cp = 1;
+  if (cp > n) cp = n;
+  if (ensure_comb_ws(w, cp * E)) return -2;
+  const size_t len = 384 * (size_t)kqed_comb_kernels(comb);
+  for (size_t c0 = 0; c0 < n; c0 += cp) {
+    const size_t c = n - c0 < cp ? n - c0 : cp;
+    double *X = w.cargs, *Y = w.cargs + 4 * (size_t)E * c;
+    kqed_comb_launch_args(comb, c, x_dev + 4 * c0, y_dev + 4 * c0, X, Y, s);
+    const int rc = run_pairs(d, w, s, KQED_L0, 0.0, (size_t)E * c, X, 4, Y, 4, w.cL, KQED_AOS, nullptr, nullptr);
+    if (rc) return rc;
+    kqed_comb_launch(comb, c, w.cL, x_dev + 4 * c0, y_dev + 4 * c0, M, out_dev + len * c0, s);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
+  return 0;
+}
+
+struct CombStage {
+  size_t cap = 0, ocap = 0;
+  double *x = nullptr, *o = nullptr;
+  hipStream_t s = nullptr;
+  ~CombStage()
+  {
+    (void)hipFree(x); (void)hipFree(o);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+static thread_local CombStage g_cstage;
+
+extern "C" int kqed_combine_host(const struct QED_kernel_temps *t, int comb, const double *M, size_t n, const double *x,
+                                 const double *y, double *out)
+{
+  if (!t || !t->dev) { set_err("kqed_combine_host: tables not initialised"); return -1; }
+  const size_t len = 384 * (size_t)kqed_comb_kernels(comb);
+  if (!len) { set_err("kqed_combine_host: unknown combination %d", comb); return -1; }
+  if (n == 0) return 0;
+  CombStage &st = g_cstage;
+  if (!st.s && hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) { set_err("stream creation failed"); return -2; }
+  if (st.cap < n) {
+    (void)hipFree(st.x); st.x = nullptr; st.cap = 0;
+    if (hipMalloc(&st.x, 64 * n) != hipSuccess) { set_err("staging allocation failed"); return -2; }
+    st.cap = n;
+  }
+  if (st.ocap < len * n) {
+    (void)hipFree(st.o); st.o = nullptr; st.ocap = 0;
+    if (hipMalloc(&st.o, 8 * len * n) != hipSuccess) { set_err("staging allocation failed"); return -2; }
+    st.ocap = len * n;
+  }
+  double *xd = st.x, *yd = st.x + 4 * n;
+  if (hipMemcpyAsync(xd, x, 32 * n, hipMemcpyHostToDevice, st.s) != hipSuccess ||
+      hipMemcpyAsync(yd, y, 32 * n, hipMemcpyHostToDevice, st.s) != hipSuccess) {
+    set_err("host to device copy failed");
+    return -3;
+  }
+  const int rc = kqed_combine(t, comb, M, n, xd, yd, st.o, st.s);
+  if (rc) return rc;
+  if (hipMemcpyAsync(out, st.o, 8 * len * n, hipMemcpyDeviceToHost, st.s) != hipSuccess ||
+      hipStreamSynchronize(st.s) != hipSuccess) {
+    set_err("device to host copy failed");
+    return -3;
+  }
+  return 0;
+}
+
+/* the reference's per-pair combination routines, one pair through kqed_combine_host */
+static void comb_one(int comb, const double *M, const double *xv, const double *yv, const struct QED_kernel_temps *t, void *K)
+{
+  if (kqed_combine_host(t, comb, M, 1, xv, yv, (double *)K) != 0) {
+    fprintf(stderr, "[KQED] %s\n", g_err);
+    memset(K, 0, 8 * 384 * (size_t)kqed_comb_kernels(comb));
+  }
+}
+
+extern "C" void compute_all_kernels(const double xv[4], const double yv[4], const struct QED_kernel_temps t, struct QED_Kernels *K)
+{ comb_one(KQED_COMB_ALL, nullptr, xv, yv, &t, K); }
+extern "C" void compute_all_Mkernels(const double M[4], const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                     struct QED_Kernels *K)
+{ comb_one(KQED_COMB_ALL_M, M, xv, yv, &t, K); }
+extern "C" void compute_all_Mkernels_v2(const double M[4], const double xv[4], const double yv[4],
+                                        const struct QED_kernel_temps t, struct QED_Kernels *K)
+{ comb_one(KQED_COMB_ALL_M_V2, M, xv, yv, &t, K); }
+extern "C" void compute_con_kernels(const double xv[4], const double yv[4], const struct QED_kernel_temps t, struct QED_Kernels *K)
+{ comb_one(KQED_COMB_CON, nullptr, xv, yv, &t, K); }
+extern "C" void compute_con_kernels_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t, struct Kernels *K)
+{ comb_one(KQED_COMB_CON_V2, nullptr, xv, yv, &t, K); }
+extern "C" void compute_con_kernelsM_L2(const double M[4], const double xv[4], const double yv[4],
+                                        const struct QED_kernel_temps t, struct Kernels *K)
+{ comb_one(KQED_COMB_CON_M_L2, M, xv, yv, &t, K); }
+extern "C" void compute_sub_kernelsM_L2(const double M[4], const double xv[4], const double yv[4],
+                                        const struct QED_kernel_temps t, struct Kernels *K)
+{ comb_one(KQED_COMB_SUB_M_L2, M, xv, yv, &t, K); }
+extern "C" void compute_all_kernels_SYMXY(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                          struct QED_Kernels *K)
+{ comb_one(KQED_COMB_SYMXY, nullptr, xv, yv, &t, K); }
+extern "C" void compute_all_kernels_SYMXY_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                             struct Kernels *K)
+{ comb_one(KQED_COMB_SYMXY_V2, nullptr, xv, yv, &t, K); }
+extern "C" void compute_all_kernels_SYMXY0(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                           struct QED_Kernels *K)
+{ comb_one(KQED_COMB_SYMXY0, nullptr, xv, yv, &t, K); }
+extern "C" void compute_all_kernels_SYMXY0_v2(const double xv[4], const double yv[4], const struct QED_kernel_temps t,
+                                              struct Kernels *K)
+{ comb_one(KQED_COMB_SYMXY0_V2, nullptr, xv, yv, &t, K); }
+
+/* K = (K + W) / 2 elementwise over nk kernels (all_kernels.c:18-82, con_kernel.c:13-59) */
+static void average_kernels(double *K, const double *W, size_t nk)
+{
+  for (size_t i = 0; i < 384 * nk; i++) K[i] = 0.5 * (K[i] + W[i]);
+}
+extern "C" void average_all_kernels(struct Kernels *K, const struct Kernels W)
+{ average_kernels(&K->L0[0][0][0][0], &W.L0[0][0][0][0], 4); }
+extern "C" void average_all_QED_kernels(struct QED_Kernels *K, const struct QED_Kernels W)
+{ average_kernels(&K->L0.xy[0][0][0][0], &W.L0.xy[0][0][0][0], 8); }
+extern "C" void average_con_kernels(struct Kernels *K, const struct Kernels *W)
+{ average_kernels(&K[0].L0[0][0][0][0], &W[0].L0[0][0][0][0], 12); }
+
+/* mu <-> nu of every .yx kernel (all_kernels.c:372-406) */
+extern "C" void swap_munu_Lyx(struct QED_Kernels *K)
+{
+  struct Kernel *ks[4] = { &K->L0, &K->L1, &K->L2, &K->L3 };
+  for (struct Kernel *k : ks)
+    for (int rs = 0; rs < 6; rs++)
+      for (int mu = 0; mu < 4; mu++)
+        for (int nu = mu + 1; nu < 4; nu++)
+          for (int la = 0; la < 4; la++) std::swap(k->yx[rs][mu][nu][la], k->yx[rs][nu][mu][la]);
 }
 
 static void one(int which, double M, const double *xv, const double *yv, const struct QED_kernel_temps *t, double *k)

@@ -1544,3 +1544,301 @@ void oq_contract(const oq_tables *T, int which, double M, size_t nx, const doubl
   free(dot);
   (void)nthreads;
 }
+
+/* ------------------------------------------------------------------ kernel combinations
+ * The reference's combination routines (all_kernels.c, con_kernel.c, sub_kernel.c, SYMXY.c,
+ * SYMXY0.c): L0 evaluated at derived arguments, combined with permuted (mu,nu,lambda) indices.
+ * Output per pair in the reference's struct layout (KQED.h:105-124), 384 doubles per kernel:
+ *   struct QED_Kernels: L0.xy, L0.yx, L1.xy, L1.yx, L2.xy, L2.yx, L3.xy, L3.yx
+ *   struct Kernels:     L0, L1, L2, L3   (the con / sub variants fill three of them: A, B, C)
+ * Expressions keep the reference's operand order (C's left-to-right + and -). */
+enum { CE_XY, CE_YX, CE_X_XMY, CE_XMY_X, CE_YMX_Y, CE_Y_YMX, CE_X_MY, CE_X0, CE_0X, CE_Y0, CE_0Y, CE_XMY0, CE_0XMY, CE_N };
+
+static inline int kx(int rs, int a, int b, int c) { return 64 * rs + 16 * a + 4 * b + c; }
+
+/* sum_a v_a L[..] with the contracted index in position 1, 2 or 3 of (mu,nu,lambda) */
+static inline double dot1(const double v[4], const double *L, int rs, int b, int c)
+{ return v[0] * L[kx(rs, 0, b, c)] + v[1] * L[kx(rs, 1, b, c)] + v[2] * L[kx(rs, 2, b, c)] + v[3] * L[kx(rs, 3, b, c)]; }
+static inline double dot2(const double v[4], const double *L, int rs, int a, int c)
+{ return v[0] * L[kx(rs, a, 0, c)] + v[1] * L[kx(rs, a, 1, c)] + v[2] * L[kx(rs, a, 2, c)] + v[3] * L[kx(rs, a, 3, c)]; }
+
+/* L0 at the derived arguments; x-y = x_i - y_i, y-x = -(x-y), -y = -y_i (all_kernels.c:296,
+ * con_kernel.c:67, SYMXY0.c:28-37) */
+static void comb_evals(const oq_tables *T, const double x[4], const double y[4], double L[CE_N][384])
+{
+  const double z[4] = { 0, 0, 0, 0 };
+  double xmy[4], ymx[4], my[4];
+  for (int i = 0; i < 4; i++) { xmy[i] = x[i] - y[i]; ymx[i] = -xmy[i]; my[i] = -y[i]; }
+  const double *a[CE_N] = { x, y, x, xmy, ymx, y, x, x, z, y, z, xmy, z };
+  const double *b[CE_N] = { y, x, xmy, x, y, ymx, my, z, x, z, y, z, xmy };
+  for (int e = 0; e < CE_N; e++) oq_L0(T, a[e], b[e], L[e]);
+}
+
+static void gauss4(const double M[4], const double v[4], double e[4])
+{
+  const double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + v[3] * v[3];
+  for (int a = 0; a < 4; a++) e[a] = exp(-M[a] * s / 2.);
+}
+
+/* compute_all_kernels (all_kernels.c:85-160): L^k(x,y), L^k(y,x) from six L0 */
+static void comb_all(double L[CE_N][384], double *K)
+{
+  for (int i = 0; i < 384; i++) {
+    const int rs = i / 64, m = (i / 16) & 3, n = (i / 4) & 3, l = i & 3;
+    const int j = kx(rs, l, n, m); /* i_to_mulam (all_kernels.c:10-15): [rs][lambda][nu][mu] */
+    const double xml = L[CE_X0][j], yml = L[CE_Y0][j];
+    const double f1 = (xml + yml) / 2.;
+    const double f2 = L[CE_X0][i] + L[CE_0Y][i];
+    const double f4 = L[CE_0X][i] + L[CE_Y0][i];
+    const double f3 = L[CE_0X][i] - L[CE_0Y][i];
+    const double a = L[CE_XY][i], b = L[CE_YX][i];
+    K[0 * 384 + i] = a;
+    K[1 * 384 + i] = b;
+    K[2 * 384 + i] = a + f1;
+    K[3 * 384 + i] = b + f1;
+    K[4 * 384 + i] = a - f2;
+    K[5 * 384 + i] = b - f4;
+    K[6 * 384 + i] = a + xml + f3;
+    K[7 * 384 + i] = b + yml - f3;
+  }
+}
+
+/* compute_all_Mkernels (all_kernels.c:211-286) and _v2 (:288-369): the M-L2 subtraction with
+ * M[a] for kernel a; v2 puts L(x,y) + L_{mu<->nu}(y,x) in .xy and the (x,-y) kernel in .yx */
+static void comb_all_m(double L[CE_N][384], const double M[4], const double x[4], const double y[4], bool v2,
+                       double *K)
+{
+  double eX[4], eY[4];
+  gauss4(M, x, eX);
+  gauss4(M, y, eY);
+  const double *Lxy = L[CE_XY], *Lyx = L[CE_YX], *Lx0 = L[CE_X0], *L0x = L[CE_0X], *Ly0 = L[CE_Y0],
+               *L0y = L[CE_0Y], *Lxmy = L[CE_X_MY];
+  for (int rs = 0; rs < 6; rs++)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++)
+        for (int l = 0; l < 4; l++) {
+          const int i = kx(rs, m, n, l), t = kx(rs, n, m, l);
+          for (int a = 0; a < 4; a++) {
+            /* inlineXY(a) (all_kernels.c:163-172) */
+            const double sxy = eX[a] * (L0y[i] - M[a] * x[m] * dot1(x, L0y, rs, n, l)) +
+                               eY[a] * (Lx0[i] - M[a] * y[n] * dot2(y, Lx0, rs, m, l));
+            if (!v2) {
+              /* inlineYX(a) (:187-196) */
+              const double syx = eY[a] * (L0x[i] - M[a] * y[m] * dot1(y, L0x, rs, n, l)) +
+                                 eX[a] * (Ly0[i] - M[a] * x[n] * dot2(x, Ly0, rs, m, l));
+              K[(2 * a) * 384 + i] = Lxy[i] - sxy;
+              K[(2 * a + 1) * 384 + i] = Lyx[i] - syx;
+            } else {
+              /* inlineYX2(a) (:199-208) and inlineXMY(a) (:175-184) */
+              const double syx2 = eY[a] * (L0x[t] - M[a] * y[n] * dot1(y, L0x, rs, m, l)) +
+                                  eX[a] * (Ly0[t] - M[a] * x[m] * dot2(x, Ly0, rs, n, l));
+              const double sxmy = -eX[a] * (L0y[i] - M[a] * x[m] * dot1(x, L0y, rs, n, l)) +
+                                  eY[a] * (Lx0[i] - M[a] * y[n] * dot2(y, Lx0, rs, m, l));
+              K[(2 * a) * 384 + i] = Lxy[i] - sxy + Lyx[t] - syx2;
+              K[(2 * a + 1) * 384 + i] = Lxmy[i] - sxmy;
+            }
+          }
+        }
+}
+
+/* compute_con_kernels_v2 (con_kernel.c:61-152): struct Kernels K[3] = A, B, C with
+ * A = L^k(x,y), B = L^k_{lambda nu mu}(x,x-y), C = A + L^k_{nu mu lambda}(y,x) - B */
+static void comb_con_v2(double L[CE_N][384], double *K)
+{
+  const double *A = L[CE_XY], *Yx = L[CE_YX], *Xm = L[CE_X_XMY], *x0 = L[CE_X0], *o_x = L[CE_0X], *y0 = L[CE_Y0],
+               *o_y = L[CE_0Y], *m0 = L[CE_XMY0], *o_m = L[CE_0XMY];
+  double *A0 = K, *A1 = K + 384, *A2 = K + 768, *A3 = K + 1152;
+  double *B0 = K + 1536, *B1 = K + 1920, *B2 = K + 2304, *B3 = K + 2688;
+  double *C0 = K + 3072, *C1 = K + 3456, *C2 = K + 3840, *C3 = K + 4224;
+  for (int rs = 0; rs < 6; rs++)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++)
+        for (int l = 0; l < 4; l++) {
+          const int mnl = kx(rs, m, n, l), nml = kx(rs, n, m, l), lnm = kx(rs, l, n, m), lmn = kx(rs, l, m, n);
+          A0[mnl] = A[mnl];
+          B0[mnl] = Xm[lnm];
+          C0[mnl] = A0[mnl] + Yx[nml] - Xm[lnm];
+          A1[mnl] = A0[mnl] + (x0[lnm] + y0[lnm]) / 2;
+          B1[mnl] = B0[mnl] + (x0[mnl] + m0[mnl]) / 2;
+          C1[mnl] = A1[mnl] - B1[mnl] + Yx[nml] + (y0[lmn] + x0[lmn]) / 2;
+          A2[mnl] = A0[mnl] - (o_y[mnl] + x0[mnl]);
+          B2[mnl] = B0[mnl] - (o_m[lnm] + x0[lnm]);
+          C2[mnl] = A2[mnl] - B2[mnl] + Yx[nml] - (o_x[nml] + y0[nml]);
+          A3[mnl] = A0[mnl] + (x0[lnm] + o_x[mnl] - o_y[mnl]);
+          B3[mnl] = B0[mnl] + (x0[mnl] + o_x[lnm] - o_m[lnm]);
+          C3[mnl] = A3[mnl] - B3[mnl] + Yx[nml] + (y0[lmn] + o_y[nml] - o_x[nml]);
+        }
+}
+
+/* compute_con_kernels (con_kernel.c:154-244): struct QED_Kernels with .xy = C and .yx = B */
+static void comb_con(double L[CE_N][384], double *K)
+{
+  const double *A = L[CE_XY], *Yx = L[CE_YX], *Xm = L[CE_X_XMY], *x0 = L[CE_X0], *o_x = L[CE_0X], *y0 = L[CE_Y0],
+               *o_y = L[CE_0Y], *m0 = L[CE_XMY0], *o_m = L[CE_0XMY];
+  for (int rs = 0; rs < 6; rs++)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++)
+        for (int l = 0; l < 4; l++) {
+          const int mnl = kx(rs, m, n, l), nml = kx(rs, n, m, l), lnm = kx(rs, l, n, m), lmn = kx(rs, l, m, n);
+          const double c0 = A[mnl] + Yx[nml] - Xm[lnm], b0 = Xm[lnm];
+          const double c2 = c0 - (o_y[mnl] + x0[mnl] + o_x[nml] + y0[nml] - o_m[lnm] - x0[lnm]);
+          K[0 * 384 + mnl] = c0;
+          K[1 * 384 + mnl] = b0;
+          K[2 * 384 + mnl] = c0 + (x0[lnm] + y0[lnm] + x0[lmn] + y0[lmn] - x0[mnl] - m0[mnl]) / 2;
+          K[3 * 384 + mnl] = b0 + (x0[mnl] + m0[mnl]) / 2;
+          K[4 * 384 + mnl] = c2;
+          K[5 * 384 + mnl] = b0 - (x0[lnm] + o_m[lnm]);
+          K[6 * 384 + mnl] = c2 + (o_x[mnl] + o_y[nml] + y0[lmn] - o_x[lnm] + y0[nml]);
+          K[7 * 384 + mnl] = b0 + (x0[mnl] + o_x[lnm] - o_m[lnm]);
+        }
+}
+
+/* compute_con_kernelsM_L2 (con_kernel.c:246-399) and compute_sub_kernelsM_L2 (sub_kernel.c:26-167):
+ * K[0].La = M-L2 at M[a] of (x,y), K[1].La the (x,x-y) one with (mu,nu,lambda) read as
+ * (lambda,nu,mu) [con] or (nu,lambda,mu) [sub], K[2] = K[0] + (y,x)_{nu mu lambda} - K[1] */
+static void comb_m_l2(double L[CE_N][384], const double M[4], const double x[4], const double y[4], bool sub,
+                      double *K)
+{
+  double xmy[4], eX[4], eY[4], eXMY[4];
+  for (int i = 0; i < 4; i++) xmy[i] = x[i] - y[i];
+  gauss4(M, x, eX);
+  gauss4(M, y, eY);
+  gauss4(M, xmy, eXMY);
+  const double *Kxy = L[CE_XY], *Kyx = L[CE_YX], *Kxxmy = L[CE_X_XMY], *Kx0 = L[CE_X0], *K0x = L[CE_0X],
+               *Ky0 = L[CE_Y0], *K0y = L[CE_0Y], *K0xmy = L[CE_0XMY];
+  for (int rs = 0; rs < 6; rs++)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++)
+        for (int l = 0; l < 4; l++) {
+          const int mnl = kx(rs, m, n, l), nml = kx(rs, n, m, l);
+          const double pf1 = dot1(x, K0y, rs, n, l);  /* x . K0y[.][nu][lambda] */
+          const double pf2 = dot2(y, Kx0, rs, m, l);  /* y . Kx0[mu][.][lambda] */
+          const double pf3 = dot1(y, K0x, rs, m, l);  /* y . K0x[.][mu][lambda] */
+          const double pf4 = dot2(x, Ky0, rs, n, l);  /* x . Ky0[nu][.][lambda] */
+          int p;
+          double pf5, pf6, f5, f6;
+          if (!sub) {
+            p = kx(rs, l, n, m);
+            pf5 = dot1(x, K0xmy, rs, n, m);           /* x . K0xmy[.][nu][mu] */
+            pf6 = dot2(xmy, Kx0, rs, l, m);           /* (x-y) . Kx0[lambda][.][mu] */
+            f5 = x[l]; f6 = xmy[n];
+          } else {
+            p = kx(rs, n, l, m);
+            pf5 = dot1(x, K0xmy, rs, l, m);           /* x . K0xmy[.][lambda][mu] */
+            pf6 = dot2(xmy, Kx0, rs, n, m);           /* (x-y) . Kx0[nu][.][mu] */
+            f5 = x[n]; f6 = xmy[l];
+          }
+          for (int a = 0; a < 4; a++) {
+            const double lxy = Kxy[mnl] - eX[a] * (K0y[mnl] - M[a] * x[m] * pf1) - eY[a] * (Kx0[mnl] - M[a] * y[n] * pf2);
+            const double lyx = Kyx[nml] - eY[a] * (K0x[nml] - M[a] * y[n] * pf3) - eX[a] * (Ky0[nml] - M[a] * x[m] * pf4);
+            const double lxm = Kxxmy[p] - eX[a] * (K0xmy[p] - M[a] * f5 * pf5) - eXMY[a] * (Kx0[p] - M[a] * f6 * pf6);
+            K[(0 + a) * 384 + mnl] = lxy;
+            K[(4 + a) * 384 + mnl] = lxm;
+            K[(8 + a) * 384 + mnl] = lxy + lyx - lxm;
+          }
+        }
+}
+
+/* compute_all_kernels_SYMXY(_v2) (SYMXY.c:12-85): (L^k(x,y) + L^k_{nu mu lambda}(y,x)) / 2 */
+static void comb_symxy(double L[CE_N][384], bool v2, double *K)
+{
+  double W[8 * 384];
+  comb_all(L, W);
+  for (int k = 0; k < 4; k++) {
+    double *o = v2 ? K + 384 * k : K + 384 * (2 * k);
+    for (int rs = 0; rs < 6; rs++)
+      for (int m = 0; m < 4; m++)
+        for (int n = 0; n < 4; n++)
+          for (int l = 0; l < 4; l++)
+            o[kx(rs, m, n, l)] = (W[384 * (2 * k) + kx(rs, m, n, l)] + W[384 * (2 * k + 1) + kx(rs, n, m, l)]) / 2;
+    if (!v2)
+      for (int rs = 0; rs < 6; rs++)
+        for (int m = 0; m < 4; m++)
+          for (int n = 0; n < 4; n++)
+            for (int l = 0; l < 4; l++) K[384 * (2 * k + 1) + kx(rs, m, n, l)] = o[kx(rs, n, m, l)];
+  }
+}
+
+/* compute_all_kernels_SYMXY0(_v2) (SYMXY0.c:21-211): the average over the six argument and index
+ * permutations of the three-point function */
+static void comb_symxy0(double L[CE_N][384], bool v2, double *K)
+{
+  const double *A = L[CE_XY], *Yx = L[CE_YX], *MX = L[CE_XMY_X], *YMY = L[CE_YMX_Y], *XXM = L[CE_X_XMY],
+               *YYM = L[CE_Y_YMX], *x0 = L[CE_X0], *o_x = L[CE_0X], *y0 = L[CE_Y0], *o_y = L[CE_0Y],
+               *m0 = L[CE_XMY0], *o_m = L[CE_0XMY];
+  const int s = v2 ? 1 : 2; /* kernel k at 384*s*k */
+  for (int rs = 0; rs < 6; rs++)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++)
+        for (int l = 0; l < 4; l++) {
+          const int mnl = kx(rs, m, n, l), nml = kx(rs, n, m, l), lnm = kx(rs, l, n, m), lmn = kx(rs, l, m, n),
+                    nlm = kx(rs, n, l, m), mln = kx(rs, m, l, n);
+          const double k0 = (A[mnl] + Yx[nml] - MX[nlm] - YMY[mln] - XXM[lnm] - YYM[lmn]) / 6;
+          const double apc = x0[mnl] - x0[lnm] + y0[nml] - y0[lmn];
+          const double k1 = k0 + (-apc + x0[lmn] - x0[mln] + y0[lnm] - y0[nlm] + m0[nml] - m0[mnl] + m0[nlm] - m0[mln]) / 12;
+          const double k2 = k0 - (apc + o_x[nml] - o_x[nlm] + o_y[mnl] - o_y[mln] + m0[mln] - m0[nlm] + o_m[lmn] - o_m[lnm]) / 6.;
+          const double k3 = k2 + (o_x[mnl] - o_x[lnm] + o_y[nml] - o_y[lmn] + o_m[mln] - o_m[nlm]) / 6.;
+          K[384 * s * 0 + mnl] = k0;
+          K[384 * s * 1 + mnl] = k1;
+          K[384 * s * 2 + mnl] = k2;
+          K[384 * s * 3 + mnl] = k3;
+        }
+  if (!v2)
+    for (int k = 0; k < 4; k++)
+      for (int rs = 0; rs < 6; rs++)
+        for (int m = 0; m < 4; m++)
+          for (int n = 0; n < 4; n++)
+            for (int l = 0; l < 4; l++) K[384 * (2 * k + 1) + kx(rs, m, n, l)] = K[384 * (2 * k) + kx(rs, n, m, l)];
+}
+
+int oq_comb_len(int comb)
+{
+  switch (comb) {
+  case OQ_COMB_ALL: case OQ_COMB_ALL_M: case OQ_COMB_ALL_M_V2: case OQ_COMB_CON: case OQ_COMB_SYMXY: case OQ_COMB_SYMXY0:
+    return 8 * 384;
+  case OQ_COMB_CON_V2: case OQ_COMB_CON_M_L2: case OQ_COMB_SUB_M_L2:
+    return 12 * 384;
+  case OQ_COMB_SYMXY_V2: case OQ_COMB_SYMXY0_V2:
+    return 4 * 384;
+  default:
+    return 0;
+  }
+}
+
+int oq_combine(const oq_tables *T, int comb, const double M[4], const double x[4], const double y[4], double *K)
+{
+  static const double M0[4] = { 0, 0, 0, 0 };
+  if (!oq_comb_len(comb)) return 1;
+  if (!M) M = M0;
+  double (*L)[384] = (double (*)[384])malloc(sizeof(double) * CE_N * 384);
+  if (!L) return 1;
+  comb_evals(T, x, y, L);
+  switch (comb) {
+  case OQ_COMB_ALL: comb_all(L, K); break;
+  case OQ_COMB_ALL_M: comb_all_m(L, M, x, y, false, K); break;
+  case OQ_COMB_ALL_M_V2: comb_all_m(L, M, x, y, true, K); break;
+  case OQ_COMB_CON: comb_con(L, K); break;
+  case OQ_COMB_CON_V2: comb_con_v2(L, K); break;
+  case OQ_COMB_CON_M_L2: comb_m_l2(L, M, x, y, false, K); break;
+  case OQ_COMB_SUB_M_L2: comb_m_l2(L, M, x, y, true, K); break;
+  case OQ_COMB_SYMXY: comb_symxy(L, false, K); break;
+  case OQ_COMB_SYMXY_V2: comb_symxy(L, true, K); break;
+  case OQ_COMB_SYMXY0: comb_symxy0(L, false, K); break;
+  case OQ_COMB_SYMXY0_V2: comb_symxy0(L, true, K); break;
+  }
+  free(L);
+  return 0;
+}
+
+void oq_combine_batch(const oq_tables *T, int comb, const double M[4], size_t n, const double *x, const double *y,
+                      double *out, int nthreads)
+{
+  maps_init();
+  const size_t len = (size_t)oq_comb_len(comb);
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+  for (size_t i = 0; i < n; i++) oq_combine(T, comb, M, x + 4 * i, y + 4 * i, out + len * i);
+  (void)nthreads;
+}

@@ -58,6 +58,28 @@ void oq_batch(const oq_tables *T, int which, double M, size_t n, const double *x
 void oq_contract(const oq_tables *T, int which, double M, size_t nx, const double *x, const float *Tc, size_t ny,
                  const double *y, double *s, double *a, int nthreads);
 
+/* Kernel combinations (all_kernels.c, con_kernel.c, sub_kernel.c, SYMXY.c, SYMXY0.c).  Output per
+ * pair in the reference's struct layout: QED_Kernels (8 x 384: L0.xy, L0.yx, ..., L3.yx), Kernels
+ * (4 x 384: L0..L3) or Kernels[3] (12 x 384).  M has 4 entries (NULL = zeros); returns 1 on a bad
+ * comb. */
+enum {
+  OQ_COMB_ALL = 0,      /* compute_all_kernels            -> QED_Kernels */
+  OQ_COMB_ALL_M = 1,    /* compute_all_Mkernels           -> QED_Kernels */
+  OQ_COMB_ALL_M_V2 = 2, /* compute_all_Mkernels_v2        -> QED_Kernels */
+  OQ_COMB_CON = 3,      /* compute_con_kernels            -> QED_Kernels */
+  OQ_COMB_CON_V2 = 4,   /* compute_con_kernels_v2         -> Kernels[3] */
+  OQ_COMB_CON_M_L2 = 5, /* compute_con_kernelsM_L2        -> Kernels[3] */
+  OQ_COMB_SUB_M_L2 = 6, /* compute_sub_kernelsM_L2        -> Kernels[3] */
+  OQ_COMB_SYMXY = 7,    /* compute_all_kernels_SYMXY      -> QED_Kernels */
+  OQ_COMB_SYMXY_V2 = 8, /* compute_all_kernels_SYMXY_v2   -> Kernels */
+  OQ_COMB_SYMXY0 = 9,   /* compute_all_kernels_SYMXY0     -> QED_Kernels */
+  OQ_COMB_SYMXY0_V2 = 10 /* compute_all_kernels_SYMXY0_v2 -> Kernels */
+};
+int oq_comb_len(int comb); /* doubles per pair */
+int oq_combine(const oq_tables *T, int comb, const double M[4], const double x[4], const double y[4], double *K);
+void oq_combine_batch(const oq_tables *T, int comb, const double M[4], size_t n, const double *x, const double *y,
+                      double *out, int nthreads);
+
 /* Pieces exposed for unit tests. */
 double oq_chebU(int order, int nk, double x, const double *co);          /* cheby.c:13-91 */
 double oq_chebU_avx(int order, int nk, double x, const double *co, const oq_tables *T); /* simd.c:182-336 */

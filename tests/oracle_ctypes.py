@@ -40,6 +40,9 @@ def _lib():
     f4 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
     L.oq_contract.argtypes = [T, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, v4, f4, ctypes.c_size_t, v4, v4, v4,
                               ctypes.c_int]
+    L.oq_comb_len.argtypes = [ctypes.c_int]
+    L.oq_combine.argtypes = [T, ctypes.c_int, v4, v4, v4, v4]
+    L.oq_combine_batch.argtypes = [T, ctypes.c_int, v4, ctypes.c_size_t, v4, v4, v4, ctypes.c_int]
     L.oq_chebU.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4]
     L.oq_chebU.restype = ctypes.c_double
     L.oq_chebU_avx.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4, T]
@@ -127,6 +130,27 @@ class Oracle:
         self.L.oq_contract(ctypes.byref(self.t), int(which), float(M), x.shape[0], x.ravel(), Tc.ravel(), ny, y.ravel(),
                            s, a, int(nthreads))
         return s, a
+
+    def combine(self, comb, x, y, M=None):
+        """One pair through a kernel combination: array (nk, 384) in the reference's struct order."""
+        nk = self.L.oq_comb_len(int(comb)) // 384
+        out = np.zeros(nk * 384)
+        M4 = np.zeros(4) if M is None else np.ascontiguousarray(M, np.float64)
+        rc = self.L.oq_combine(ctypes.byref(self.t), int(comb), M4, np.ascontiguousarray(x, np.float64),
+                               np.ascontiguousarray(y, np.float64), out)
+        if rc:
+            raise ValueError(f"bad combination {comb}")
+        return out.reshape(nk, 384)
+
+    def combine_batch(self, comb, x, y, M=None, nthreads=0):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        nk = self.L.oq_comb_len(int(comb)) // 384
+        out = np.zeros((x.shape[0], nk, 384))
+        M4 = np.zeros(4) if M is None else np.ascontiguousarray(M, np.float64)
+        self.L.oq_combine_batch(ctypes.byref(self.t), int(comb), M4, x.shape[0], x.ravel(), y.ravel(), out.ravel(),
+                                int(nthreads))
+        return out
 
     def stv(self, x, y):
         s = np.zeros(200)

@@ -43,6 +43,9 @@ def test_c_caller_compiles_and_links(tmp_path):
         "  const double x[4] = {0, 0, 0.1, 0.2}, y[4] = {0, 0, 0, 0.3};\n"
         "  if (initialise(&t)) return 1;\n"
         "  QED_kernel_L0(x, y, t, kerv); QED_kernel_L2(x, y, t, kerv); QED_Mkernel_L2(0.5, x, y, t, kerv);\n"
+        "  { struct QED_Kernels K; struct Kernels C[3]; const double M[4] = {0, .4, .8, 1.2};\n"
+        "    compute_all_kernels(x, y, t, &K); compute_con_kernelsM_L2(M, x, y, t, C); swap_munu_Lyx(&K);\n"
+        "    average_con_kernels(C, C); printf(\"%g\\n\", K.L2.xy[0][0][0][0] + C[2].L1[0][1][2][3]); }\n"
         "  printf(\"%g %g\\n\", kerv[0][0][0][0], AlfQED);\n"
         "  free_QED_temps(&t); return 0;\n}\n")
     inc = os.path.join(kqed_amd.ROOT, "include")
