@@ -84,6 +84,29 @@ def contraction_leg(t, dev, stream, L):
             "s_first": float(s[0])}
 
 
+def combination_leg(t, dev, stream, xd, yd, n):
+    """The reference's kernel combinations (all_kernels.c, con_kernel.c, SYMXY0.c) on the first n
+    pairs of the workload: combinations/s and the L0 evaluations/s inside them."""
+    import torch
+    import kqed_amd
+    M = np.array([0.0, 0.4, 0.8, 1.2])
+    legs = (("compute_all_kernels", kqed_amd.KQED_COMB_ALL, 6, None),
+            ("compute_con_kernelsM_L2", kqed_amd.KQED_COMB_CON_M_L2, 8, M),
+            ("compute_all_kernels_SYMXY0_v2", kqed_amd.KQED_COMB_SYMXY0_V2, 12, None))
+    out = torch.empty((n, 12 * 384), dtype=torch.float64, device=dev)
+    res = {"workload": f"first {n} pairs of the L0 workload, FP64, M = (0, 0.4, 0.8, 1.2) for the M variant"}
+    for name, comb, evals, m in legs:
+        t.combine_device(comb, n, xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=m, stream=stream)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t.combine_device(comb, n, xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=m, stream=stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"pairs_per_s": n / dt, "l0_evals_per_s": evals * n / dt, "ms": dt * 1e3}
+    del out
+    return res
+
+
 def ragged_leg(t, dev, stream, reps=3):
     """256 y points with lognormal x-sets (median 16384, sigma 1, clipped [1024, 262144]), L0, FP32
     output (FP32 assembly maps, FP64 interpolation and Taylor branch), autotuned, one graph launch."""
@@ -254,6 +277,7 @@ def main():
     if not a.no_extra:
         extra["contraction"] = contraction_leg(t, dev, stream, a.lattice)
         extra["ragged"] = ragged_leg(t, dev, stream)
+        extra["combinations"] = combination_leg(t, dev, stream, xd, yd, min(a.n, 1 << 19))
         if a.surrogate:
             del out
             extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
