@@ -1,8 +1,8 @@
 /*
  * KQED.h -- source-compatibility header for callers written against the reference library's main
  * header (reference src/HEADERS/KQED.h): the definitions those callers use (KQED_ALIGN, M_PI,
- * AlfQED) and the kernel API of kqed.h.  The reference's kernel-combination, lepton-loop and timer
- * headers (KQED.h:155-163) are not part of this library.
+ * AlfQED) and the API of kqed.h, which covers the reference's kernel, kernel-combination and
+ * lepton-loop headers.  The reference's timer helpers (KQED.h:155-163) are not part of this library.
  */
 #ifndef KQED_COMPAT_H
 #define KQED_COMPAT_H

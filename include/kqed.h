@@ -123,6 +123,22 @@ void average_all_QED_kernels(struct QED_Kernels *K, const struct QED_Kernels W);
 void average_con_kernels(struct Kernels *K /* [3] */, const struct Kernels *W /* [3] */);
 void swap_munu_Lyx(struct QED_Kernels *K);
 
+/* Perturbative lepton loop (reference pi_pert.h): the free-lepton four-point function
+ * iPihat_{rho sigma; mu nu lambda}(x, y) (x, y in units of the loop lepton's inverse mass; singular
+ * where x, y or x-y vanishes) and the analytic light-by-light results it is checked against.
+ * ipihatFermLoop_antisym and pihat1 run on the GPU (one pair each, synchronous); the rest is host
+ * arithmetic.  Replace pi_pert.c:50-397.  amu_heavlept / amu_lightlept do not print their terms. */
+double amu_heavlept(const double mh);
+double amu_lightlept(const double ml);
+double vac_pol(const double ml);
+void bessk(const int n, const double x, double *bv);
+int diractrace(const int n, const int *idx);
+void init_g8(struct QED_kernel_temps *t);
+void ipihatFermLoop_antisym(const double x[4], const double y[4], const struct QED_kernel_temps t,
+                            double vpihat[6][4][4][4]);
+void pihat1(const double x[4], const double y[4], const struct QED_kernel_temps t,
+            double vpihat1[4][4][4][4][4], double vpihatr1[4][4][4][4]);
+
 /* ---------------------------------------------------------------- batched entry points (new) */
 enum kqed_kernel { KQED_L0 = 0, KQED_L1 = 1, KQED_L2 = 2, KQED_L3 = 3, KQED_ML2 = 4 };
 /* Output layouts.  The FP32 layouts also run the sparse assembly maps in FP32 (interpolation, the
@@ -274,6 +290,17 @@ int kqed_combine(const struct QED_kernel_temps *t, int comb, const double *M, si
 /* Same on host buffers (copies in and out, synchronous). */
 int kqed_combine_host(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
                       const double *x, const double *y, double *out);
+
+/* Lepton loop, batched (device pointers, enqueued on `stream`):
+ *   kqed_pihat:       out[n][384] = iPihat(x_i, y_i)  (ipihatFermLoop_antisym per pair)
+ *   kqed_lepton_loop: out[n] = sum_c L(x_i, y_i)_c iPihat(Mv x_i, Mv y_i)_c, the integrand of the
+ *                     reference's example1 (amu_forlattice.c:66-126) before its normalisation,
+ *                     with L = L0..L3 / M-L2 (`which`, `M`) and Mv = lepton mass / muon mass. */
+int kqed_pihat(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
+               double *out_dev, void *stream);
+int kqed_pihat_host(const struct QED_kernel_temps *t, size_t n, const double *x, const double *y, double *out);
+int kqed_lepton_loop(const struct QED_kernel_temps *t, int which, double M, double Mv, size_t n,
+                     const double *x_dev, const double *y_dev, double *out_dev, void *stream);
 
 #ifdef __cplusplus
 }

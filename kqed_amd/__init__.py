@@ -108,6 +108,18 @@ def lib():
     L.kqed_comb_kernels.argtypes = [ctypes.c_int]
     L.kqed_combine.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, vp, vp]
     L.kqed_combine_host.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, arr, arr, vp]
+    L.kqed_pihat.argtypes = [P, ctypes.c_size_t, vp, vp, vp, vp]
+    L.kqed_pihat_host.argtypes = [P, ctypes.c_size_t, arr, arr, arr]
+    L.kqed_lepton_loop.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, vp, vp, vp,
+                                   vp]
+    L.pihat1.argtypes = [arr, arr, _Temps, arr, arr]
+    L.pihat1.restype = None
+    for name in ("amu_heavlept", "amu_lightlept", "vac_pol"):
+        getattr(L, name).argtypes = [ctypes.c_double]
+        getattr(L, name).restype = ctypes.c_double
+    L.bessk.argtypes = [ctypes.c_int, ctypes.c_double, arr]
+    L.bessk.restype = None
+    L.diractrace.argtypes = [ctypes.c_int, np.ctypeslib.ndpointer(np.intc, flags="C_CONTIGUOUS")]
     L.kqed_get_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_set_tune.argtypes = [P, ctypes.POINTER(Tune)]
     L.kqed_autotune.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p,
@@ -272,6 +284,37 @@ class QEDKernelTemps:
                                 ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError("kqed_combine failed: " + lib().kqed_last_error().decode())
+
+    def pihat(self, x, y):
+        """ipihatFermLoop_antisym for n pairs of host arrays -> (n, 384) float64 (computed on the GPU)."""
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 4)
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 4)
+        out = np.zeros((x.shape[0], 384))
+        rc = lib().kqed_pihat_host(ctypes.byref(self._t), x.shape[0], x.reshape(-1), y.reshape(-1), out.reshape(-1))
+        if rc != 0:
+            raise RuntimeError("kqed_pihat_host failed: " + lib().kqed_last_error().decode())
+        return out
+
+    def pihat1(self, x, y):
+        """pihat1 of one pair -> (v1[4,4,4,4,4], r1[4,4,4,4])."""
+        v = np.zeros(1024)
+        r = np.zeros(256)
+        lib().pihat1(np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64), self._t, v, r)
+        return v.reshape(4, 4, 4, 4, 4), r.reshape(4, 4, 4, 4)
+
+    def pihat_device(self, n, x_ptr, y_ptr, out_ptr, stream=None):
+        rc = lib().kqed_pihat(ctypes.byref(self._t), int(n), ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr),
+                              ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_pihat failed: " + lib().kqed_last_error().decode())
+
+    def lepton_loop_device(self, which, n, x_ptr, y_ptr, out_ptr, M=0.0, Mv=1.0, stream=None):
+        """out[i] = sum_c L(x_i, y_i)_c iPihat(Mv x_i, Mv y_i)_c on device pointers."""
+        rc = lib().kqed_lepton_loop(ctypes.byref(self._t), int(which), float(M), float(Mv), int(n),
+                                    ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr),
+                                    ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_lepton_loop failed: " + lib().kqed_last_error().decode())
 
     def get_tune(self):
         tu = Tune()

@@ -33,11 +33,11 @@ def _run(cmd):
 def build(verbose=True):
     os.makedirs(OUT, exist_ok=True)
     objs = []
-    for c in ("kqed_tables.c", "kqed_assembly.c"):
+    for c in ("kqed_tables.c", "kqed_assembly.c", "kqed_pert_host.c"):
         o = os.path.join(OUT, c + ".o")
         _run(["gcc", *C_FLAGS, "-c", os.path.join(SRC, c), "-o", o])
         objs.append(o)
-    for h in ("kqed_kernels.hip", "kqed_surrogate.hip", "kqed_combine.hip", "kqed_api.hip"):
+    for h in ("kqed_kernels.hip", "kqed_surrogate.hip", "kqed_combine.hip", "kqed_pert.hip", "kqed_api.hip"):
         o = os.path.join(OUT, h + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(SRC, h), "-o", o])
         objs.append(o)

@@ -19,6 +19,7 @@
 #include "kqed_assembly.h"
 #include "kqed_combine.h"
 #include "kqed_device.h"
+#include "kqed_pert.h"
 #include "kqed_tables.h"
 
 /* kernels (kqed_kernels.hip) */
@@ -60,7 +61,7 @@ __global__ void k_surr_binerr(size_t n, const double *a, const double *b, const 
                               float *err);
 template <typename R>
 __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, void *out, int layout,
-                           int nslots, size_t ntot, const float *Tc, const double *Wc);
+                           int nslots, size_t ntot, const void *Tc, const double *Wc);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
 __global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
                                    double *W);
@@ -97,6 +98,7 @@ extern "C" const char *kqed_last_error(void) { return g_err; }
 /* internal output modes: per-pair dot products with a site tensor (assembly kernel), and the
  * extracted interpolation results of the surrogate's training data (no algebra, no assembly) */
 static const int LAYOUT_DOT = 16;
+static const int LAYOUT_DOT64 = 64; /* per-pair dot with an FP64 tensor (passed as Tc) */
 static const int LAYOUT_SCALARS = 32;
 
 /* The surrogate as the pipeline sees it: network parameters for the inference kernel and the
@@ -117,6 +119,9 @@ struct Workspace {
   /* kernel combinations: derived argument pairs [2][ccap][4] and their L0 tensors [ccap][384] */
   size_t ccap = 0;
   double *cargs = nullptr, *cL = nullptr;
+  /* lepton loop: pihat1 evaluations [3 pcap][1280], iPihat [pcap][384], scaled pairs [2][pcap][4] */
+  size_t pcap = 0;
+  double *pV = nullptr, *pP = nullptr, *pX = nullptr;
 };
 
 /* optional per-stage timing: events recorded on the launch stream around every kernel */
@@ -136,6 +141,8 @@ struct kqed_device {
   int *csc_cp = nullptr;
   short *csc_rows = nullptr;
   double *csc_vals = nullptr;
+  /* trace tables of the perturbative lepton loop (built on first use) */
+  signed char *pert_tab = nullptr;
   std::mutex mu;
   std::unordered_map<void *, Workspace> ws;
   bool profiling = false;
@@ -195,6 +202,7 @@ static void free_ws(Workspace &w)
   (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
   (void)hipFree(w.cargs); (void)hipFree(w.cL);
+  (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
   w = Workspace();
 }
 
@@ -206,6 +214,7 @@ extern "C" void free_QED_temps(struct QED_kernel_temps *t)
     (void)hipSetDevice(d->device);
     (void)hipDeviceSynchronize();
     for (auto &kv : d->ws) free_ws(kv.second);
+    (void)hipFree(d->pert_tab);
     for (int i = 0; i < d->nalloc; i++) (void)hipFree(d->allocs[i]);
     free(d->maps);
     delete d;
@@ -445,7 +454,7 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
  * chain-rule algebra, assembly).  Only kernel launches: it can be captured into a graph once the
  * workspace is sized (ensure_ws with the same sizes beforehand). */
 static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, double M, size_t n, const double *x_dev,
-                     int xs, const double *y_dev, int ys, void *out_dev, int layout, const float *Tc,
+                     int xs, const double *y_dev, int ys, void *out_dev, int layout, const void *Tc,
                      const double *Wc, const SurrUse *sur = nullptr)
 {
   if (n == 0) return 0;
@@ -1001,6 +1010,172 @@ extern "C" int kqed_combine_host(const struct QED_kernel_temps *t, int comb, con
     return -3;
   }
   return 0;
+}
+
+/* ------------------------------------------------------------------ perturbative lepton loop */
+
+static int ensure_pert_tab(kqed_device *d, const struct QED_kernel_temps *t, hipStream_t s)
+{
+  if (d->pert_tab) return 0;
+  if (!t->G8) { set_err("lepton loop: the G8 trace table is missing"); return -1; }
+  std::vector<signed char> h(kqed_pert_table_bytes());
+  kqed_pert_build_table(t->G8, h.data());
+  signed char *p = nullptr;
+  if (hipMalloc(&p, h.size()) != hipSuccess) { set_err("lepton loop: table allocation failed"); return -2; }
+  if (hipMemcpyAsync(p, h.data(), h.size(), hipMemcpyHostToDevice, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+    (void)hipFree(p);
+    set_err("lepton loop: table upload failed");
+    return -3;
+  }
+  d->pert_tab = p;
+  return 0;
+}
+
+static int ensure_pert_ws(Workspace &w, size_t c)
+{
+  if (w.pcap >= c) return 0;
+  (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
+  w.pV = w.pP = w.pX = nullptr; w.pcap = 0;
+  if (hipMalloc(&w.pV, 8 * 3 * 1280 * c) != hipSuccess || hipMalloc(&w.pP, 8 * 384 * c) != hipSuccess ||
+      hipMalloc(&w.pX, 64 * c) != hipSuccess) {
+    set_err("lepton loop workspace allocation of %zu pairs failed", c);
+    return -2;
+  }
+  w.pcap = c;
+  return 0;
+}
+
+/* pairs per lepton-loop pass: 30 KiB of pihat1 evaluations and 3 KiB of iPihat each */
+static const size_t PERT_CHUNK = (size_t)1 << 16;
+
+extern "C" int kqed_pihat(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
+                          double *out_dev, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_pihat: tables not initialised"); return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull / 3) { set_err("kqed_pihat: too many pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_pert_tab(d, t, s);
+  if (rc) return rc;
+  Workspace &w = d->ws[stream];
+  const size_t CH = n < PERT_CHUNK ? n : PERT_CHUNK;
+  if ((rc = ensure_pert_ws(w, CH))) return rc;
+  for (size_t c0 = 0; c0 < n; c0 += CH) {
+    const size_t c = n - c0 < CH ? n - c0 : CH;
+    kqed_pert_launch_ipihat(c, x_dev + 4 * c0, y_dev + 4 * c0, d->pert_tab, w.pV, out_dev + 384 * c0, s);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
+  return 0;
+}
+
+extern "C" int kqed_lepton_loop(const struct QED_kernel_temps *t, int which, double M, double Mv, size_t n,
+                                const double *x_dev, const double *y_dev, double *out_dev, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_lepton_loop: tables not initialised"); return -1; }
+  if (which < 0 || which > 4) { set_err("kqed_lepton_loop: bad kernel"); return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull / 3) { set_err("kqed_lepton_loop: too many pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_pert_tab(d, t, s);
+  if (rc) return rc;
+  Workspace &w = d->ws[stream];
+  const size_t CH = n < PERT_CHUNK ? n : PERT_CHUNK;
+  if ((rc = ensure_pert_ws(w, CH))) return rc;
+  for (size_t c0 = 0; c0 < n; c0 += CH) {
+    const size_t c = n - c0 < CH ? n - c0 : CH;
+    double *xs = w.pX, *ys = w.pX + 4 * c;
+    kqed_pert_launch_scale(c, Mv, x_dev + 4 * c0, y_dev + 4 * c0, xs, ys, s);
+    kqed_pert_launch_ipihat(c, xs, ys, d->pert_tab, w.pV, w.pP, s);
+    rc = run_pairs(d, w, s, which, M, c, x_dev + 4 * c0, 4, y_dev + 4 * c0, 4, out_dev + c0, LAYOUT_DOT64, w.pP, nullptr);
+    if (rc) return rc;
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
+  return 0;
+}
+
+/* one pair on host buffers through a staging stream (the reference's per-pair routines) */
+struct PertStage {
+  double *b = nullptr;
+  hipStream_t s = nullptr;
+  ~PertStage()
+  {
+    (void)hipFree(b);
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+static thread_local PertStage g_pstage;
+
+static int pert_one(const struct QED_kernel_temps *t, bool antisym, const double *x, const double *y, double *out)
+{
+  if (!t || !t->dev) { set_err("lepton loop: tables not initialised"); return -1; }
+  PertStage &st = g_pstage;
+  if (!st.s && hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess) { set_err("stream creation failed"); return -2; }
+  if (!st.b && hipMalloc(&st.b, 8 * (8 + 1280 + 384)) != hipSuccess) { set_err("staging allocation failed"); return -2; }
+  double *xd = st.b, *yd = st.b + 4, *o = st.b + 8;
+  if (hipMemcpyAsync(xd, x, 32, hipMemcpyHostToDevice, st.s) != hipSuccess ||
+      hipMemcpyAsync(yd, y, 32, hipMemcpyHostToDevice, st.s) != hipSuccess) {
+    set_err("host to device copy failed");
+    return -3;
+  }
+  int rc;
+  size_t len;
+  if (antisym) {
+    rc = kqed_pihat(t, 1, xd, yd, o, st.s);
+    len = 384;
+  } else {
+    kqed_device *d = t->dev;
+    std::lock_guard<std::mutex> lk(d->mu);
+    rc = ensure_pert_tab(d, t, st.s);
+    if (!rc) kqed_pert_launch_pihat1(1, xd, yd, d->pert_tab, o, st.s);
+    len = 1280;
+  }
+  if (rc) return rc;
+  if (hipMemcpyAsync(out, o, 8 * len, hipMemcpyDeviceToHost, st.s) != hipSuccess || hipStreamSynchronize(st.s) != hipSuccess) {
+    set_err("device to host copy failed");
+    return -3;
+  }
+  return 0;
+}
+
+extern "C" int kqed_pihat_host(const struct QED_kernel_temps *t, size_t n, const double *x, const double *y, double *out)
+{
+  for (size_t i = 0; i < n; i++) {
+    const int rc = pert_one(t, true, x + 4 * i, y + 4 * i, out + 384 * i);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+extern "C" void ipihatFermLoop_antisym(const double x[4], const double y[4], const struct QED_kernel_temps t,
+                                       double vpihat[6][4][4][4])
+{
+  if (pert_one(&t, true, x, y, &vpihat[0][0][0][0])) {
+    fprintf(stderr, "[KQED] %s\n", g_err);
+    memset(vpihat, 0, 384 * sizeof(double));
+  }
+}
+
+extern "C" void pihat1(const double x[4], const double y[4], const struct QED_kernel_temps t,
+                       double vpihat1[4][4][4][4][4], double vpihatr1[4][4][4][4])
+{
+  double o[1280];
+  if (pert_one(&t, false, x, y, o)) {
+    fprintf(stderr, "[KQED] %s\n", g_err);
+    memset(o, 0, sizeof o);
+  }
+  memcpy(vpihat1, o, 1024 * sizeof(double));
+  memcpy(vpihatr1, o + 1024, 256 * sizeof(double));
+}
+
+extern "C" void init_g8(struct QED_kernel_temps *t)
+{
+  if (t && t->G8) kqed_fill_g8(t->G8);
 }
 
 /* the reference's per-pair combination routines, one pair through kqed_combine_host */

@@ -23,6 +23,8 @@
 #include "kqed_assembly.h"
 
 /* 1/4 Tr(g_i0 ... g_i{n-1}) by expanding the last gamma against each earlier one */
+static int qtrace(int n, const int *idx);
+int kqed_quarter_trace(int n, const int *idx) { return n >= 0 && n <= 16 ? qtrace(n, idx) : 0; }
 static int qtrace(int n, const int *idx)
 {
   if (n & 1) return 0;

@@ -24,6 +24,8 @@ void kqed_dense_maps(double *Gfull);
 int kqed_build_maps(kqed_map *std_map, kqed_map *eq0_map, kqed_map *eq0_mulam_map);
 /* the reference's 65536-entry eight-gamma trace table */
 void kqed_fill_g8(int *G8);
+/* 1/4 Tr of n <= 16 Euclidean gammas with the given indices (0 beyond 16) */
+int kqed_quarter_trace(int n, const int *idx);
 
 #ifdef __cplusplus
 }

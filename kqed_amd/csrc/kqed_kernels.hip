@@ -1583,12 +1583,13 @@ DEV double wave_sum(double v)
  * combinations); the slot combination of L1..L3 / M-L2 and every output sum stay FP64.
  * Output modes: layout bit 0 selects out[384][ntot] (else out[q][384]), bit 1 FP32 outputs (else
  * FP64); layout 16 (contraction) writes the FP64 dot product out[q] = sum_c L_c Tc[q][c] with the
- * FP32 site tensor Tc.  With Wc (layout 16 and a single standard L0 slot) the dot takes the linear
+ * FP32 site tensor Tc, layout 64 the same with an FP64 tensor (Tc then points to doubles).  With Wc
+ * (layout 16 and a single standard L0 slot) the dot takes the linear
  * shortcut sum_c L_c T_c = sum_j u_j W_j, u the 200 tensor entries and W = A^T T per site. */
 template <typename MT>
 __global__ void __launch_bounds__(64 * KQED_ASM_WAVES)
 k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, void *__restrict__ outv,
-           int layout, int nslots, size_t ntot, const float *__restrict__ Tc, const double *__restrict__ Wc)
+           int layout, int nslots, size_t ntot, const void *__restrict__ Tc, const double *__restrict__ Wc)
 {
   typedef MT R_t;
   __shared__ unsigned sE[KQED_WM_MAX];
@@ -1715,11 +1716,16 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       wave_sync();
     }
-    if (layout == 16) {
+    if (layout == 16 || layout == 64) {
       if (!have_dot) {
-        const float *tq = Tc + q * 384;
         double part = 0.0;
-        for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
+        if (layout == 16) {
+          const float *tq = (const float *)Tc + q * 384;
+          for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
+        } else {
+          const double *tq = (const double *)Tc + q * 384;
+          for (int i = lane; i < 384; i += 64) part = fma(O[i], tq[i], part);
+        }
         dot = wave_sum(part);
       }
       if (lane == 0) ((double *)outv)[q] = dot;
@@ -1809,9 +1815,9 @@ __global__ void k_combine(int ny, const double *__restrict__ sigma, const double
 }
 
 template __global__ void k_assemble<double>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
-                                            const float *, const double *);
+                                            const void *, const double *);
 template __global__ void k_assemble<float>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
-                                           const float *, const double *);
+                                           const void *, const double *);
 
 /* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,

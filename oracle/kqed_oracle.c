@@ -1842,3 +1842,192 @@ void oq_combine_batch(const oq_tables *T, int comb, const double M[4], size_t n,
   for (size_t i = 0; i < n; i++) oq_combine(T, comb, M, x + 4 * i, y + 4 * i, out + len * i);
   (void)nthreads;
 }
+
+/* ------------------------------------------------------------------ perturbative lepton loop
+ * pi_pert.c: the free-fermion loop iPihat_{rho sigma; mu nu lambda}(x, y) that the reference's
+ * example1 contracts with L0 (amu_forlattice.c:66-126), and the analytic a_mu results it is
+ * checked against. */
+
+/* 1/4 Tr of gammas (pi_pert.c:136-157): expand the last index against each earlier one */
+int oq_diractrace(int n, const int *idx)
+{
+  if (n & 1) return 0;
+  if (n == 0) return 1;
+  int red[16], it = 0;
+  for (int i = 0; i < n - 1; i++) {
+    if (idx[i] != idx[n - 1]) continue;
+    int k = 0;
+    for (int j = 0; j < n - 1; j++)
+      if (j != i) red[k++] = idx[j];
+    it += (1 - 2 * (i & 1)) * oq_diractrace(n - 2, red);
+  }
+  return it;
+}
+
+static int g_g8[65536];
+static int g_g8_ready = 0;
+/* G8[i] = 4 * (1/4 Tr) of eight gammas, i = sum_k idx_k 4^(7-k) (pi_pert.c:161-177) */
+static void g8_init(void)
+{
+  if (g_g8_ready) return;
+#ifdef _OPENMP
+#pragma omp critical(oq_g8)
+#endif
+  {
+    if (!g_g8_ready) {
+      for (int i = 0; i < 65536; i++) {
+        int idx[8];
+        for (int k = 0; k < 8; k++) idx[k] = (i >> (2 * (7 - k))) & 3;
+        g_g8[i] = 4 * oq_diractrace(8, idx);
+      }
+      g_g8_ready = 1;
+    }
+  }
+}
+static inline int G8i(int a, int b, int c, int d, int e, int f, int g, int h)
+{ return g_g8[h + 4 * (g + 4 * (f + 4 * (e + 4 * (d + 4 * (c + 4 * (b + 4 * a))))))]; }
+
+/* K_0, K_1, ..., K_n at x (pi_pert.c:92-133): Abramowitz & Stegun 9.8.5-9.8.8 polynomial forms
+ * of K0 and K1, then the upward recurrence K_{j+1} = K_{j-1} + (2j/x) K_j */
+void oq_bessk(int n, double x, double *bv)
+{
+  double bkm, bk;
+  const double tox = 2.0 / x;
+  if (x <= 2.0) {
+    double y = x / 3.75;
+    y *= y;
+    const double i0 = 1.0 + y * (3.5156229 + y * (3.0899424 + y * (1.2067492 + y * (0.2659732 + y * (0.360768e-1 + y * 0.45813e-2)))));
+    const double i1 = x * (0.5 + y * (0.87890594 + y * (0.51498869 + y * (0.15084934 + y * (0.2658733e-1 + y * (0.301532e-2 + y * 0.32411e-3))))));
+    y = 0.25 * x * x;
+    const double l = log(x / 2.0);
+    bkm = (-l * i0) + (-0.57721566 + y * (0.42278420 + y * (0.23069756 + y * (0.3488590e-1 + y * (0.262698e-2 + y * (0.10750e-3 + y * 0.74e-5))))));
+    bk = (l * i1) + (1.0 / x) * (1.0 + y * (0.15443144 + y * (-0.67278579 + y * (-0.18156897 + y * (-0.1919402e-1 + y * (-0.110404e-2 + y * (-0.4686e-4)))))));
+  } else {
+    const double e = (exp(-x) / sqrt(x));
+    bkm = e * (1.25331414 + tox * (-0.7832358e-1 + tox * (0.2189568e-1 + tox * (-0.1062446e-1 + tox * (0.587872e-2 + tox * (-0.251540e-2 + tox * 0.53208e-3))))));
+    bk = e * (1.25331414 + tox * (0.23498619 + tox * (-0.3655620e-1 + tox * (0.1504268e-1 + tox * (-0.780353e-2 + tox * (0.325614e-2 + tox * (-0.68245e-3)))))));
+  }
+  bv[0] = bkm;
+  bv[1] = bk;
+  for (int j = 1; j < n; j++) {
+    const double bkp = bkm + j * tox * bk;
+    bkm = bk;
+    bv[j + 1] = bk = bkp;
+  }
+}
+
+/* pihat1 (pi_pert.c:222-381): v1[rho][mu][nu][lda][sig] (1024) and r1[mu][nu][lda][sig] (256) */
+void oq_pihat1(const double x[4], const double y[4], double *v1, double *r1)
+{
+  g8_init();
+  const double pipi = M_PI * M_PI, twopipi = 2 * pipi, norm = 8.2336243479292e-07;
+  double xmy[4];
+  for (int k = 0; k < 4; k++) xmy[k] = x[k] - y[k];
+  const double xn = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2] + x[3] * x[3]);
+  const double yn = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2] + y[3] * y[3]);
+  const double xmyn = sqrt(xmy[0] * xmy[0] + xmy[1] * xmy[1] + xmy[2] * xmy[2] + xmy[3] * xmy[3]);
+  double bx[3], by[3], bz[3];
+  oq_bessk(2, xn, bx); bx[1] /= xn; bx[2] /= (xn * xn);
+  oq_bessk(2, yn, by); by[1] /= yn; by[2] /= (yn * yn);
+  oq_bessk(2, xmyn, bz); bz[1] /= xmyn; bz[2] /= (xmyn * xmyn);
+  double e[4], f[4], g[4], q[4];
+  for (int k = 0; k < 4; k++) {
+    e[k] = twopipi * y[k] * by[2];
+    f[k] = pipi * y[k] * by[1];
+    g[k] = pipi * y[k] * by[0];
+    q[k] = 2 * f[k];
+  }
+  const double p = twopipi * by[0], s = twopipi * by[1];
+  double h[4][4], ell[4][4], fhat[4][4], fv[4][4][4];
+  for (int r = 0; r < 4; r++)
+    for (int d = 0; d < 4; d++) {
+      h[r][d] = r == d ? f[r] * y[d] - 0.5 * p : f[r] * y[d];
+      ell[r][d] = r == d ? e[r] * y[d] - s : e[r] * y[d];
+    }
+  for (int r = 0; r < 4; r++) {
+    for (int d = 0; d < 4; d++) {
+      fhat[d][r] = h[r][d];
+      for (int c = 0; c < 4; c++)
+        fv[c][d][r] = pipi * (by[2] * y[c] * y[d] * y[r] + ((r == d) * y[c] - (c == r) * y[d] - (c == d) * y[r]) * by[1]);
+    }
+    fhat[r][r] += p;
+  }
+  const double b11 = bx[1] * bz[1], b12 = bx[1] * bz[2], b21 = bx[2] * bz[1], b22 = bx[2] * bz[2];
+  double xab[4][4], x2[4], x3[4];
+  for (int a = 0; a < 4; a++) {
+    for (int b = 0; b < 4; b++) xab[a][b] = x[a] * xmy[b] * b22;
+    x2[a] = x[a] * b21;
+    x3[a] = xmy[a] * b12;
+  }
+  for (int i = 0; i < 256; i++) {
+    const int sg = i / 64, mu = (i / 16) & 3, nu = (i / 4) & 3, ld = i & 3;
+    double tr1 = 0.0, t1[4] = { 0, 0, 0, 0 };
+    for (int a = 0; a < 4; a++)
+      for (int b = 0; b < 4; b++) {
+        for (int c = 0; c < 4; c++)
+          for (int d = 0; d < 4; d++) {
+            const double T = xab[a][b] * G8i(a, mu, b, nu, c, sg, d, ld);
+            for (int r = 0; r < 4; r++) t1[r] -= fv[c][d][r] * T;
+            tr1 -= ell[c][d] * T;
+          }
+        const double m1 = G8i(0, 0, a, mu, b, nu, sg, ld) * xab[a][b];
+        const double m2 = G8i(0, 0, a, mu, nu, b, sg, ld) * x2[a] - G8i(0, 0, mu, a, nu, b, sg, ld) * x3[a];
+        const double m4 = G8i(0, 0, a, mu, nu, sg, b, ld) * x2[a] - G8i(0, 0, mu, a, nu, sg, b, ld) * x3[a];
+        const double m6 = G8i(0, 0, mu, nu, a, sg, b, ld) * b11;
+        for (int r = 0; r < 4; r++) t1[r] += -g[r] * m1 - h[r][b] * m2 - fhat[b][r] * m4 + fv[a][b][r] * m6;
+        tr1 -= p * m1;
+        tr1 -= q[b] * (m2 + m4);
+        tr1 += ell[a][b] * m6;
+      }
+    const double m = G8i(0, 0, 0, 0, mu, nu, sg, ld) * b11;
+    for (int r = 0; r < 4; r++) {
+      t1[r] += g[r] * m;
+      v1[256 * r + 64 * mu + 16 * nu + 4 * ld + sg] = norm * t1[r];
+    }
+    tr1 += p * m;
+    r1[64 * mu + 16 * nu + 4 * ld + sg] = norm * tr1;
+  }
+}
+
+/* ipihatFermLoop_antisym (pi_pert.c:180-220): out[rhosig][mu][nu][lda] (384) */
+void oq_ipihat(const double x[4], const double y[4], double *out)
+{
+  double mx[4], ymx[4];
+  for (int k = 0; k < 4; k++) { mx[k] = -x[k]; ymx[k] = y[k] - x[k]; }
+  double *v = (double *)malloc(sizeof(double) * 3 * 1280);
+  if (!v) return;
+  double *v1 = v, *r1 = v + 1024, *v2 = v + 1280, *r2 = v + 2304, *v3 = v + 2560, *r3 = v + 3584;
+  oq_pihat1(x, y, v1, r1);
+  oq_pihat1(ymx, mx, v2, r2);
+  oq_pihat1(mx, ymx, v3, r3);
+  (void)r1;
+#define V5(A, a, b, c, d, e) A[256 * (a) + 64 * (b) + 16 * (c) + 4 * (d) + (e)]
+#define R4(A, b, c, d, e) A[64 * (b) + 16 * (c) + 4 * (d) + (e)]
+  int rs = -1;
+  for (int rho = 0; rho < 4; rho++)
+    for (int sig = rho + 1; sig < 4; sig++) {
+      ++rs;
+      for (int mu = 0; mu < 4; mu++)
+        for (int nu = 0; nu < 4; nu++)
+          for (int ld = 0; ld < 4; ld++)
+            out[64 * rs + 16 * mu + 4 * nu + ld] =
+                0.5 * (+V5(v1, rho, mu, nu, ld, sig) + V5(v2, rho, nu, ld, mu, sig) + x[rho] * R4(r2, nu, ld, mu, sig) +
+                       V5(v3, rho, ld, nu, mu, sig) + x[rho] * R4(r3, ld, nu, mu, sig) -
+                       (+V5(v1, sig, mu, nu, ld, rho) + V5(v2, sig, nu, ld, mu, rho) + x[sig] * R4(r2, nu, ld, mu, rho) +
+                        V5(v3, sig, ld, nu, mu, rho) + x[sig] * R4(r3, ld, nu, mu, rho)));
+    }
+#undef V5
+#undef R4
+  free(v);
+}
+
+void oq_ipihat_batch(size_t n, const double *x, const double *y, double *out, int nthreads)
+{
+  g8_init();
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+  for (size_t i = 0; i < n; i++) oq_ipihat(x + 4 * i, y + 4 * i, out + 384 * i);
+  (void)nthreads;
+}

@@ -80,6 +80,15 @@ int oq_combine(const oq_tables *T, int comb, const double M[4], const double x[4
 void oq_combine_batch(const oq_tables *T, int comb, const double M[4], size_t n, const double *x, const double *y,
                       double *out, int nthreads);
 
+/* Perturbative lepton loop (pi_pert.c).  x and y are in units of the loop lepton's inverse mass
+ * (the reference's example1 passes M_v x, M_v y).  Singular where x, y or x-y vanishes. */
+int oq_diractrace(int n, const int *idx);                     /* pi_pert.c:136-157 */
+void oq_bessk(int n, double x, double *bv);                   /* pi_pert.c:92-133: K_0..K_n */
+void oq_pihat1(const double x[4], const double y[4], double *v1 /* [4][4][4][4][4] */,
+               double *r1 /* [4][4][4][4] */);                  /* pi_pert.c:222-381 */
+void oq_ipihat(const double x[4], const double y[4], double *out /* [6][4][4][4] */); /* :180-220 */
+void oq_ipihat_batch(size_t n, const double *x, const double *y, double *out, int nthreads);
+
 /* Pieces exposed for unit tests. */
 double oq_chebU(int order, int nk, double x, const double *co);          /* cheby.c:13-91 */
 double oq_chebU_avx(int order, int nk, double x, const double *co, const oq_tables *T); /* simd.c:182-336 */
