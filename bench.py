@@ -107,6 +107,39 @@ def combination_leg(t, dev, stream, xd, yd, n):
     return res
 
 
+# FP64 flops of one pihat1 evaluation in the reference's loop order (pi_pert.c:320-377): per
+# output (256) and (alpha, beta) pair (16): 16 inner steps of 11 flops, then 47 for the six-gamma
+# terms; plus 12 per output at the end.  iPihat = 3 pihat1 + 14 per output of the antisymmetrised sum.
+PIHAT_FLOPS = 3 * 256 * (16 * (16 * 11 + 47) + 12) + 384 * 14
+
+
+def lepton_loop_leg(t, dev, stream, n):
+    """iPihat of the free-lepton loop (pi_pert.c) and the example1 integrand sum_c L0 . iPihat on n
+    pairs with |x|, |y| uniform in [0.05, 1], isotropic, Mv = 2."""
+    import torch
+    from pairs import iso
+    rng = np.random.default_rng(5)
+    x = iso(rng, n) * rng.uniform(0.05, 1.0, (n, 1))
+    y = iso(rng, n) * rng.uniform(0.05, 1.0, (n, 1))
+    xd, yd = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    P = torch.empty((n, 384), dtype=torch.float64, device=dev)
+    o = torch.empty(n, dtype=torch.float64, device=dev)
+    t.pihat_device(1024, xd.data_ptr(), yd.data_ptr(), P.data_ptr(), stream=stream)  # warm-up (tables, workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t.pihat_device(n, xd.data_ptr(), yd.data_ptr(), P.data_ptr(), stream=stream)
+    torch.cuda.synchronize()
+    dt_p = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    t.lepton_loop_device(0, n, xd.data_ptr(), yd.data_ptr(), o.data_ptr(), Mv=2.0, stream=stream)
+    torch.cuda.synchronize()
+    dt_l = time.perf_counter() - t0
+    del xd, yd, P, o
+    return {"workload": f"{n} pairs, |x|,|y| U[0.05,1], isotropic, FP64; integrand with L0 and Mv = 2",
+            "pihat_pairs_per_s": n / dt_p, "pihat_tflops": PIHAT_FLOPS * n / dt_p / 1e12,
+            "pihat_flops_per_pair": PIHAT_FLOPS, "integrand_pairs_per_s": n / dt_l}
+
+
 def ragged_leg(t, dev, stream, reps=3):
     """256 y points with lognormal x-sets (median 16384, sigma 1, clipped [1024, 262144]), L0, FP32
     output (FP32 assembly maps, FP64 interpolation and Taylor branch), autotuned, one graph launch."""
@@ -278,6 +311,7 @@ def main():
         extra["contraction"] = contraction_leg(t, dev, stream, a.lattice)
         extra["ragged"] = ragged_leg(t, dev, stream)
         extra["combinations"] = combination_leg(t, dev, stream, xd, yd, min(a.n, 1 << 19))
+        extra["lepton_loop"] = lepton_loop_leg(t, dev, stream, 1 << 18)
         if a.surrogate:
             del out
             extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
