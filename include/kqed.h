@@ -291,6 +291,23 @@ int kqed_combine(const struct QED_kernel_temps *t, int comb, const double *M, si
 int kqed_combine_host(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
                       const double *x, const double *y, double *out);
 
+/* Table ingestion checks (host only, no GPU).  kqed_tables_info reads the three table files of a
+ * directory with the same readers and checksum verification as initialise() and reports their
+ * shape; kqed_tables_rewrite writes them back in the reference's format (the writers of reference
+ * io.c:292-413, which write fixed names under ./PRECOMP) into dst_dir, for byte-level round trips.
+ * Return 0, or the reader's code: 1 missing file, 2 bad magic, 3 bad dimension, 4 checksum
+ * mismatch, 5 short read; 6 write error. */
+struct kqed_table_info {
+  int nstpx, nstpy, nffa;
+  int nfmin, nfmax;          /* Chebyshev coefficients per x node (nfx) */
+  double x_first, x_last;    /* XX[0], XX[nstpx-1] */
+  double y_first, y_last;    /* YY[0], YY[nstpy-1] */
+  int x_increasing, y_increasing;
+  size_t nonfinite;          /* non-finite form-factor coefficients */
+};
+int kqed_tables_info(const char *dir, struct kqed_table_info *info);
+int kqed_tables_rewrite(const char *src_dir, const char *dst_dir);
+
 /* Lepton loop, batched (device pointers, enqueued on `stream`):
  *   kqed_pihat:       out[n][384] = iPihat(x_i, y_i)  (ipihatFermLoop_antisym per pair)
  *   kqed_lepton_loop: out[n] = sum_c L(x_i, y_i)_c iPihat(Mv x_i, Mv y_i)_c, the integrand of the

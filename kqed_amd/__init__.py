@@ -145,6 +145,8 @@ def lib():
 TABLES = {
     "full": dict(nstpx=100, nstpy=810, nf0=32, nf1=64, xh=6.54 / 810, yh=6.54 / 810, seed=20240611),
     "small": dict(nstpx=100, nstpy=218, nf0=8, nf1=16, xh=6.54 / 810, yh=0.03, seed=20240611),
+    # the longest Chebyshev rows the reference's GPU port admits (nk = nfx + 2 <= 128)
+    "wide": dict(nstpx=100, nstpy=64, nf0=96, nf1=127, xh=6.54 / 810, yh=0.1, seed=20240611),
 }
 
 

@@ -9,6 +9,7 @@
  *   TY : int magic, int 14, 14 x {int 100, double[100]}, u32 ck[2]
  */
 #include "kqed_tables.h"
+#include "../../include/kqed.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -372,5 +373,65 @@ int kqed_synth_ff(const char *path, const kqed_synth_params *p)
   }
   const int rc = write_ff_rows(path, p->nstpx, XX, p->nstpy, YY, nfx, synth_row, &s);
   free(XX); free(YY); free(nfx); free(s.w);
+  return rc;
+}
+
+/* ------------------------------------------------------------------ ingestion checks (public) */
+
+static void join(char *out, size_t n, const char *dir, const char *name) { snprintf(out, n, "%s/%s", dir, name); }
+
+static int read_all(const char *dir, kqed_host_tables *t)
+{
+  char p[4096];
+  memset(t, 0, sizeof *t);
+  join(p, sizeof p, dir, "FFxy_single_cksum.bin");
+  int rc = kqed_read_ff(p, t);
+  if (!rc) { join(p, sizeof p, dir, "taylorx_cksum.bin"); rc = kqed_read_taylorx(p, t); }
+  if (!rc) { join(p, sizeof p, dir, "taylory_cksum.bin"); rc = kqed_read_taylory(p, t); }
+  return rc;
+}
+
+int kqed_tables_info(const char *dir, struct kqed_table_info *info)
+{
+  kqed_host_tables t;
+  const int rc = read_all(dir, &t);
+  if (!rc && info) {
+    memset(info, 0, sizeof *info);
+    info->nstpx = t.nstpx;
+    info->nstpy = t.nstpy;
+    info->nffa = KQED_NFFA;
+    info->nfmax = t.nfmax;
+    info->nfmin = t.nfmax;
+    for (int i = 0; i < t.nstpx; i++) if (t.nfx[i] < info->nfmin) info->nfmin = t.nfx[i];
+    info->x_first = t.XX[0];
+    info->x_last = t.XX[t.nstpx - 1];
+    info->y_first = t.YY[0];
+    info->y_last = t.YY[t.nstpy - 1];
+    info->x_increasing = info->y_increasing = 1;
+    for (int i = 1; i < t.nstpx; i++) if (!(t.XX[i] > t.XX[i - 1])) info->x_increasing = 0;
+    for (int i = 1; i < t.nstpy; i++) if (!(t.YY[i] > t.YY[i - 1])) info->y_increasing = 0;
+    size_t bad = 0;
+    const size_t nv = (size_t)KQED_NFFA * t.nstpx * t.nstpy * t.nfmax;
+    for (size_t i = 0; i < nv; i++) bad += !isfinite(t.Ffm[i]) || !isfinite(t.Ffp[i]);
+    info->nonfinite = bad;
+  }
+  kqed_free_host_tables(&t);
+  return rc;
+}
+
+int kqed_tables_rewrite(const char *src_dir, const char *dst_dir)
+{
+  kqed_host_tables t;
+  int rc = read_all(src_dir, &t);
+  if (!rc) {
+    char p[4096];
+    join(p, sizeof p, dst_dir, "FFxy_single_cksum.bin");
+    rc = kqed_write_ff(p, &t) ? 6 : 0;
+    join(p, sizeof p, dst_dir, "taylorx_cksum.bin");
+    if (!rc) rc = kqed_write_taylor(p, &t.TX[0][0], KQED_NTAYX_ROWS, KQED_NTAYX_COLS) ? 6 : 0;
+    join(p, sizeof p, dst_dir, "taylory_cksum.bin");
+    if (!rc) rc = kqed_write_taylor(p, &t.TY[0][0], KQED_NTAYY_ROWS, KQED_NTAYY_COLS) ? 6 : 0;
+  }
+  kqed_free_host_tables(&t);
   return rc;
 }

@@ -104,3 +104,49 @@ def test_deterministic_and_order_independent(gpu_full):
     perm = np.random.default_rng(5).permutation(len(x))
     c = gpu_full.batch_host(0, x[perm], y[perm])
     assert np.array_equal(c, a[perm])
+
+
+@pytest.mark.gpu
+def test_reg_fixtures_on_the_real_table():
+    """The reference's golden outputs (PRECOMP/REGexample*.txt, all columns) through the GPU path,
+    when the real form-factor table is supplied (KQED_REAL_TABLE_DIR); validated with
+    kqed_tables_info first (shape, monotone grids, finite coefficients)."""
+    import json
+    import os
+    real = os.environ.get("KQED_REAL_TABLE_DIR")
+    if not real or not os.path.exists(os.path.join(real, "FFxy_single_cksum.bin")):
+        pytest.skip("needs the reference's form-factor table FFxy_single_cksum.bin (set KQED_REAL_TABLE_DIR); "
+                    "not available here")
+    from test_tables import table_info
+    rc, info = table_info(real)
+    assert rc == 0 and info.x_increasing and info.y_increasing and info.nonfinite == 0
+    cases = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reg_examples.json")))["cases"]
+    t = kqed_amd.QEDKernelTemps.initialise(real)
+    try:
+        for c in cases:
+            got = t.QED_kernel_L0(c["x"], c["y"]).reshape(384)
+            exp = np.array(c["expected"])
+            bad = np.abs(got - exp) > 5.01e-6 * np.abs(exp) + 1e-300
+            assert not bad.any(), (c["example"], c["column"], np.flatnonzero(bad)[:5])
+    finally:
+        t.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", [0, 2])
+def test_longest_chebyshev_rows(which):
+    """A synthetic table with nfx up to 126 (Clenshaw lengths up to 128, the limit of the reference's
+    GPU port), as a real table may have: every regime against the oracle."""
+    from oracle_ctypes import Oracle
+    d = kqed_amd.table_dir("wide")
+    o = Oracle(d)
+    t = kqed_amd.QEDKernelTemps.initialise(d)
+    try:
+        x, y, _ = regime_mix(17, n_each=24)
+        print(f"wide table which={which} n={len(x)}", flush=True)
+        got = t.batch_host(which, x, y)
+        ref = o.batch(which, x, y)
+        assert normwise(got, ref).max() < 1e-12
+    finally:
+        t.free()
+        o.close()

@@ -37,11 +37,28 @@ def test_reg_fixture_taylor_columns(oracle_small, case):
     assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], exp[bad][:5])
 
 
+REAL = os.environ.get("KQED_REAL_TABLE_DIR")  # directory holding the reference's real three table files
+NO_REAL = "needs the reference's form-factor table FFxy_single_cksum.bin (set KQED_REAL_TABLE_DIR); it is " \
+          "not available here, so these columns stay parity unpinned and the synthetic table is used instead"
+
+
+@pytest.fixture(scope="module")
+def oracle_real():
+    if not REAL or not os.path.exists(os.path.join(REAL, "FFxy_single_cksum.bin")):
+        pytest.skip(NO_REAL)
+    o = Oracle(REAL)
+    yield o
+    o.close()
+
+
 @pytest.mark.parametrize("case", [c for c in REG if not _taylor_only(c)],
                          ids=lambda c: f"ex{c['example']}b{c['block']}c{c['column']}")
-def test_reg_fixture_needs_real_table(oracle_small, case):
-    pytest.skip("needs the reference's form-factor table FFxy_single_cksum.bin, which is not available "
-                "(parity unpinned against these columns; the synthetic table is used instead)")
+def test_reg_fixture_needs_real_table(oracle_real, case):
+    """REGexample4/5/6/7 and the standard-regime columns of 2 and 10, on the real table."""
+    got = oracle_real.L0(case["x"], case["y"])
+    exp = np.array(case["expected"])
+    bad = np.abs(got - exp) > 5.01e-6 * np.abs(exp) + 1e-300
+    assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], exp[bad][:5])
 
 
 def test_shipped_taylor_checksums(small_dir):

@@ -96,3 +96,45 @@ def test_missing_taylor_file(small_dir, tmp_path):
     L = _L()
     h = HostTables()
     assert L.kqed_read_taylorx(str(tmp_path / "nope.bin").encode(), ctypes.byref(h)) == 1
+
+
+class TableInfo(ctypes.Structure):
+    _fields_ = [("nstpx", ctypes.c_int), ("nstpy", ctypes.c_int), ("nffa", ctypes.c_int), ("nfmin", ctypes.c_int),
+                ("nfmax", ctypes.c_int), ("x_first", ctypes.c_double), ("x_last", ctypes.c_double),
+                ("y_first", ctypes.c_double), ("y_last", ctypes.c_double), ("x_increasing", ctypes.c_int),
+                ("y_increasing", ctypes.c_int), ("nonfinite", ctypes.c_size_t)]
+
+
+def table_info(d):
+    L = kqed_amd.lib()
+    L.kqed_tables_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(TableInfo)]
+    info = TableInfo()
+    rc = L.kqed_tables_info(d.encode(), ctypes.byref(info))
+    return rc, info
+
+
+def test_public_info_and_directory_round_trip(small_dir, tmp_path):
+    """kqed_tables_info reports the shape; kqed_tables_rewrite reproduces every file byte for byte
+    (the shipped Taylor files are the reference's own output)."""
+    rc, info = table_info(small_dir)
+    p = kqed_amd.TABLES["small"]
+    assert rc == 0
+    assert (info.nstpx, info.nstpy, info.nffa, info.nfmin, info.nfmax) == (p["nstpx"], p["nstpy"], 14, p["nf0"],
+                                                                             max(p["nf0"], p["nf1"] - 1))
+    assert info.x_increasing == 1 and info.y_increasing == 1 and info.nonfinite == 0
+    assert info.x_first > 0 and info.x_last > info.x_first and info.y_last == pytest.approx(p["yh"] * p["nstpy"])
+    L = kqed_amd.lib()
+    L.kqed_tables_rewrite.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    assert L.kqed_tables_rewrite(small_dir.encode(), str(tmp_path).encode()) == 0
+    for f in ("FFxy_single_cksum.bin", "taylorx_cksum.bin", "taylory_cksum.bin"):
+        assert (tmp_path / f).read_bytes() == open(os.path.join(small_dir, f), "rb").read(), f
+
+
+def test_public_info_rejects_a_damaged_table(small_dir, tmp_path):
+    for f in ("FFxy_single_cksum.bin", "taylorx_cksum.bin", "taylory_cksum.bin"):
+        shutil.copyfile(os.path.join(small_dir, f), tmp_path / f)
+    b = bytearray((tmp_path / "taylorx_cksum.bin").read_bytes())
+    b[1000] ^= 0x10
+    (tmp_path / "taylorx_cksum.bin").write_bytes(bytes(b))
+    rc, _ = table_info(str(tmp_path))
+    assert rc == 4
