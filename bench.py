@@ -84,6 +84,29 @@ def contraction_leg(t, dev, stream, L):
             "s_first": float(s[0])}
 
 
+def lattice_comb_leg(t, dev, stream, L, ny=4):
+    """Per-y contraction of a lattice through the x<->y symmetrised kernels (SYMXY.c, all four L^k):
+    s_k(y) = sum_x L^k_sym(x,y) . T(x), L^4 lattice, the first ny of the 16 fixed y sites."""
+    import torch
+    import kqed_amd
+    from kqed_amd import workloads as W
+    x = W.lattice_sites(L)
+    T = W.site_tensors(x, seed=3)
+    y = W.y_sites()[:ny]
+    xd, Td = torch.from_numpy(x).to(dev), torch.from_numpy(T).to(dev)
+    del T
+    comb = kqed_amd.KQED_COMB_SYMXY_V2
+    t.contract_comb_device(comb, len(x), xd.data_ptr(), Td.data_ptr(), y[:1], stream=stream)  # warm-up (workspace)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s = t.contract_comb_device(comb, len(x), xd.data_ptr(), Td.data_ptr(), y, stream=stream)
+    dt = time.perf_counter() - t0
+    del xd, Td
+    return {"workload": f"sum_x L^k_SYMXY(x,y).T(x), k = 0..3, {L}^4 lattice, {ny} y sites, FP32 T, FP64 sums",
+            "pairs": len(x) * ny, "seconds": dt, "pairs_per_s": len(x) * ny / dt,
+            "l0_evals_per_s": 6 * len(x) * ny / dt, "s_first": [float(v) for v in s[0]]}
+
+
 def combination_leg(t, dev, stream, xd, yd, n):
     """The reference's kernel combinations (all_kernels.c, con_kernel.c, SYMXY0.c) on the first n
     pairs of the workload: combinations/s and the L0 evaluations/s inside them."""
@@ -312,6 +335,7 @@ def main():
         extra["ragged"] = ragged_leg(t, dev, stream)
         extra["combinations"] = combination_leg(t, dev, stream, xd, yd, min(a.n, 1 << 19))
         extra["lepton_loop"] = lepton_loop_leg(t, dev, stream, 1 << 18)
+        extra["lattice_symxy"] = lattice_comb_leg(t, dev, stream, a.lattice)
         if a.surrogate:
             del out
             extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)

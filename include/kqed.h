@@ -287,6 +287,13 @@ enum kqed_comb {
 int kqed_comb_kernels(int comb);
 int kqed_combine(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
                  const double *x_dev, const double *y_dev, double *out_dev, void *stream);
+/* Contraction through a combination, for a lattice driver: s[j * nk + q] = sum over the nx sites
+ * of sum_c K_q(x_i, y_j)_c T_i[c] for every output kernel q (nk = kqed_comb_kernels(comb)) and y
+ * point j; x_dev [nx][4] FP64 and T_dev [nx][384] FP32 on the device, y_host [ny][4] and s_host on
+ * the host.  FP64 sums in a fixed order (deterministic).  Synchronous on `stream`. */
+int kqed_contract_comb(const struct QED_kernel_temps *t, int comb, const double *M, size_t nx,
+                       const double *x_dev, const float *T_dev, size_t ny, const double *y_host,
+                       double *s_host, void *stream);
 /* Same on host buffers (copies in and out, synchronous). */
 int kqed_combine_host(const struct QED_kernel_temps *t, int comb, const double *M, size_t n,
                       const double *x, const double *y, double *out);

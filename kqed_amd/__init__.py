@@ -108,6 +108,7 @@ def lib():
     L.kqed_comb_kernels.argtypes = [ctypes.c_int]
     L.kqed_combine.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, vp, vp]
     L.kqed_combine_host.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, arr, arr, vp]
+    L.kqed_contract_comb.argtypes = [P, ctypes.c_int, vp, ctypes.c_size_t, vp, vp, ctypes.c_size_t, arr, arr, vp]
     L.kqed_pihat.argtypes = [P, ctypes.c_size_t, vp, vp, vp, vp]
     L.kqed_pihat_host.argtypes = [P, ctypes.c_size_t, arr, arr, arr]
     L.kqed_lepton_loop.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_size_t, vp, vp, vp,
@@ -286,6 +287,20 @@ class QEDKernelTemps:
                                 ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError("kqed_combine failed: " + lib().kqed_last_error().decode())
+
+    def contract_comb_device(self, comb, nx, x_ptr, T_ptr, y, M=None, stream=None):
+        """s[j, q] = sum_i K_q(x_i, y_j) . T_i for the output kernels of a combination; x[nx][4] FP64 and
+        T[nx][384] FP32 on the device (pointers), y on the host."""
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 4)
+        nk = lib().kqed_comb_kernels(int(comb))
+        s = np.zeros((y.shape[0], nk))
+        M4 = None if M is None else np.ascontiguousarray(M, np.float64).reshape(4)
+        rc = lib().kqed_contract_comb(ctypes.byref(self._t), int(comb), None if M4 is None else M4.ctypes.data, int(nx),
+                                      ctypes.c_void_p(x_ptr), ctypes.c_void_p(T_ptr), y.shape[0], y.reshape(-1),
+                                      s.reshape(-1), ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_contract_comb failed: " + lib().kqed_last_error().decode())
+        return s
 
     def pihat(self, x, y):
         """ipihatFermLoop_antisym for n pairs of host arrays -> (n, 384) float64 (computed on the GPU)."""

@@ -119,6 +119,9 @@ struct Workspace {
   /* kernel combinations: derived argument pairs [2][ccap][4] and their L0 tensors [ccap][384] */
   size_t ccap = 0;
   double *cargs = nullptr, *cL = nullptr;
+  /* combination contraction: per-pair dots [NK][nx], y points, partial and final sums */
+  size_t cdcap = 0;
+  double *cdots = nullptr;
   /* lepton loop: pihat1 evaluations [3 pcap][1280], iPihat [pcap][384], scaled pairs [2][pcap][4] */
   size_t pcap = 0;
   double *pV = nullptr, *pP = nullptr, *pX = nullptr;
@@ -201,7 +204,7 @@ static void free_ws(Workspace &w)
 {
   (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
-  (void)hipFree(w.cargs); (void)hipFree(w.cL);
+  (void)hipFree(w.cargs); (void)hipFree(w.cL); (void)hipFree(w.cdots);
   (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
   w = Workspace();
 }
@@ -955,13 +958,64 @@ extern "C" int kqed_combine(const struct QED_kernel_temps *t, int comb, const do
   for (size_t c0 = 0; c0 < n; c0 += cp) {
     const size_t c = n - c0 < cp ? n - c0 : cp;
     double *X = w.cargs, *Y = w.cargs + 4 * (size_t)E * c;
-    kqed_comb_launch_args(comb, c, x_dev + 4 * c0, y_dev + 4 * c0, X, Y, s);
+    kqed_comb_launch_args(comb, c, x_dev + 4 * c0, y_dev + 4 * c0, 4, X, Y, s);
     const int rc = run_pairs(d, w, s, KQED_L0, 0.0, (size_t)E * c, X, 4, Y, 4, w.cL, KQED_AOS, nullptr, nullptr);
     if (rc) return rc;
-    kqed_comb_launch(comb, c, w.cL, x_dev + 4 * c0, y_dev + 4 * c0, M, out_dev + len * c0, s);
+    kqed_comb_launch(comb, c, w.cL, x_dev + 4 * c0, y_dev + 4 * c0, 4, M, out_dev + len * c0, nullptr, 0, s);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
+  return 0;
+}
+
+/* s[j][q] = sum_x sum_c K_q(x, y_j)_c T(x)_c for every output kernel q of a combination: per y and
+ * pass, derived arguments, the L0 pipeline, then the combination kernel in its dot mode (the outputs
+ * never leave LDS); per-pair dots summed over x in a fixed order. */
+extern "C" int kqed_contract_comb(const struct QED_kernel_temps *t, int comb, const double *M, size_t nx,
+                                  const double *x_dev, const float *T_dev, size_t ny, const double *y_host,
+                                  double *s_host, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_contract_comb: tables not initialised"); return -1; }
+  const int E = kqed_comb_evals(comb), NK = kqed_comb_kernels(comb);
+  if (!E) { set_err("kqed_contract_comb: unknown combination %d", comb); return -1; }
+  if (ny == 0) return 0;
+  if (nx > 0xFFFFFFFFull || ny > 65536) { set_err("kqed_contract_comb: at most 2^32-1 sites and 65536 y points"); return -1; }
+  if (nx == 0) {
+    memset(s_host, 0, 8 * (size_t)NK * ny);
+    return 0;
+  }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  Workspace &w = d->ws[stream];
+  hipStream_t s = (hipStream_t)stream;
+  size_t cp = d->tune.chunk / (size_t)E;
+  if (cp < 1) cp = 1;
+  if (cp > nx) cp = nx;
+  if (ensure_comb_ws(w, cp * E)) return -2;
+  const size_t need = (size_t)NK * nx + (size_t)NK * ny + 4 * ny + RED_BLOCKS;
+  if (w.cdcap < need) {
+    (void)hipFree(w.cdots); w.cdots = nullptr; w.cdcap = 0;
+    if (hipMalloc(&w.cdots, 8 * need) != hipSuccess) { set_err("contraction workspace allocation failed"); return -2; }
+    w.cdcap = need;
+  }
+  double *dots = w.cdots, *sd = dots + (size_t)NK * nx, *yd = sd + (size_t)NK * ny, *tmp = yd + 4 * ny;
+  if (hipMemcpyAsync(yd, y_host, 32 * ny, hipMemcpyHostToDevice, s) != hipSuccess) { set_err("y upload failed"); return -3; }
+  for (size_t j = 0; j < ny; j++) {
+    for (size_t c0 = 0; c0 < nx; c0 += cp) {
+      const size_t c = nx - c0 < cp ? nx - c0 : cp;
+      double *X = w.cargs, *Y = w.cargs + 4 * (size_t)E * c;
+      kqed_comb_launch_args(comb, c, x_dev + 4 * c0, yd + 4 * j, 0, X, Y, s);
+      const int rc = run_pairs(d, w, s, KQED_L0, 0.0, (size_t)E * c, X, 4, Y, 4, w.cL, KQED_AOS, nullptr, nullptr);
+      if (rc) return rc;
+      kqed_comb_launch(comb, c, w.cL, x_dev + 4 * c0, yd + 4 * j, 0, M, dots + c0, T_dev + 384 * c0, nx, s);
+    }
+    for (int q = 0; q < NK; q++) dev_sum(s, nx, dots + (size_t)q * nx, tmp, sd + j * NK + q);
+  }
+  if (hipMemcpyAsync(s_host, sd, 8 * (size_t)NK * ny, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_err("kqed_contract_comb: %s", hipGetErrorString(hipGetLastError()));
+    return -3;
+  }
   return 0;
 }
 

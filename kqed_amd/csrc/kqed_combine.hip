@@ -44,7 +44,7 @@ __host__ __device__ constexpr int popc(unsigned v) { return v ? (int)(v & 1u) + 
 /* compact slot of derived pair e within a combination's evaluations */
 __host__ __device__ constexpr int slot(int comb, int e) { return popc(comb_mask(comb) & ((1u << e) - 1u)); }
 
-__global__ void k_comb_args(size_t c, const double *__restrict__ x, const double *__restrict__ y, unsigned mask,
+__global__ void k_comb_args(size_t c, const double *__restrict__ x, const double *__restrict__ y, int ys, unsigned mask,
                             double *__restrict__ X, double *__restrict__ Y)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,7 +52,7 @@ __global__ void k_comb_args(size_t c, const double *__restrict__ x, const double
   double xv[4], yv[4], xmy[4], ymx[4], my[4];
   for (int k = 0; k < 4; k++) {
     xv[k] = x[4 * i + k];
-    yv[k] = y[4 * i + k];
+    yv[k] = y[(size_t)ys * i + k];
     xmy[k] = xv[k] - yv[k];
     ymx[k] = -xmy[k];
     my[k] = -yv[k];
@@ -142,9 +142,12 @@ struct CombArgs {
   double M[4];
 };
 
-template <int COMB>
+/* DOT: instead of storing the outputs, dots[q * dstride + p] = sum_c K_q(x_p, y_p)_c Tc[p][c] (FP64,
+ * fixed-order block reduction).  ys = 0 broadcasts one y to every pair. */
+template <int COMB, bool DOT>
 __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restrict__ L, const double *__restrict__ x,
-                                                 const double *__restrict__ y, CombArgs A, double *__restrict__ out)
+                                                 const double *__restrict__ y, int ys, CombArgs A, double *__restrict__ out,
+                                                 const float *__restrict__ Tc, size_t dstride)
 {
   constexpr int E = popc(comb_mask(COMB));
   constexpr int NK = comb_nk(COMB);
@@ -156,7 +159,7 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
   double xv[4], yv[4], xmy[4];
   for (int k = 0; k < 4; k++) {
     xv[k] = x[4 * p + k];
-    yv[k] = y[4 * p + k];
+    yv[k] = y[(size_t)ys * p + k];
     xmy[k] = xv[k] - yv[k];
   }
   if (t < 12) {
@@ -167,13 +170,20 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
   __syncthreads();
   const Pair<COMB> P = { sh, t / 64 };
   const int m = (t / 16) & 3, n = (t / 4) & 3, l = t & 3;
-  double *o = out + p * (size_t)(NK * 384) + t;
   const double *M = A.M;
+  /* output q of this thread's (rhosig, mu, nu, lambda): stored, or multiplied by the site tensor */
+  double acc[NK];
+  double *o = DOT ? nullptr : out + p * (size_t)(NK * 384) + t;
+  const double tv = DOT ? (double)Tc[p * 384 + t] : 0.0;
+  auto put = [&](int q, double v) {
+    if constexpr (DOT) acc[q] = v * tv;
+    else o[384 * q] = v;
+  };
 
   if constexpr (COMB == KQED_COMB_ALL) {
     double k[8];
     all_at(P, m, n, l, k);
-    for (int q = 0; q < 8; q++) o[384 * q] = k[q];
+    for (int q = 0; q < 8; q++) put(q, k[q]);
   } else if constexpr (COMB == KQED_COMB_SYMXY || COMB == KQED_COMB_SYMXY_V2) {
     /* SYMXY.c:12-85: (L^k.xy + L^k.yx_{nu mu lambda}) / 2, and for SYMXY .yx = .xy_{nu mu lambda} */
     double a[8], b[8];
@@ -182,23 +192,23 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
     for (int q = 0; q < 4; q++) {
       const double xy = (a[2 * q] + b[2 * q + 1]) / 2;
       if constexpr (COMB == KQED_COMB_SYMXY_V2) {
-        o[384 * q] = xy;
+        put(q, xy);
       } else {
-        o[384 * (2 * q)] = xy;
-        o[384 * (2 * q + 1)] = (b[2 * q] + a[2 * q + 1]) / 2;
+        put((2 * q), xy);
+        put((2 * q + 1), (b[2 * q] + a[2 * q + 1]) / 2);
       }
     }
   } else if constexpr (COMB == KQED_COMB_SYMXY0 || COMB == KQED_COMB_SYMXY0_V2) {
     double a[4];
     symxy0_at(P, m, n, l, a);
     if constexpr (COMB == KQED_COMB_SYMXY0_V2) {
-      for (int q = 0; q < 4; q++) o[384 * q] = a[q];
+      for (int q = 0; q < 4; q++) put(q, a[q]);
     } else {
       double b[4];
       symxy0_at(P, n, m, l, b);
       for (int q = 0; q < 4; q++) {
-        o[384 * (2 * q)] = a[q];
-        o[384 * (2 * q + 1)] = b[q];
+        put((2 * q), a[q]);
+        put((2 * q + 1), b[q]);
       }
     }
   } else if constexpr (COMB == KQED_COMB_ALL_M || COMB == KQED_COMB_ALL_M_V2) {
@@ -210,14 +220,14 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
       if constexpr (COMB == KQED_COMB_ALL_M) {
         const double syx = eY * (P.at(CE_0X, m, n, l) - M[a] * yv[m] * P.dot1(yv, CE_0X, n, l)) +
                            eX * (P.at(CE_Y0, m, n, l) - M[a] * xv[n] * P.dot2(xv, CE_Y0, m, l));
-        o[384 * (2 * a)] = P.at(CE_XY, m, n, l) - sxy;
-        o[384 * (2 * a + 1)] = P.at(CE_YX, m, n, l) - syx;
+        put((2 * a), P.at(CE_XY, m, n, l) - sxy);
+        put((2 * a + 1), P.at(CE_YX, m, n, l) - syx);
       } else {
         const double syx2 = eY * (P.at(CE_0X, n, m, l) - M[a] * yv[n] * P.dot1(yv, CE_0X, m, l)) +
                             eX * (P.at(CE_Y0, n, m, l) - M[a] * xv[m] * P.dot2(xv, CE_Y0, n, l));
         const double sxmy = -eX * (P.at(CE_0Y, m, n, l) - M[a] * xv[m] * pxy1) + eY * (P.at(CE_X0, m, n, l) - M[a] * yv[n] * pxy2);
-        o[384 * (2 * a)] = P.at(CE_XY, m, n, l) - sxy + P.at(CE_YX, n, m, l) - syx2;
-        o[384 * (2 * a + 1)] = P.at(CE_X_MY, m, n, l) - sxmy;
+        put((2 * a), P.at(CE_XY, m, n, l) - sxy + P.at(CE_YX, n, m, l) - syx2);
+        put((2 * a + 1), P.at(CE_X_MY, m, n, l) - sxmy);
       }
     }
   } else if constexpr (COMB == KQED_COMB_CON_V2 || COMB == KQED_COMB_CON) {
@@ -235,20 +245,20 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
       const double B3 = B0 + (P.at(CE_X0, m, n, l) + P.at(CE_0X, l, n, m) - P.at(CE_0XMY, l, n, m));
       const double C3 = A3 - B3 + Yx + (P.at(CE_Y0, l, m, n) + P.at(CE_0Y, n, m, l) - P.at(CE_0X, n, m, l));
       const double v[12] = { A0, A1, A2, A3, B0, B1, B2, B3, C0, C1, C2, C3 };
-      for (int q = 0; q < 12; q++) o[384 * q] = v[q];
+      for (int q = 0; q < 12; q++) put(q, v[q]);
     } else {
       const double c2 = C0 - (P.at(CE_0Y, m, n, l) + P.at(CE_X0, m, n, l) + P.at(CE_0X, n, m, l) + P.at(CE_Y0, n, m, l) -
                               P.at(CE_0XMY, l, n, m) - P.at(CE_X0, l, n, m));
-      o[0] = C0;
-      o[384] = B0;
-      o[384 * 2] = C0 + (P.at(CE_X0, l, n, m) + P.at(CE_Y0, l, n, m) + P.at(CE_X0, l, m, n) + P.at(CE_Y0, l, m, n) -
-                         P.at(CE_X0, m, n, l) - P.at(CE_XMY0, m, n, l)) / 2;
-      o[384 * 3] = B0 + (P.at(CE_X0, m, n, l) + P.at(CE_XMY0, m, n, l)) / 2;
-      o[384 * 4] = c2;
-      o[384 * 5] = B0 - (P.at(CE_X0, l, n, m) + P.at(CE_0XMY, l, n, m));
-      o[384 * 6] = c2 + (P.at(CE_0X, m, n, l) + P.at(CE_0Y, n, m, l) + P.at(CE_Y0, l, m, n) - P.at(CE_0X, l, n, m) +
-                         P.at(CE_Y0, n, m, l));
-      o[384 * 7] = B0 + (P.at(CE_X0, m, n, l) + P.at(CE_0X, l, n, m) - P.at(CE_0XMY, l, n, m));
+      put(0, C0);
+      put(1, B0);
+      put(2, C0 + (P.at(CE_X0, l, n, m) + P.at(CE_Y0, l, n, m) + P.at(CE_X0, l, m, n) + P.at(CE_Y0, l, m, n) -
+                         P.at(CE_X0, m, n, l) - P.at(CE_XMY0, m, n, l)) / 2);
+      put(3, B0 + (P.at(CE_X0, m, n, l) + P.at(CE_XMY0, m, n, l)) / 2);
+      put(4, c2);
+      put(5, B0 - (P.at(CE_X0, l, n, m) + P.at(CE_0XMY, l, n, m)));
+      put(6, c2 + (P.at(CE_0X, m, n, l) + P.at(CE_0Y, n, m, l) + P.at(CE_Y0, l, m, n) - P.at(CE_0X, l, n, m) +
+                         P.at(CE_Y0, n, m, l)));
+      put(7, B0 + (P.at(CE_X0, m, n, l) + P.at(CE_0X, l, n, m) - P.at(CE_0XMY, l, n, m)));
     }
   } else if constexpr (COMB == KQED_COMB_CON_M_L2 || COMB == KQED_COMB_SUB_M_L2) {
     /* con_kernel.c:246-399 (B read as lambda,nu,mu) and sub_kernel.c:26-167 (nu,lambda,mu) */
@@ -266,19 +276,41 @@ __global__ void __launch_bounds__(384) k_combine(size_t c, const double *__restr
                          eX * (P.at(CE_Y0, n, m, l) - M[a] * xv[m] * pf4);
       const double lxm = P.at(CE_X_XMY, p1, p2, m) - eX * (P.at(CE_0XMY, p1, p2, m) - M[a] * f5 * pf5) -
                          eXMY * (P.at(CE_X0, p1, p2, m) - M[a] * f6 * pf6);
-      o[384 * a] = lxy;
-      o[384 * (4 + a)] = lxm;
-      o[384 * (8 + a)] = lxy + lyx - lxm;
+      put(a, lxy);
+      put((4 + a), lxm);
+      put((8 + a), lxy + lyx - lxm);
+    }
+  }
+  if constexpr (DOT) {
+    /* per output kernel: wavefront sums, then the six wavefront partials in order */
+    __shared__ double part[6][NK];
+    const int w = t >> 6, lane = t & 63;
+#pragma unroll
+    for (int q = 0; q < NK; q++) {
+      double v = acc[q];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) part[w][q] = v;
+    }
+    __syncthreads();
+    if (t < NK) {
+      double v = part[0][t];
+      for (int j = 1; j < 6; j++) v += part[j][t];
+      out[(size_t)t * dstride + p] = v;
     }
   }
 }
 
 template <int COMB>
-void launch_combine(size_t c, const double *L, const double *x, const double *y, const double *M, double *out, hipStream_t s)
+void launch_combine(size_t c, const double *L, const double *x, const double *y, int ys, const double *M, double *out,
+                    const float *Tc, size_t dstride, hipStream_t s)
 {
   CombArgs A;
   for (int a = 0; a < 4; a++) A.M[a] = M ? M[a] : 0.0;
-  hipLaunchKernelGGL(k_combine<COMB>, dim3((unsigned)c), dim3(384), 0, s, c, L, x, y, A, out);
+  if (Tc)
+    hipLaunchKernelGGL((k_combine<COMB, true>), dim3((unsigned)c), dim3(384), 0, s, c, L, x, y, ys, A, out, Tc, dstride);
+  else
+    hipLaunchKernelGGL((k_combine<COMB, false>), dim3((unsigned)c), dim3(384), 0, s, c, L, x, y, ys, A, out, Tc, dstride);
 }
 
 } // namespace
@@ -287,26 +319,26 @@ int kqed_comb_evals(int comb) { return comb >= 0 && comb < KQED_COMB_N ? popc(co
 
 extern "C" int kqed_comb_kernels(int comb) { return comb_nk(comb); }
 
-void kqed_comb_launch_args(int comb, size_t c, const double *x, const double *y, double *X, double *Y, hipStream_t s)
+void kqed_comb_launch_args(int comb, size_t c, const double *x, const double *y, int ys, double *X, double *Y, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_comb_args, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, s, c, x, y, comb_mask(comb), X, Y);
+  hipLaunchKernelGGL(k_comb_args, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, s, c, x, y, ys, comb_mask(comb), X, Y);
 }
 
-void kqed_comb_launch(int comb, size_t c, const double *L, const double *x, const double *y, const double *M, double *out,
-                      hipStream_t s)
+void kqed_comb_launch(int comb, size_t c, const double *L, const double *x, const double *y, int ys, const double *M,
+                      double *out, const float *Tc, size_t dstride, hipStream_t s)
 {
   switch (comb) {
-  case KQED_COMB_ALL: launch_combine<KQED_COMB_ALL>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_ALL_M: launch_combine<KQED_COMB_ALL_M>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_ALL_M_V2: launch_combine<KQED_COMB_ALL_M_V2>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_CON: launch_combine<KQED_COMB_CON>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_CON_V2: launch_combine<KQED_COMB_CON_V2>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_CON_M_L2: launch_combine<KQED_COMB_CON_M_L2>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_SUB_M_L2: launch_combine<KQED_COMB_SUB_M_L2>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_SYMXY: launch_combine<KQED_COMB_SYMXY>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_SYMXY_V2: launch_combine<KQED_COMB_SYMXY_V2>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_SYMXY0: launch_combine<KQED_COMB_SYMXY0>(c, L, x, y, M, out, s); break;
-  case KQED_COMB_SYMXY0_V2: launch_combine<KQED_COMB_SYMXY0_V2>(c, L, x, y, M, out, s); break;
+  case KQED_COMB_ALL: launch_combine<KQED_COMB_ALL>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_ALL_M: launch_combine<KQED_COMB_ALL_M>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_ALL_M_V2: launch_combine<KQED_COMB_ALL_M_V2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_CON: launch_combine<KQED_COMB_CON>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_CON_V2: launch_combine<KQED_COMB_CON_V2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_CON_M_L2: launch_combine<KQED_COMB_CON_M_L2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_SUB_M_L2: launch_combine<KQED_COMB_SUB_M_L2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_SYMXY: launch_combine<KQED_COMB_SYMXY>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_SYMXY_V2: launch_combine<KQED_COMB_SYMXY_V2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_SYMXY0: launch_combine<KQED_COMB_SYMXY0>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
+  case KQED_COMB_SYMXY0_V2: launch_combine<KQED_COMB_SYMXY0_V2>(c, L, x, y, ys, M, out, Tc, dstride, s); break;
   default: break;
   }
 }

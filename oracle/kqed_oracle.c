@@ -2031,3 +2031,45 @@ void oq_ipihat_batch(size_t n, const double *x, const double *y, double *out, in
   for (size_t i = 0; i < n; i++) oq_ipihat(x + 4 * i, y + 4 * i, out + 384 * i);
   (void)nthreads;
 }
+
+/* Contraction through a combination (the pattern of amu_forlattice.c:99-107 per output kernel):
+ * s[j*nk + q] = sum_i sum_c K_q(x_i, y_j)_c Tc[i][c], a[j*nk + q] = sum_i sum_c |...| */
+void oq_contract_comb(const oq_tables *T, int comb, const double M[4], size_t nx, const double *x, const float *Tc,
+                      size_t ny, const double *y, double *s, double *a, int nthreads)
+{
+  maps_init();
+  const int nk = oq_comb_len(comb) / 384;
+  if (!nk) return;
+  double *dot = (double *)malloc(2 * nk * nx * sizeof(double));
+  if (!dot) return;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#endif
+  for (size_t j = 0; j < ny; j++) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+#endif
+    for (size_t i = 0; i < nx; i++) {
+      double K[12 * 384];
+      oq_combine(T, comb, M, x + 4 * i, y + 4 * j, K);
+      for (int q = 0; q < nk; q++) {
+        double t = 0.0, u = 0.0;
+        for (int c = 0; c < 384; c++) {
+          const double v = K[384 * q + c] * (double)Tc[384 * i + c];
+          t += v;
+          u += fabs(v);
+        }
+        dot[2 * (nk * i + q)] = t;
+        dot[2 * (nk * i + q) + 1] = u;
+      }
+    }
+    for (int q = 0; q < nk; q++) {
+      double st = 0.0, sa = 0.0;
+      for (size_t i = 0; i < nx; i++) { st += dot[2 * (nk * i + q)]; sa += dot[2 * (nk * i + q) + 1]; }
+      s[j * nk + q] = st;
+      if (a) a[j * nk + q] = sa;
+    }
+  }
+  free(dot);
+  (void)nthreads;
+}

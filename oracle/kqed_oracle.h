@@ -79,6 +79,9 @@ int oq_comb_len(int comb); /* doubles per pair */
 int oq_combine(const oq_tables *T, int comb, const double M[4], const double x[4], const double y[4], double *K);
 void oq_combine_batch(const oq_tables *T, int comb, const double M[4], size_t n, const double *x, const double *y,
                       double *out, int nthreads);
+/* s[j*nk + q] = sum_i K_q(x_i, y_j) . Tc[i] for the nk output kernels of comb; a: sum of |terms| */
+void oq_contract_comb(const oq_tables *T, int comb, const double M[4], size_t nx, const double *x, const float *Tc,
+                      size_t ny, const double *y, double *s, double *a, int nthreads);
 
 /* Perturbative lepton loop (pi_pert.c).  x and y are in units of the loop lepton's inverse mass
  * (the reference's example1 passes M_v x, M_v y).  Singular where x, y or x-y vanishes. */

@@ -43,6 +43,8 @@ def _lib():
     L.oq_comb_len.argtypes = [ctypes.c_int]
     L.oq_combine.argtypes = [T, ctypes.c_int, v4, v4, v4, v4]
     L.oq_combine_batch.argtypes = [T, ctypes.c_int, v4, ctypes.c_size_t, v4, v4, v4, ctypes.c_int]
+    L.oq_contract_comb.argtypes = [T, ctypes.c_int, v4, ctypes.c_size_t, v4, f4, ctypes.c_size_t, v4, v4, v4,
+                                   ctypes.c_int]
     L.oq_chebU.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4]
     L.oq_chebU.restype = ctypes.c_double
     L.oq_chebU_avx.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, v4, T]
@@ -151,6 +153,20 @@ class Oracle:
         self.L.oq_combine_batch(ctypes.byref(self.t), int(comb), M4, x.shape[0], x.ravel(), y.ravel(), out.ravel(),
                                 int(nthreads))
         return out
+
+    def contract_comb(self, comb, x, Tc, y, M=None, nthreads=0):
+        """s[j, q] = sum_i K_q(x_i, y_j) . Tc[i] for the output kernels of a combination, and the error
+        scale a[j, q] = sum of |terms|."""
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64).reshape(-1, 4)
+        Tc = np.ascontiguousarray(Tc, np.float32)
+        nk = self.L.oq_comb_len(int(comb)) // 384
+        s = np.zeros((y.shape[0], nk))
+        a = np.zeros((y.shape[0], nk))
+        M4 = np.zeros(4) if M is None else np.ascontiguousarray(M, np.float64)
+        self.L.oq_contract_comb(ctypes.byref(self.t), int(comb), M4, x.shape[0], x.ravel(), Tc.ravel(), y.shape[0],
+                                y.ravel(), s.ravel(), a.ravel(), int(nthreads))
+        return s, a
 
     def stv(self, x, y):
         s = np.zeros(200)

@@ -194,3 +194,16 @@ def test_gpu_combine_full_table_many_passes(gpu_full, oracle_full):
     scale = np.abs(ref).max(axis=2, keepdims=True) + 1e-300
     err = (np.abs(got[idx] - ref) / scale).max()
     assert err < 1e-12, err
+
+
+def test_oracle_contract_comb_is_the_dot_of_the_combination(oracle_small):
+    """The oracle's combination contraction equals the dot of its combination outputs with T."""
+    rng = np.random.default_rng(3)
+    x, _ = combo_pairs(2, n=3)
+    T = rng.normal(size=(len(x), 384)).astype(np.float32)
+    y = np.array([[0.1, -0.2, 0.3, 0.5], [0.0, 0.0, 0.0, 0.7]])
+    s, a = oracle_small.contract_comb(COMB["CON_M_L2"], x, T, y, M=M4)
+    for j in range(len(y)):
+        K = oracle_small.combine_batch(COMB["CON_M_L2"], x, np.repeat(y[j:j + 1], len(x), axis=0), M4)
+        ref = np.einsum("iqc,ic->q", K, T.astype(np.float64))
+        assert np.abs(s[j] - ref).max() <= 1e-13 * a[j].max()

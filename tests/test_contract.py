@@ -56,3 +56,47 @@ def test_contract_deterministic(gpu_full):
     a = gpu_full.contract(2, x, T, y[:2])
     b = gpu_full.contract(2, x, T, y[:2])
     assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ through a kernel combination
+M4 = np.array([0.0, 0.4, 0.8, 1.2])
+
+
+@pytest.mark.parametrize("comb,M", [(kqed_amd.KQED_COMB_ALL, None), (kqed_amd.KQED_COMB_CON_M_L2, M4),
+                                    (kqed_amd.KQED_COMB_SYMXY_V2, None), (kqed_amd.KQED_COMB_SYMXY0_V2, None)])
+def test_contract_through_combination(gpu_full, oracle_full, comb, M):
+    """A lattice driver's per-y contraction with symmetrised / connected kernels: every output kernel
+    of the combination against the oracle, 4^4 lattice (x = 0 and x = y sites included), 3 y sites."""
+    import torch
+    x = W.lattice_sites(4)
+    T = W.site_tensors(x, seed=5)
+    y = W.y_sites()[:3]
+    dev = torch.device("cuda:0")
+    xd, Td = torch.from_numpy(x).to(dev), torch.from_numpy(T).to(dev)
+    print(f"contract comb={comb} nx={len(x)}", flush=True)
+    s = gpu_full.contract_comb_device(comb, len(x), xd.data_ptr(), Td.data_ptr(), y, M=M,
+                                      stream=torch.cuda.current_stream().cuda_stream)
+    ref, a = oracle_full.contract_comb(comb, x, T, y, M=M)
+    err = np.abs(s - ref) / a
+    assert err.max() <= 1e-12, err
+
+
+def test_contract_through_combination_multi_pass(gpu_full, oracle_full):
+    """More sites than one combination pass; deterministic across calls."""
+    import torch
+    x = W.lattice_sites(6)
+    T = W.site_tensors(x, seed=6)
+    y = W.y_sites()[3:5]
+    dev = torch.device("cuda:0")
+    xd, Td = torch.from_numpy(x).to(dev), torch.from_numpy(T).to(dev)
+    old = gpu_full.get_tune()
+    try:
+        gpu_full.set_tune(chunk=2048)
+        st = torch.cuda.current_stream().cuda_stream
+        s1 = gpu_full.contract_comb_device(kqed_amd.KQED_COMB_CON_V2, len(x), xd.data_ptr(), Td.data_ptr(), y, stream=st)
+        s2 = gpu_full.contract_comb_device(kqed_amd.KQED_COMB_CON_V2, len(x), xd.data_ptr(), Td.data_ptr(), y, stream=st)
+    finally:
+        gpu_full.set_tune(old)
+    assert np.array_equal(s1, s2)
+    ref, a = oracle_full.contract_comb(kqed_amd.KQED_COMB_CON_V2, x, T, y)
+    assert (np.abs(s1 - ref) / a).max() <= 1e-12
