@@ -1615,11 +1615,31 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
     if (lane + 128 < WREC) nx2 = rp[lane + 128];
     nk = kinds[p];
   }
+  /* The outputs of a pair wait in O and go to HBM at the start of the next iteration, after the
+   * next record is staged: vector loads and stores share one completion counter, so a store issued
+   * just before that staging would make the wavefront wait for its write to complete. */
+  size_t pq = 0;        /* pair whose outputs are pending in O (dot layouts: in pdot) */
+  double pdot = 0.0;
+  bool pending = false;
+  auto flush = [&]() {
+    if (layout == 16 || layout == 64) {
+      if (lane == 0) ((double *)outv)[pq] = pdot;
+    } else if (layout & 2) {
+      float *out = (float *)outv;
+      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + pq] = (float)O[i];
+      else for (int i = lane; i < 384; i += 64) out[pq * 384 + i] = (float)O[i];
+    } else {
+      double *out = (double *)outv;
+      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + pq] = O[i];
+      else for (int i = lane; i < 384; i += 64) out[pq * 384 + i] = O[i];
+    }
+  };
   for (; p < n; p += stride) {
     R[lane] = nx0;
     R[lane + 64] = nx1;
     if (lane + 128 < WREC) R[lane + 128] = nx2;
     const unsigned kall = __builtin_amdgcn_readfirstlane(nk);
+    if (pending) flush();
     if (p + stride < n) {
       const double *rp = rec + (p + stride) * WREC;
       nx0 = rp[lane];
@@ -1628,14 +1648,44 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       nk = kinds[p + stride];
     }
     if (lane == 0) Tv[200] = 0.0;
-    for (int i = lane; i < 384; i += 64) O[i] = 0.0;
+    /* one evaluation slot (L0): each lane's six rows go from registers straight into O (or the
+     * dot); several slots accumulate in O, which starts at zero */
+    const bool direct = nslots == 1;
+    const bool dotl = layout == 16 || layout == 64;
+    if (!direct)
+      for (int i = lane; i < 384; i += 64) O[i] = 0.0;
     wave_sync();
     const size_t q = (size_t)R[KQED_REC_P];
     double dot = 0.0;
     bool have_dot = false;
+    /* rows v[m] of map mq of the single slot: into O, or straight into the dot */
+    auto emit = [&](const double *v, int mq) {
+      if (dotl) {
+        double tq[6];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+          const int r = sRow[mq * 384 + 64 * m + lane];
+          tq[m] = layout == 16 ? (double)((const float *)Tc)[q * 384 + r] : ((const double *)Tc)[q * 384 + r];
+        }
+        double part = 0.0;
+#pragma unroll
+        for (int m = 0; m < 6; m++) part = fma(v[m], tq[m], part);
+        dot = wave_sum(part);
+        have_dot = true;
+        return;
+      }
+#pragma unroll
+      for (int m = 0; m < 6; m++) O[sRow[mq * 384 + 64 * m + lane]] = v[m];
+    };
     for (int slot = 0; slot < nslots; slot++) {
       const unsigned kd = (kall >> (3 * slot)) & 7u;
-      if (kd == KIND_NONE) continue;
+      if (kd == KIND_NONE) {
+        if (direct) {
+          const double z[6] = { 0, 0, 0, 0, 0, 0 };
+          emit(z, 0);
+        }
+        continue;
+      }
       const double *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
       if (kd == KIND_STD) {
         const double rx = 1.0 / pl[15], ry = 1.0 / pl[16];
@@ -1706,6 +1756,8 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
           const double sum = S[16 * rs + 4 * (xm ? nu : mu) + la];
           O[r] -= g * (v[m] - Mv * vv[xm ? mu : nu] * sum);
         }
+      } else if (direct) {
+        emit(v, mq);
       } else {
         const double cf = R[KQED_REC_COEF + slot];
 #pragma unroll
@@ -1716,30 +1768,23 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       wave_sync();
     }
-    if (layout == 16 || layout == 64) {
-      if (!have_dot) {
-        double part = 0.0;
-        if (layout == 16) {
-          const float *tq = (const float *)Tc + q * 384;
-          for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
-        } else {
-          const double *tq = (const double *)Tc + q * 384;
-          for (int i = lane; i < 384; i += 64) part = fma(O[i], tq[i], part);
-        }
-        dot = wave_sum(part);
+    if (dotl && !have_dot) {
+      double part = 0.0;
+      if (layout == 16) {
+        const float *tq = (const float *)Tc + q * 384;
+        for (int i = lane; i < 384; i += 64) part = fma(O[i], (double)tq[i], part);
+      } else {
+        const double *tq = (const double *)Tc + q * 384;
+        for (int i = lane; i < 384; i += 64) part = fma(O[i], tq[i], part);
       }
-      if (lane == 0) ((double *)outv)[q] = dot;
-    } else if (layout & 2) {
-      float *out = (float *)outv;
-      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = (float)O[i];
-      else for (int i = lane; i < 384; i += 64) out[q * 384 + i] = (float)O[i];
-    } else {
-      double *out = (double *)outv;
-      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + q] = O[i];
-      else for (int i = lane; i < 384; i += 64) out[q * 384 + i] = O[i];
+      dot = wave_sum(part);
     }
+    pq = q;
+    pdot = dot;
+    pending = true;
     wave_sync();
   }
+  if (pending) flush();
 }
 
 /* ------------------------------------------------------------------ contraction helpers */

@@ -145,19 +145,23 @@ __global__ void __launch_bounds__(256) k_pihat1(size_t items, int ev3, const dou
   const signed char *tm = tab + sg * 64 + lane;
   const signed char *t6 = tab + PT_MAIN + sg * 384 + lane;
   double t1[4] = { 0.0, 0.0, 0.0, 0.0 }, tr1 = 0.0;
+  /* (alpha, beta) stay loops: the propagator factors stay in registers across them without spills */
+#pragma unroll 1
   for (int a = 0; a < 4; a++) {
+#pragma unroll 1
     for (int b = 0; b < 4; b++) {
       const int ab = 4 * a + b;
       const double xab = C.xab[ab];
       const signed char *gp = tm + ab * 4096;
+      /* the reference's t1 -= fv * T as fused multiply-adds (its order; <= 2e-15 normwise apart) */
 #pragma unroll
       for (int cd = 0; cd < 16; cd++) {
         const double T = xab * (double)gp[cd * 256];
-        t1[0] -= C.fv[4 * cd + 0] * T;
-        t1[1] -= C.fv[4 * cd + 1] * T;
-        t1[2] -= C.fv[4 * cd + 2] * T;
-        t1[3] -= C.fv[4 * cd + 3] * T;
-        tr1 -= C.ell[cd] * T;
+        t1[0] = fma(-C.fv[4 * cd + 0], T, t1[0]);
+        t1[1] = fma(-C.fv[4 * cd + 1], T, t1[1]);
+        t1[2] = fma(-C.fv[4 * cd + 2], T, t1[2]);
+        t1[3] = fma(-C.fv[4 * cd + 3], T, t1[3]);
+        tr1 = fma(-C.ell[cd], T, tr1);
       }
       const signed char *sp = t6 + ab * 4 * 384;
       const double m1 = (double)sp[0] * xab;
