@@ -368,9 +368,9 @@ DEV double interpol3(double y, double y1, double y2, double dy, double f1, doubl
 }
 
 /* Scalar-path Clenshaw sums (reference cheby.c:13-91) of order D over the value and d/dy arrays of
- * one or two grid nodes in one pass: r = {a.x, a.y, b.x, b.y}.  Order 0 is the reference's
- * expression; for D >= 1 the step factors (j+1)/j, (j+3)/(j+1), ... are read from the ratio table
- * instead of dividing at every step (the two forms agree to rounding). */
+ * one or two grid nodes in one pass: r = {a.x, a.y, b.x, b.y}.  The step factors (j+1)/j,
+ * (j+3)/(j+1), ... of D >= 1 are read from the ratio table instead of dividing at every step, and
+ * every step is two fused multiply-adds per chain (the forms agree to rounding). */
 template <int D, typename P1>
 DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const double *LUTg, double r[4])
 {
@@ -387,16 +387,16 @@ DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const do
       for (int i = 0; i < 4; i++) {
         if (i >= nc) break;
         const double t = yb[i];
-        yb[i] = twox * yb[i] - ya[i] + c[i];
+        yb[i] = fma(twox, yb[i], c[i] - ya[i]);
         ya[i] = t;
       }
     } else {
-      const double PA = LUT[8 * j + 2 * D - 1], PB = LUT[8 * j + 2 * D];
+      const double PA = twox * LUT[8 * j + 2 * D - 1], PB = LUT[8 * j + 2 * D];
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         if (i >= nc) break;
         const double t = yb[i];
-        yb[i] = twox * yb[i] * PA - ya[i] * PB + c[i];
+        yb[i] = fma(PA, yb[i], fma(-PB, ya[i], c[i]));
         ya[i] = t;
       }
     }
