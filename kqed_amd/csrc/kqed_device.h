@@ -33,6 +33,7 @@
 #define KQED_REC_P 15     /* input index of the pair */
 #define KQED_REC_MAIN 16  /* slot 0 payload [64]: stage-1 record, then the 36 primitives */
 #define KQED_REC_AUX 80   /* slots 1..3 payloads [20] each */
+#define KQED_HELP_BASE 36 /* main payload: after the 36 primitives, the assembly's helper inputs [26] */
 #define KQED_AUX_LEN 20
 
 /* Learned surrogate of the packed-path interpolation: the interpolated weight functions it stands

@@ -1139,6 +1139,27 @@ k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds
   o[30] = D.ddg1dxdcb; o[31] = D.ddg2dxdcb; o[32] = D.ddg1dydcb; o[33] = D.ddg2dydcb;
   o[34] = D.ddg1dcbdcb; o[35] = D.ddg2dcbdcb;
   for (int i = 0; i < 36; i++) m[i] = o[i];
+  /* Per-pair vectors and sums the assembly's helper vectors are built from (std_helper), written
+   * after the primitives: xh = x/|x|, yh = y/|y|, c2 = (xh - cb yh)/|y|, c4 = (yh - cb xh)/|x|,
+   * e12 = xh/|y| + yh/|x| - cb (xh/|x| + yh/|y|), P21 + P22, P23 + P24, 1/|x|, 1/|y|, 1, 0. */
+  const double *hx = rec + p * KQED_REC_LEN + KQED_REC_X, *hy = rec + p * KQED_REC_LEN + KQED_REC_Y;
+  const double rx = 1.0 / I.x, ry = 1.0 / I.yorig, cb = I.cborig;
+  double h[26];
+  for (int i = 0; i < 4; i++) {
+    const double xh = hx[i] * rx, yh = hy[i] * ry;
+    h[i] = xh;
+    h[4 + i] = yh;
+    h[8 + i] = (xh - cb * yh) * ry;
+    h[12 + i] = (yh - cb * xh) * rx;
+    h[16 + i] = xh * ry + yh * rx - cb * (xh * rx + yh * ry);
+  }
+  h[20] = o[21] + o[22];
+  h[21] = o[23] + o[24];
+  h[22] = rx;
+  h[23] = ry;
+  h[24] = 1.0;
+  h[25] = 0.0;
+  for (int i = 0; i < 26; i++) m[KQED_HELP_BASE + i] = h[i];
 }
 
 /* ------------------------------------------------------------------ Taylor-table primitives */
@@ -1417,32 +1438,43 @@ template __global__ void k_interp<2>(DT, size_t, const double *, int, const doub
 #define NH 92
 /* rx = 1/x, ry = 1/y: the helpers multiply by the reciprocals where the reference divides by x
  * and y (the results agree to rounding) */
-DEV double std_helper(const double *P, const double *xv, const double *yv, int j, double rx, double ry)
+/* Helper vectors of the standard assembly (per pair, in LDS), groups of four (index i = j & 3):
+ *   0 xh, 1 yh, 2 c2, 3 c4, 4..8 x P[26+2k] + y P[27+2k], 9..10 (x P[21+2k] + y P[22+2k])/|x|,
+ *   11 (P21+P22) xh + (P23+P24) c4, 12 P21 xh + P25 yh + P23 e12,
+ *   13..15 xh P[6+3k] + c4 P[8+3k], 16..18 yh P[7+3k] + c2 P[8+3k],
+ * then 16 quadratic ones, j = 76 + 4a + b.  Groups 0..18 are all ((U_i a + V_i b) + W_i c) s with
+ * the vectors U, V, W and scalars a, b, c, s read from the record (offsets from its start, HDESC),
+ * so the 64 lanes run one instruction stream; the vectors xh, yh, c2, c4, e12 and the sums come
+ * from the algebra kernel, and every helper keeps its expression's order of operations. */
+#define HR(k) (KQED_REC_MAIN + (k))                  /* primitive k */
+#define HV(k) (KQED_REC_MAIN + KQED_HELP_BASE + (k)) /* algebra-kernel value k */
+enum { HX = KQED_REC_X, HY = KQED_REC_Y, HXH = HV(0), HYH = HV(4), HC2 = HV(8), HC4 = HV(12), HE12 = HV(16),
+       H2122 = HV(20), H2324 = HV(21), HRX = HV(22), HRY = HV(23), HONE = HV(24), HZERO = HV(25) };
+/* U, a, V, b, W, c, s */
+__constant__ unsigned char HDESC[19][7] = {
+  { HXH, HONE, HX, HZERO, HX, HZERO, HONE },       { HYH, HONE, HX, HZERO, HX, HZERO, HONE },
+  { HC2, HONE, HX, HZERO, HX, HZERO, HONE },       { HC4, HONE, HX, HZERO, HX, HZERO, HONE },
+  { HX, HR(26), HY, HR(27), HX, HZERO, HONE },     { HX, HR(28), HY, HR(29), HX, HZERO, HONE },
+  { HX, HR(30), HY, HR(31), HX, HZERO, HONE },     { HX, HR(32), HY, HR(33), HX, HZERO, HONE },
+  { HX, HR(34), HY, HR(35), HX, HZERO, HONE },     { HX, HR(21), HY, HR(22), HX, HZERO, HRX },
+  { HX, HR(23), HY, HR(24), HX, HZERO, HRX },      { HXH, H2122, HC4, H2324, HX, HZERO, HONE },
+  { HXH, HR(21), HYH, HR(25), HE12, HR(23), HONE }, { HXH, HR(6), HC4, HR(8), HX, HZERO, HONE },
+  { HXH, HR(9), HC4, HR(11), HX, HZERO, HONE },    { HXH, HR(12), HC4, HR(14), HX, HZERO, HONE },
+  { HYH, HR(7), HC2, HR(8), HX, HZERO, HONE },     { HYH, HR(10), HC2, HR(11), HX, HZERO, HONE },
+  { HYH, HR(13), HC2, HR(14), HX, HZERO, HONE } };
+
+/* d: this helper's descriptor packed as bytes U|a|V|b (d.x) and W|c|s (d.y); R: the record in LDS */
+DEV double std_helper(const double *R, int j, uint2 d)
 {
-  const double x = P[15], y = P[16], cb = P[17];
-  (void)x; (void)y;
-  const int i = j & 3;
-  const double xh = xv[i] * rx, yh = yv[i] * ry;
-  if (j < 4) return xh;
-  if (j < 8) return yh;
-  const double c2 = (xh - cb * yh) * ry, c4 = (yh - cb * xh) * rx;
-  if (j < 12) return c2;
-  if (j < 16) return c4;
-  if (j < 20) return xv[i] * P[26] + yv[i] * P[27];
-  if (j < 24) return xv[i] * P[28] + yv[i] * P[29];
-  if (j < 28) return xv[i] * P[30] + yv[i] * P[31];
-  if (j < 32) return xv[i] * P[32] + yv[i] * P[33];
-  if (j < 36) return xv[i] * P[34] + yv[i] * P[35];
-  if (j < 40) return (xv[i] * P[21] + yv[i] * P[22]) * rx;
-  if (j < 44) return (xv[i] * P[23] + yv[i] * P[24]) * rx;
-  if (j < 48) return (P[21] + P[22]) * xh + (P[23] + P[24]) * c4;
-  if (j < 52) return (P[21] * xh + P[25] * yh + P[23] * (xh * ry + yh * rx - cb * (xh * rx + yh * ry)));
-  if (j < 56) return xh * P[6] + c4 * P[8];
-  if (j < 60) return xh * P[9] + c4 * P[11];
-  if (j < 64) return xh * P[12] + c4 * P[14];
-  if (j < 68) return yh * P[7] + c2 * P[8];
-  if (j < 72) return yh * P[10] + c2 * P[11];
-  if (j < 76) return yh * P[13] + c2 * P[14];
+  if (j < 76) {
+    const int i = j & 3;
+    const double u = R[(d.x & 0xFF) + i] * R[(d.x >> 8) & 0xFF];
+    const double v = R[((d.x >> 16) & 0xFF) + i] * R[d.x >> 24];
+    const double w = R[(d.y & 0xFF) + i] * R[(d.y >> 8) & 0xFF];
+    return ((u + v) + w) * R[(d.y >> 16) & 0xFF];
+  }
+  const double *xv = R + KQED_REC_X, *yv = R + KQED_REC_Y;
+  const double rx = R[HRX], ry = R[HRY], cb = R[HR(17)];
   const int a = (j - 76) >> 2, b = (j - 76) & 3;
   const double xa = xv[a] * rx, xb = xv[b] * rx, ya = yv[a] * ry, yb = yv[b] * ry;
   const int d_ab = (a == b);
@@ -1605,6 +1637,13 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
   /* the next pair's record and kinds are loaded into registers while the current pair is assembled,
    * so the wavefront never waits on a record load from HBM */
   static_assert(WREC <= 192, "record prefetch covers three doubles per lane");
+  static_assert(WH == 92 && KQED_HELP_BASE + 26 <= 64, "helper layout");
+  /* descriptors of this lane's two helpers (j = lane, lane + 64) */
+  auto hdesc = [](int g) {
+    const unsigned char *e = HDESC[g < 19 ? g : 0];
+    return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
+  };
+  const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
   size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w;
   double nx0 = 0.0, nx1 = 0.0, nx2 = 0.0;
   unsigned nk = 0;
@@ -1688,8 +1727,8 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       const double *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
       if (kd == KIND_STD) {
-        const double rx = 1.0 / pl[15], ry = 1.0 / pl[16];
-        for (int j = lane; j < WH; j += 64) H[j] = std_helper(pl, R + KQED_REC_X, R + KQED_REC_Y, j, rx, ry);
+        H[lane] = std_helper(R, lane, hd0);
+        if (lane + 64 < WH) H[lane + 64] = std_helper(R, lane + 64, hd1);
       } else if (lane < 4) {
         H[lane] = pl[lane] / pl[4];
       }
