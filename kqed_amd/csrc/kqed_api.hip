@@ -33,7 +33,7 @@ __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
 template <int CLS>
 __global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys,
                          const unsigned *perm, const unsigned *bounds, int ilo, int ihi, size_t c0, size_t c1, int which,
-                         double M, double *rec, unsigned *kinds, const unsigned char *smask);
+                         double M, double *rec, unsigned *kinds, const unsigned char *smask, int fuse);
 __global__ void k_extract_scalars(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, double *out);
 __global__ void k_surr_rowbin(kqed_dev_tables T, size_t n, const double *data, int *bin);
 /* surrogate kernels (kqed_surrogate.hip) */
@@ -482,6 +482,9 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     prof_end(d, s, ev);
   }
   static const int nslots[5] = { 1, 3, 3, 4, 3 };
+  /* the interpolation kernels run the chain-rule algebra themselves unless stage-1 values are
+   * still to be replaced (surrogate) or are the output (extraction) */
+  const int fuse = sur == nullptr && layout != LAYOUT_SCALARS;
   for (size_t c0 = 0; c0 < n; c0 += CH) {
     const size_t c = n - c0 < CH ? n - c0 : CH, c1 = c0 + c;
     const unsigned blocks = (unsigned)((c + 255) / 256);
@@ -492,23 +495,23 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       const int ip = 1 + T.nstpx * T.nstpy;
       const unsigned char *smask = sur ? sur->mask : nullptr;
       hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
-                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
       hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
-                         which, M, w.rec, w.kinds, smask);
+                         which, M, w.rec, w.kinds, smask, fuse);
       hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
-                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
       if (smask)
         hipLaunchKernelGGL(k_surr_records, dim3((unsigned)((c + 63) / 64)), dim3(256), 0, s, c, w.rec, w.kinds, sur->net);
     } else {
       hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
-                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr);
+                         which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
     }
     if (layout == LAYOUT_SCALARS) {
       hipLaunchKernelGGL(k_extract_scalars, dim3(blocks), dim3(256), 0, s, T, c, w.rec, w.kinds, (double *)out_dev);
       prof_end(d, s, ev);
       continue;
     }
-    hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
+    if (!fuse) hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
     const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
     const unsigned grid = (unsigned)(nblk < (size_t)tu.asm_grid ? nblk : (size_t)tu.asm_grid);

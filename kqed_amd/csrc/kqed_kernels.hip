@@ -1106,12 +1106,8 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
 
 /* Chain-rule algebra of chnr_dT/chnr_dV (standard branch), the x <-> x-y swap, and the 36
  * primitives the assembly kernel reads (S: 3, T: 12, invariants: 6, V: 15), written in place. */
-__global__ void __launch_bounds__(256)
-k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds)
+DEV void algebra_main(double *__restrict__ m, const double *hx, const double *hy)
 {
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n || (kinds[p] & 7u) != (unsigned)KIND_STD) return;
-  double *m = rec + p * KQED_REC_LEN + KQED_REC_MAIN;
   Stage1 s = *(const Stage1 *)m;
   Inv I;
   I.x = s.x; I.cb = s.cb; I.y = s.y; I.xsq = s.xsq; I.ysq = s.ysq; I.xdoty = s.xdoty;
@@ -1142,7 +1138,6 @@ k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds
   /* Per-pair vectors and sums the assembly's helper vectors are built from (std_helper), written
    * after the primitives: xh = x/|x|, yh = y/|y|, c2 = (xh - cb yh)/|y|, c4 = (yh - cb xh)/|x|,
    * e12 = xh/|y| + yh/|x| - cb (xh/|x| + yh/|y|), P21 + P22, P23 + P24, 1/|x|, 1/|y|, 1, 0. */
-  const double *hx = rec + p * KQED_REC_LEN + KQED_REC_X, *hy = rec + p * KQED_REC_LEN + KQED_REC_Y;
   const double rx = 1.0 / I.x, ry = 1.0 / I.yorig, cb = I.cborig;
   double h[26];
   for (int i = 0; i < 4; i++) {
@@ -1160,6 +1155,16 @@ k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds
   h[24] = 1.0;
   h[25] = 0.0;
   for (int i = 0; i < 26; i++) m[KQED_HELP_BASE + i] = h[i];
+}
+
+/* The algebra as its own pass (records whose stage-1 values come from elsewhere: the surrogate) */
+__global__ void __launch_bounds__(256)
+k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds)
+{
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || (kinds[p] & 7u) != (unsigned)KIND_STD) return;
+  double *r = rec + p * KQED_REC_LEN;
+  algebra_main(r + KQED_REC_MAIN, r + KQED_REC_X, r + KQED_REC_Y);
 }
 
 /* ------------------------------------------------------------------ Taylor-table primitives */
@@ -1376,7 +1381,7 @@ template <int CLS>
 __global__ void __launch_bounds__(256, InterpWaves<CLS>::v)
 k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
          const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi, size_t c0, size_t c1,
-         int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds, const unsigned char *smask)
+         int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds, const unsigned char *smask, int fuse)
 {
   size_t lo = ilo < 0 ? 0 : bounds[ilo], hi = ihi < 0 ? n : bounds[ihi];
   lo = lo > c0 ? lo : c0;
@@ -1417,17 +1422,19 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   r[KQED_REC_GY] = exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.);
   r[KQED_REC_P] = (double)p;
   kinds[ri] = kd;
+  /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
+  if (fuse && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
 }
 
 template __global__ void k_interp<0>(DT, size_t, const double *, int, const double *, int, const unsigned *,
                                      const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
-                                     const unsigned char *);
+                                     const unsigned char *, int);
 template __global__ void k_interp<1>(DT, size_t, const double *, int, const double *, int, const unsigned *,
                                      const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
-                                     const unsigned char *);
+                                     const unsigned char *, int);
 template __global__ void k_interp<2>(DT, size_t, const double *, int, const double *, int, const unsigned *,
                                      const unsigned *, int, int, size_t, size_t, int, double, double *, unsigned *,
-                                     const unsigned char *);
+                                     const unsigned char *, int);
 
 /* ------------------------------------------------------------------ tensor entries */
 /* Per-pair helper vectors of the standard path (shared by many tensor entries), each evaluated with
