@@ -1652,13 +1652,15 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
   size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w;
+  /* one evaluation slot reads the header and main payload only (80 of the record's doubles) */
+  const int used = nslots == 1 ? KQED_REC_AUX : WREC;
   double nx0 = 0.0, nx1 = 0.0, nx2 = 0.0;
   unsigned nk = 0;
   if (p < n) {
     const double *rp = rec + p * WREC;
     nx0 = rp[lane];
-    nx1 = rp[lane + 64];
-    if (lane + 128 < WREC) nx2 = rp[lane + 128];
+    if (lane + 64 < used) nx1 = rp[lane + 64];
+    if (lane + 128 < used) nx2 = rp[lane + 128];
     nk = kinds[p];
   }
   /* The outputs of a pair wait in O and go to HBM at the start of the next iteration, after the
@@ -1682,15 +1684,15 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
   };
   for (; p < n; p += stride) {
     R[lane] = nx0;
-    R[lane + 64] = nx1;
-    if (lane + 128 < WREC) R[lane + 128] = nx2;
+    if (lane + 64 < used) R[lane + 64] = nx1;
+    if (lane + 128 < used) R[lane + 128] = nx2;
     const unsigned kall = __builtin_amdgcn_readfirstlane(nk);
     if (pending) flush();
     if (p + stride < n) {
       const double *rp = rec + (p + stride) * WREC;
       nx0 = rp[lane];
-      nx1 = rp[lane + 64];
-      if (lane + 128 < WREC) nx2 = rp[lane + 128];
+      if (lane + 64 < used) nx1 = rp[lane + 64];
+      if (lane + 128 < used) nx2 = rp[lane + 128];
       nk = kinds[p + stride];
     }
     if (lane == 0) Tv[200] = 0.0;
