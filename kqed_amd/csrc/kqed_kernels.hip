@@ -411,20 +411,34 @@ DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const do
 }
 
 /* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays */
+/* UNI: ix and iy are the same for every active lane of the wavefront (scalar registers), so the
+ * coefficient arrays are read through the constant address space: scalar loads */
+template <bool UNI = false>
 DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm, bool ndy, int ndcb,
                      double cb, double y)
 {
   const int mm = mm_of(nm);
   const int nk = T.nfx[ix] + mm;
   const int ord = mm + ndcb;
-  const double *a = (const double *)(T.S + node(T, nm, ix, iy));
-  const double *b = flag_hy ? nullptr : (const double *)(T.S + node(T, nm, ix, iy + 1));
   double r[4];
-  switch (ord) {
-  case 0: cheb_r<0>(nk, cb, a, b, T.LUT, r); break;
-  case 1: cheb_r<1>(nk, cb, a, b, T.LUT, r); break;
-  case 2: cheb_r<2>(nk, cb, a, b, T.LUT, r); break;
-  default: cheb_r<3>(nk, cb, a, b, T.LUT, r); break;
+  if (UNI) {
+    const cdouble *a = (const cdouble *)(T.S + node(T, nm, ix, iy));
+    const cdouble *b = flag_hy ? nullptr : (const cdouble *)(T.S + node(T, nm, ix, iy + 1));
+    switch (ord) {
+    case 0: cheb_r<0>(nk, cb, a, b, T.LUT, r); break;
+    case 1: cheb_r<1>(nk, cb, a, b, T.LUT, r); break;
+    case 2: cheb_r<2>(nk, cb, a, b, T.LUT, r); break;
+    default: cheb_r<3>(nk, cb, a, b, T.LUT, r); break;
+    }
+  } else {
+    const double *a = (const double *)(T.S + node(T, nm, ix, iy));
+    const double *b = flag_hy ? nullptr : (const double *)(T.S + node(T, nm, ix, iy + 1));
+    switch (ord) {
+    case 0: cheb_r<0>(nk, cb, a, b, T.LUT, r); break;
+    case 1: cheb_r<1>(nk, cb, a, b, T.LUT, r); break;
+    case 2: cheb_r<2>(nk, cb, a, b, T.LUT, r); break;
+    default: cheb_r<3>(nk, cb, a, b, T.LUT, r); break;
+    }
   }
   const double r10 = ndy ? r[1] : r[0], r11 = r[1], r20 = ndy ? r[3] : r[2], r21 = r[3];
   if (!flag_hy) {
@@ -620,10 +634,10 @@ DEV void V_algebra(Vt &D, const Inv &I)
 }
 
 /* Taylor-x branch of chnr_dS (reference chnr_dS.c:10-45) */
-DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, const Inv &I)
+template <bool UNI>
+DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, const Inv &I, int iy)
 {
   const int iy_tay = bsrch(T.TX + TYY * NYTAY, I.y, 0, NYTAY);
-  const int iy = bsrch(T.YY, I.y, 0, T.nstpy);
   if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T.TX;
@@ -631,22 +645,22 @@ DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, cons
   const double ay = (TXr(TYY, i2) - I.y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
   const double ax = (T.XX[0] - I.x) / (T.XX[0]);
   double f1 = I.cb * lerp(ay, TXr(G0dx, iy_tay), TXr(G0dx, i2));
-  double f2 = accessv_s(T, false, true, 0, iy, dxQG0, false, 0, I.cb, I.y);
+  double f2 = accessv_s<UNI>(T, false, true, 0, iy, dxQG0, false, 0, I.cb, I.y);
   dg0dx = lerp(ax, f1, f2);
   f1 = lerp(ay, TXr(G0dy, iy_tay), TXr(G0dy, i2));
-  f2 = accessv_s(T, false, false, 0, iy, QG0, true, 0, I.cb, I.y);
+  f2 = accessv_s<UNI>(T, false, false, 0, iy, QG0, true, 0, I.cb, I.y);
   dg0dy = lerp(ax, f1, f2);
-  f2 = accessv_s(T, false, true, 0, iy, QG0, false, 1, I.cb, I.y);
+  f2 = accessv_s<UNI>(T, false, true, 0, iy, QG0, false, 1, I.cb, I.y);
   dg0dcb = lerp(ax, 0.0, f2);
   return 0;
 }
 
 /* Taylor-x branch of chnr_dT (reference chnr_dT.c:15-143) */
-DEV int T_taylor(const DT &T_, Tt &T, const Inv &I)
+template <bool UNI>
+DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy)
 {
   const double ysq = I.flag2 ? I.xmysq : I.ysq;
   const int iy_tay = bsrch(T_.TX + TYY * NYTAY, I.y, 0, NYTAY);
-  const int iy = bsrch(T_.YY, I.y, 0, T_.nstpy);
   if (iy_tay == NYTAY - 1 || iy == T_.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T_.TX;
@@ -654,24 +668,24 @@ DEV int T_taylor(const DT &T_, Tt &T, const Inv &I)
   const double ax = (T_.XX[0] - I.x) / (T_.XX[0]);
   const double xb = T_.XX[0], xbsq = xb * xb, cb = I.cb, y = I.y;
   const double ell2a = lerp(ay, TXr(Gl2, iy_tay), TXr(Gl2, i2));
-  const double ell2b = accessv_s(T_, false, true, 0, iy, QL2, false, 0, cb, y);
+  const double ell2b = accessv_s<UNI>(T_, false, true, 0, iy, QL2, false, 0, cb, y);
   T.ell2 = lerp(ax, ell2a, ell2b);
   const double dell2adx = lerp(ay, TXr(Gl21, iy_tay), TXr(Gl21, i2)) * cb;
-  const double dell2bdx = accessv_s(T_, false, true, 0, iy, dxQL2, false, 0, cb, y);
+  const double dell2bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL2, false, 0, cb, y);
   T.dell2dx = lerp(ax, dell2adx, dell2bdx);
   const double dell2ady = lerp(ay, TXr(Gl2dy, iy_tay), TXr(Gl2dy, i2));
-  const double dell2bdy = accessv_s(T_, false, false, 0, iy, QL2, true, 0, cb, y);
+  const double dell2bdy = accessv_s<UNI>(T_, false, false, 0, iy, QL2, true, 0, cb, y);
   T.dell2dy = lerp(ax, dell2ady, dell2bdy);
-  const double dell2bdcb = accessv_s(T_, false, true, 0, iy, QL2, false, 1, cb, y);
+  const double dell2bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL2, false, 1, cb, y);
   T.dell2dcb = lerp(ax, 0.0, dell2bdcb);
-  const double v1b = accessv_s(T_, false, true, 0, iy, QG1, false, 0, cb, y);
-  const double dv1bdx = accessv_s(T_, false, true, 0, iy, dxQG1, false, 0, cb, y);
-  const double dv1bdy = accessv_s(T_, false, false, 0, iy, QG1, true, 0, cb, y);
-  const double dv1bdcb = accessv_s(T_, false, true, 0, iy, QG1, false, 1, cb, y);
-  const double ell4b = accessv_s(T_, false, true, 0, iy, QL4, false, 0, cb, y);
-  const double dell4bdx = accessv_s(T_, false, true, 0, iy, dxQL4, false, 0, cb, y);
-  const double dell4bdy = accessv_s(T_, false, false, 0, iy, QL4, true, 0, cb, y);
-  const double dell4bdcb = accessv_s(T_, false, true, 0, iy, QL4, false, 1, cb, y);
+  const double v1b = accessv_s<UNI>(T_, false, true, 0, iy, QG1, false, 0, cb, y);
+  const double dv1bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQG1, false, 0, cb, y);
+  const double dv1bdy = accessv_s<UNI>(T_, false, false, 0, iy, QG1, true, 0, cb, y);
+  const double dv1bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QG1, false, 1, cb, y);
+  const double ell4b = accessv_s<UNI>(T_, false, true, 0, iy, QL4, false, 0, cb, y);
+  const double dell4bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL4, false, 0, cb, y);
+  const double dell4bdy = accessv_s<UNI>(T_, false, false, 0, iy, QL4, true, 0, cb, y);
+  const double dell4bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL4, false, 1, cb, y);
   const double ell3a = lerp(ay, TXr(Gl3, iy_tay), TXr(Gl3, i2));
   const double ell3b = ell4b / (2.0 * xbsq * ysq) - y * cb * ell2b / xb;
   T.ell3 = lerp(ax, ell3a, ell3b);
@@ -696,44 +710,44 @@ DEV int T_taylor(const DT &T_, Tt &T, const Inv &I)
 }
 
 /* Taylor-x branch of chnr_dV (reference chnr_dV.c:18-174) */
-DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2, double cb)
+template <bool UNI>
+DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2, double cb, int iy)
 {
   const int iy_tay = bsrch(T.TX + TYY * NYTAY, y, 0, NYTAY);
-  const int iy = bsrch(T.YY, y, 0, T.nstpy);
   if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T.TX;
   const double ay = (TXr(TYY, i2) - y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
   const double ax = (T.XX[0] - x) / (T.XX[0]);
   const double xb = T.XX[0], xbsq = xb * xb;
-  const double g2b = accessv_s(T, false, true, 0, iy, QG2, false, 0, cb, y);
+  const double g2b = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 0, cb, y);
   D.g2 = lerp(ax, 0.0, g2b);
   const double ddg2adxdcb = lerp(ay, TXr(G21, iy_tay), TXr(G21, i2));
-  const double ddg2bdxdcb = accessv_s(T, false, true, 0, iy, dxQG2, false, 1, cb, y);
+  const double ddg2bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG2, false, 1, cb, y);
   D.ddg2dxdcb = lerp(ax, ddg2adxdcb, ddg2bdxdcb);
-  const double dg2bdx = accessv_s(T, false, true, 0, iy, dxQG2, false, 0, cb, y);
+  const double dg2bdx = accessv_s<UNI>(T, false, true, 0, iy, dxQG2, false, 0, cb, y);
   D.dg2dx = D.ddg2dxdcb * cb;
-  const double dg2bdcb = accessv_s(T, false, true, 0, iy, QG2, false, 1, cb, y);
+  const double dg2bdcb = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 1, cb, y);
   D.dg2dcb = D.ddg2dxdcb * x;
-  const double dg2bdy = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, y);
+  const double dg2bdy = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, y);
   D.dg2dy = lerp(ax, 0.0, dg2bdy);
   const double ddg2adxdy = cb * lerp(ay, TXr(G21dy, iy_tay), TXr(G21dy, i2));
-  const double ddg2bdxdy = accessv_s(T, false, false, 0, iy, dxQG2, true, 0, cb, y);
+  const double ddg2bdxdy = accessv_s<UNI>(T, false, false, 0, iy, dxQG2, true, 0, cb, y);
   D.ddg2dxdy = lerp(ax, ddg2adxdy, ddg2bdxdy);
   const double ddg2adxdx = 2.0 * (lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2))
                                   + lerp(ay, TXr(G22B, iy_tay), TXr(G22B, i2)) * (0.4 + 0.6 * (2.0 * cb * cb - 1)));
-  const double ddg2bdxdx = accessv_s(T, false, true, 0, iy, d2xQG2, false, 0, cb, y);
+  const double ddg2bdxdx = accessv_s<UNI>(T, false, true, 0, iy, d2xQG2, false, 0, cb, y);
   D.ddg2dxdx = lerp(ax, ddg2adxdx, ddg2bdxdx);
-  const double ddg2bdcbdcb = accessv_s(T, false, true, 0, iy, QG2, false, 2, cb, y);
+  const double ddg2bdcbdcb = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 2, cb, y);
   D.ddg2dcbdcb = ddg2bdcbdcb * xsq / xbsq;
-  const double ddg2bdydcb = accessv_s(T, false, false, 0, iy, QG2, true, 1, cb, y);
+  const double ddg2bdydcb = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 1, cb, y);
   D.ddg2dydcb = lerp(ax, 0.0, ddg2bdydcb);
-  const double dg3bdy = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, y);
-  const double ddg3bdxdy = accessv_s(T, false, false, 0, iy, dxQG3, true, 0, cb, y);
-  const double ddg3bdxdcb = accessv_s(T, false, true, 0, iy, dxQG3, false, 1, cb, y);
-  const double ddg3bdxdx = accessv_s(T, false, true, 0, iy, d2xQG3, false, 0, cb, y);
-  const double ddg3bdcbdcb = accessv_s(T, false, true, 0, iy, QG3, false, 2, cb, y);
-  const double ddg3bdydcb = accessv_s(T, false, false, 0, iy, QG3, true, 1, cb, y);
+  const double dg3bdy = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, y);
+  const double ddg3bdxdy = accessv_s<UNI>(T, false, false, 0, iy, dxQG3, true, 0, cb, y);
+  const double ddg3bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG3, false, 1, cb, y);
+  const double ddg3bdxdx = accessv_s<UNI>(T, false, true, 0, iy, d2xQG3, false, 0, cb, y);
+  const double ddg3bdcbdcb = accessv_s<UNI>(T, false, true, 0, iy, QG3, false, 2, cb, y);
+  const double ddg3bdydcb = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 1, cb, y);
   const double dg1bdy = dg3bdy - cb / xb * g2b - (y / xb) * cb * dg2bdy;
   const double ddg1bdxdx = ddg3bdxdx - 2.0 * y / (xbsq * xb) * cb * g2b + y / xbsq * cb * dg2bdx + y / xbsq * cb * dg2bdx
     - y / xb * cb * ddg2bdxdx;
@@ -759,14 +773,14 @@ DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2,
   D.ddg1dydcb = lerp(ax, 0.0, ddg1bdydcb);
   if (flag2) {
     const double ya = T.YY[iy], yb = T.YY[iy + 1];
-    const double dg2bdy_ya = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, ya);
-    const double dg2bdy_yb = accessv_s(T, false, false, 0, iy, QG2, true, 0, cb, yb);
+    const double dg2bdy_ya = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, ya);
+    const double dg2bdy_yb = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, yb);
     const double ddg2bdydy = (dg2bdy_yb - dg2bdy_ya) / (T.YY[1] - T.YY[0]);
     D.ddg2dydy = ax * 0.0 + (1.0 - ax) * ddg2bdydy;
     const double ddg1adydy = (TXr(G3Ady, i2) - TXr(G3Bdy, i2) - (TXr(G3Ady, iy_tay) - TXr(G3Bdy, iy_tay)))
       / (TXr(TYY, i2) - TXr(TYY, iy_tay));
-    const double dg3bdy_ya = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, ya);
-    const double dg3bdy_yb = accessv_s(T, false, false, 0, iy, QG3, true, 0, cb, yb);
+    const double dg3bdy_ya = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, ya);
+    const double dg3bdy_yb = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, yb);
     const double ddg3bdydy = (dg3bdy_yb - dg3bdy_ya) / (T.YY[1] - T.YY[0]);
     const double ddg1bdydy = ddg3bdydy - (y * ddg2bdydy + 2.0 * dg2bdy) * cb / x;
     D.ddg1dydy = ax * ddg1adydy + (1.0 - ax) * ddg1bdydy;
@@ -1023,13 +1037,23 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0); the Taylor
      * branches already apply the algebra (chnr_dT.c:16-143, chnr_dV.c:19-174) */
     double dg0dx, dg0dy, dg0dcb;
-    if (S_taylor(T, dg0dy, dg0dx, dg0dcb, I)) return 1;
     Tt Q;
     for (int i = 0; i < (int)(sizeof(Tt) / 8); i++) ((double *)&Q)[i] = 0.0;
-    if (T_taylor(T, Q, I)) return 1;
     Vt D;
     for (int i = 0; i < (int)(sizeof(Vt) / 8); i++) ((double *)&D)[i] = 0.0;
-    if (V_taylor(T, D, I.x, I.y, I.xsq, I.flag2, I.cb)) return 1;
+    const int iy = bsrch(T.YY, I.y, 0, T.nstpy);
+    /* the sort keys the Taylor region by y node, so a wavefront's pairs nearly always share it:
+     * then every coefficient array is read by scalar loads */
+    const int iyu = __builtin_amdgcn_readfirstlane(iy);
+    if (CLS == 2 && __ballot(iy != iyu) == 0) {
+      if (S_taylor<true>(T, dg0dy, dg0dx, dg0dcb, I, iyu)) return 1;
+      if (T_taylor<true>(T, Q, I, iyu)) return 1;
+      if (V_taylor<true>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iyu)) return 1;
+    } else {
+      if (S_taylor<false>(T, dg0dy, dg0dx, dg0dcb, I, iy)) return 1;
+      if (T_taylor<false>(T, Q, I, iy)) return 1;
+      if (V_taylor<false>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iy)) return 1;
+    }
     st->dg0dx = dg0dx; st->dg0dy = dg0dy; st->dg0dcb = dg0dcb;
     st->T = Q;
     st->V = D;
