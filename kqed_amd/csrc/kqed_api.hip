@@ -135,7 +135,7 @@ struct kqed_device {
   int device = 0;
   /* launch parameters: pairs per pipeline pass (bounds the record workspace), assembly grid,
    * sort key granularity and the batch size from which the sort is used */
-  kqed_tune tune = { (size_t)1 << 20, 16384, KQED_SORT_CELL, 2048 };
+  kqed_tune tune = { (size_t)1 << 20, 1024, KQED_SORT_CELL, 2048 };
   kqed_dev_tables dt;
   void *allocs[32];
   int nalloc = 0;
@@ -732,7 +732,7 @@ extern "C" int kqed_autotune(const struct QED_kernel_temps *t, int which, double
   double bms = 1e300;
   int rc = 0;
   const size_t chunks[] = { (size_t)1 << 17, (size_t)1 << 18, (size_t)1 << 19, (size_t)1 << 20, (size_t)1 << 21 };
-  const int grids[] = { 4096, 16384, 65536 };
+  const int grids[] = { 512, 1024, 4096 };
   const int modes[] = { KQED_SORT_CELL, KQED_SORT_XNODE };
   for (size_t ci = 0; ci < 5 && !rc; ci++) {
     if (ci > 0 && chunks[ci - 1] >= n) break; /* longer passes than the batch change nothing */
