@@ -195,7 +195,7 @@ typedef struct {
 int kqed_get_tune(const struct QED_kernel_temps *t, kqed_tune *tune);
 int kqed_set_tune(const struct QED_kernel_temps *t, const kqed_tune *tune);
 
-/* Autotuner: times every candidate (pass length 2^17..2^21, assembly grid 512..4096, sort by
+/* Autotuner: times every candidate (pass length 2^18..2^22, assembly grid 512..4096, sort by
  * cell or x node) on the given batch, `reps` timed runs each, installs the fastest and returns it
  * (and its time per run in ms). */
 int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,

@@ -135,7 +135,7 @@ struct kqed_device {
   int device = 0;
   /* launch parameters: pairs per pipeline pass (bounds the record workspace), assembly grid,
    * sort key granularity and the batch size from which the sort is used */
-  kqed_tune tune = { (size_t)1 << 20, 1024, KQED_SORT_CELL, 2048 };
+  kqed_tune tune = { (size_t)1 << 21, 1024, KQED_SORT_CELL, 2048 };
   kqed_dev_tables dt;
   void *allocs[32];
   int nalloc = 0;
@@ -420,7 +420,7 @@ extern "C" int initialise(struct QED_kernel_temps *t)
 
 extern "C" size_t kqed_batch_workspace(size_t n)
 {
-  const size_t c = n < ((size_t)1 << 20) ? n : ((size_t)1 << 20);
+  const size_t c = n < ((size_t)1 << 21) ? n : ((size_t)1 << 21);
   return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (1 + 100 * 4096 + 4096);
 }
 
@@ -731,7 +731,7 @@ extern "C" int kqed_autotune(const struct QED_kernel_temps *t, int which, double
   kqed_tune bt = orig;
   double bms = 1e300;
   int rc = 0;
-  const size_t chunks[] = { (size_t)1 << 17, (size_t)1 << 18, (size_t)1 << 19, (size_t)1 << 20, (size_t)1 << 21 };
+  const size_t chunks[] = { (size_t)1 << 18, (size_t)1 << 19, (size_t)1 << 20, (size_t)1 << 21, (size_t)1 << 22 };
   const int grids[] = { 512, 1024, 4096 };
   const int modes[] = { KQED_SORT_CELL, KQED_SORT_XNODE };
   for (size_t ci = 0; ci < 5 && !rc; ci++) {

@@ -81,13 +81,19 @@ def test_soa_layout_sorted(gpu_full):
 
 
 def test_multi_pass_batch(gpu_full, oracle_full):
-    """A batch longer than one pass (2^20 pairs): sorted once, walked in passes; results equal the
-    unsorted small-batch path pair by pair and the oracle on a sample."""
+    """A batch longer than one pass (passes of 2^19 pairs here): sorted once, walked in passes;
+    results equal the unsorted small-batch path pair by pair and the oracle on a sample."""
     n = (1 << 20) + 3001
     x, y = log_uniform(9, n)
-    got = gpu_full.batch_host(0, x, y)
+    old = gpu_full.get_tune()
+    try:
+        gpu_full.set_tune(chunk=1 << 19)
+        got = gpu_full.batch_host(0, x, y)
+    finally:
+        gpu_full.set_tune(old)
     rng = np.random.default_rng(4)
-    idx = np.concatenate([rng.choice(n, 1500, replace=False), np.arange((1 << 20) - 8, (1 << 20) + 8), [n - 1]])
+    idx = np.concatenate([rng.choice(n, 1500, replace=False), np.arange((1 << 19) - 8, (1 << 19) + 8),
+                          np.arange((1 << 20) - 8, (1 << 20) + 8), [n - 1]])
     small = gpu_full.batch_host(0, x[idx], y[idx])
     assert normwise(got[idx], small).max() <= 1e-15
     ref = oracle_full.batch(0, x[idx[:300]], y[idx[:300]])
