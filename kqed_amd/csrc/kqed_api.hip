@@ -488,22 +488,23 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
   for (size_t c0 = 0; c0 < n; c0 += CH) {
     const size_t c = n - c0 < CH ? n - c0 : CH, c1 = c0 + c;
     const unsigned blocks = (unsigned)((c + 255) / 256);
+    const unsigned iblocks = (unsigned)((c + KQED_INTERP_THREADS - 1) / KQED_INTERP_THREADS);
     prof_begin(d, s, ev, ST_INTERP);
     if (perm) {
       /* count[b] is the start of bucket b: special pairs [0, count[1]), packed path
        * [count[1], count[1 + nstpx*nstpy]), Taylor-x region [count[1 + nstpx*nstpy], n) */
       const int ip = 1 + T.nstpx * T.nstpy;
       const unsigned char *smask = sur ? sur->mask : nullptr;
-      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
+      hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
-      hipLaunchKernelGGL(k_interp<1>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
+      hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
                          which, M, w.rec, w.kinds, smask, fuse);
-      hipLaunchKernelGGL(k_interp<2>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
+      hipLaunchKernelGGL(k_interp<2>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
       if (smask)
         hipLaunchKernelGGL(k_surr_records, dim3((unsigned)((c + 63) / 64)), dim3(256), 0, s, c, w.rec, w.kinds, sur->net);
     } else {
-      hipLaunchKernelGGL(k_interp<0>, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
+      hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
     }
     if (layout == LAYOUT_SCALARS) {

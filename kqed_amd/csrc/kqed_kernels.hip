@@ -1403,7 +1403,7 @@ template <> struct InterpWaves<2> { static constexpr int v = 3; };
  * The record of position i is rec[i - c0]; it carries the input index p.  Pair p is
  * (x[xs*p .. +4], y[ys*p .. +4]): a stride of 0 repeats one vector for every pair. */
 template <int CLS>
-__global__ void __launch_bounds__(256, InterpWaves<CLS>::v)
+__global__ void __launch_bounds__(KQED_INTERP_THREADS, InterpWaves<CLS>::v)
 k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys,
          const unsigned *__restrict__ perm, const unsigned *__restrict__ bounds, int ilo, int ihi, size_t c0, size_t c1,
          int which, double M, double *__restrict__ rec, unsigned *__restrict__ kinds, const unsigned char *smask, int fuse)
