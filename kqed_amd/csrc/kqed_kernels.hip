@@ -233,8 +233,9 @@ DEV void clen_step(double &ya, double &yb, double c, double twox, double PA, dou
     ya = ya - c;
     yb = fma(twox, yb, -ya);
   } else {
+    /* PA is the step's ratio times 2 cos(beta), formed once per step for every chain of the order
+     * (the reference multiplies yb by 2 cos(beta) first: the two forms agree to rounding) */
     ya = fma(PB, ya, -c);
-    yb = yb * twox;
     yb = fma(PA, yb, -ya);
   }
   ya = t;
@@ -275,7 +276,7 @@ DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
         const int d = O0 + i;
         if (d > 0 && k < d) continue;
         double PA = 0.0, PB = 0.0;
-        if (d > 0) { PA = LUT[8 * k + 2 * d - 1]; PB = LUT[8 * k + 2 * d]; }
+        if (d > 0) { PA = LUT[8 * k + 2 * d - 1] * twox; PB = LUT[8 * k + 2 * d]; }
 #pragma unroll
         for (int a = 0; a < 2; a++) {
           if (d == 0) {
@@ -322,7 +323,7 @@ DEV void clen_two(P1 __restrict__ q1, P1 __restrict__ q2, int nk, int d, double 
         clen_step<0>(ya1, yb1, a[u], twox, 0.0, 0.0);
         clen_step<0>(ya2, yb2, b[u], twox, 0.0, 0.0);
       } else {
-        const double PA = LUT[8 * k + 2 * d - 1], PB = LUT[8 * k + 2 * d];
+        const double PA = LUT[8 * k + 2 * d - 1] * twox, PB = LUT[8 * k + 2 * d];
         clen_step<1>(ya1, yb1, a[u], twox, PA, PB);
         clen_step<1>(ya2, yb2, b[u], twox, PA, PB);
       }
