@@ -411,17 +411,18 @@ DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const do
   }
 }
 
-/* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays */
+/* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays, in two
+ * parts: the Clenshaw sums of the value and d/dy arrays at y nodes iy, iy+1 (r = {a.x, a.y, b.x,
+ * b.y}), which depend on (nm, ndcb) only, and their y interpolation.  Calls that differ only in
+ * ndy, use_y_derivs or y share the sums. */
 /* UNI: ix and iy are the same for every active lane of the wavefront (scalar registers), so the
  * coefficient arrays are read through the constant address space: scalar loads */
 template <bool UNI = false>
-DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm, bool ndy, int ndcb,
-                     double cb, double y)
+DEV void sums_s(const DT &T, bool flag_hy, int ix, int iy, int nm, int ndcb, double cb, double r[4])
 {
   const int mm = mm_of(nm);
   const int nk = T.nfx[ix] + mm;
   const int ord = mm + ndcb;
-  double r[4];
   if (UNI) {
     const cdouble *a = (const cdouble *)(T.S + node(T, nm, ix, iy));
     const cdouble *b = flag_hy ? nullptr : (const cdouble *)(T.S + node(T, nm, ix, iy + 1));
@@ -441,6 +442,10 @@ DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int i
     default: cheb_r<3>(nk, cb, a, b, T.LUT, r); break;
     }
   }
+}
+
+DEV double ysum_s(const DT &T, bool flag_hy, bool use_y_derivs, int iy, bool ndy, const double r[4], double y)
+{
   const double r10 = ndy ? r[1] : r[0], r11 = r[1], r20 = ndy ? r[3] : r[2], r21 = r[3];
   if (!flag_hy) {
     const double y1 = T.YY[iy], y2 = T.YY[iy + 1];
@@ -448,6 +453,15 @@ DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int i
     return lerp((y2 - y) / (y2 - y1), r10, r20);
   }
   return r10;
+}
+
+template <bool UNI = false>
+DEV double accessv_s(const DT &T, bool flag_hy, bool use_y_derivs, int ix, int iy, int nm, bool ndy, int ndcb,
+                     double cb, double y)
+{
+  double r[4];
+  sums_s<UNI>(T, flag_hy, ix, iy, nm, ndcb, cb, r);
+  return ysum_s(T, flag_hy, use_y_derivs, iy, ndy, r, y);
 }
 
 DEV double interpol4(const Ip &P, double f1, double f2, double g1, double g2)
@@ -669,23 +683,28 @@ DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy)
   const double ax = (T_.XX[0] - I.x) / (T_.XX[0]);
   const double xb = T_.XX[0], xbsq = xb * xb, cb = I.cb, y = I.y;
   const double ell2a = lerp(ay, TXr(Gl2, iy_tay), TXr(Gl2, i2));
-  const double ell2b = accessv_s<UNI>(T_, false, true, 0, iy, QL2, false, 0, cb, y);
+  double r[4];
+  /* value and d/dy of QL2, QG1 and QL4 come from the same node sums */
+  sums_s<UNI>(T_, false, 0, iy, QL2, 0, cb, r);
+  const double ell2b = ysum_s(T_, false, true, iy, false, r, y);
+  const double dell2bdy = ysum_s(T_, false, false, iy, true, r, y);
   T.ell2 = lerp(ax, ell2a, ell2b);
   const double dell2adx = lerp(ay, TXr(Gl21, iy_tay), TXr(Gl21, i2)) * cb;
   const double dell2bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL2, false, 0, cb, y);
   T.dell2dx = lerp(ax, dell2adx, dell2bdx);
   const double dell2ady = lerp(ay, TXr(Gl2dy, iy_tay), TXr(Gl2dy, i2));
-  const double dell2bdy = accessv_s<UNI>(T_, false, false, 0, iy, QL2, true, 0, cb, y);
   T.dell2dy = lerp(ax, dell2ady, dell2bdy);
   const double dell2bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL2, false, 1, cb, y);
   T.dell2dcb = lerp(ax, 0.0, dell2bdcb);
-  const double v1b = accessv_s<UNI>(T_, false, true, 0, iy, QG1, false, 0, cb, y);
+  sums_s<UNI>(T_, false, 0, iy, QG1, 0, cb, r);
+  const double v1b = ysum_s(T_, false, true, iy, false, r, y);
+  const double dv1bdy = ysum_s(T_, false, false, iy, true, r, y);
   const double dv1bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQG1, false, 0, cb, y);
-  const double dv1bdy = accessv_s<UNI>(T_, false, false, 0, iy, QG1, true, 0, cb, y);
   const double dv1bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QG1, false, 1, cb, y);
-  const double ell4b = accessv_s<UNI>(T_, false, true, 0, iy, QL4, false, 0, cb, y);
+  sums_s<UNI>(T_, false, 0, iy, QL4, 0, cb, r);
+  const double ell4b = ysum_s(T_, false, true, iy, false, r, y);
+  const double dell4bdy = ysum_s(T_, false, false, iy, true, r, y);
   const double dell4bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL4, false, 0, cb, y);
-  const double dell4bdy = accessv_s<UNI>(T_, false, false, 0, iy, QL4, true, 0, cb, y);
   const double dell4bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL4, false, 1, cb, y);
   const double ell3a = lerp(ay, TXr(Gl3, iy_tay), TXr(Gl3, i2));
   const double ell3b = ell4b / (2.0 * xbsq * ysq) - y * cb * ell2b / xb;
@@ -721,19 +740,28 @@ DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2,
   const double ay = (TXr(TYY, i2) - y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
   const double ax = (T.XX[0] - x) / (T.XX[0]);
   const double xb = T.XX[0], xbsq = xb * xb;
-  const double g2b = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 0, cb, y);
+  /* calls that differ only in ndy (and, with flag2, in the interpolation point) share node sums */
+  double r[4], rg2[4];
+  sums_s<UNI>(T, false, 0, iy, QG2, 0, cb, rg2);
+  const double g2b = ysum_s(T, false, true, iy, false, rg2, y);
+  const double dg2bdy = ysum_s(T, false, false, iy, true, rg2, y);
+  /* flag2: the same d/dy sums at the two y nodes (finite difference in y below) */
+  const double dg2bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, rg2, T.YY[iy]) : 0.0;
+  const double dg2bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, rg2, T.YY[iy + 1]) : 0.0;
   D.g2 = lerp(ax, 0.0, g2b);
   const double ddg2adxdcb = lerp(ay, TXr(G21, iy_tay), TXr(G21, i2));
   const double ddg2bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG2, false, 1, cb, y);
   D.ddg2dxdcb = lerp(ax, ddg2adxdcb, ddg2bdxdcb);
-  const double dg2bdx = accessv_s<UNI>(T, false, true, 0, iy, dxQG2, false, 0, cb, y);
+  sums_s<UNI>(T, false, 0, iy, dxQG2, 0, cb, r);
+  const double dg2bdx = ysum_s(T, false, true, iy, false, r, y);
+  const double ddg2bdxdy = ysum_s(T, false, false, iy, true, r, y);
   D.dg2dx = D.ddg2dxdcb * cb;
-  const double dg2bdcb = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 1, cb, y);
+  sums_s<UNI>(T, false, 0, iy, QG2, 1, cb, r);
+  const double dg2bdcb = ysum_s(T, false, true, iy, false, r, y);
+  const double ddg2bdydcb = ysum_s(T, false, false, iy, true, r, y);
   D.dg2dcb = D.ddg2dxdcb * x;
-  const double dg2bdy = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, y);
   D.dg2dy = lerp(ax, 0.0, dg2bdy);
   const double ddg2adxdy = cb * lerp(ay, TXr(G21dy, iy_tay), TXr(G21dy, i2));
-  const double ddg2bdxdy = accessv_s<UNI>(T, false, false, 0, iy, dxQG2, true, 0, cb, y);
   D.ddg2dxdy = lerp(ax, ddg2adxdy, ddg2bdxdy);
   const double ddg2adxdx = 2.0 * (lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2))
                                   + lerp(ay, TXr(G22B, iy_tay), TXr(G22B, i2)) * (0.4 + 0.6 * (2.0 * cb * cb - 1)));
@@ -741,9 +769,12 @@ DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2,
   D.ddg2dxdx = lerp(ax, ddg2adxdx, ddg2bdxdx);
   const double ddg2bdcbdcb = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 2, cb, y);
   D.ddg2dcbdcb = ddg2bdcbdcb * xsq / xbsq;
-  const double ddg2bdydcb = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 1, cb, y);
   D.ddg2dydcb = lerp(ax, 0.0, ddg2bdydcb);
-  const double dg3bdy = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, y);
+  double rg3[4];
+  sums_s<UNI>(T, false, 0, iy, QG3, 0, cb, rg3);
+  const double dg3bdy = ysum_s(T, false, false, iy, true, rg3, y);
+  const double dg3bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, rg3, T.YY[iy]) : 0.0;
+  const double dg3bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, rg3, T.YY[iy + 1]) : 0.0;
   const double ddg3bdxdy = accessv_s<UNI>(T, false, false, 0, iy, dxQG3, true, 0, cb, y);
   const double ddg3bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG3, false, 1, cb, y);
   const double ddg3bdxdx = accessv_s<UNI>(T, false, true, 0, iy, d2xQG3, false, 0, cb, y);
@@ -773,15 +804,10 @@ DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2,
   D.ddg1dcbdcb = ddg1bdcbdcb * xsq / xbsq;
   D.ddg1dydcb = lerp(ax, 0.0, ddg1bdydcb);
   if (flag2) {
-    const double ya = T.YY[iy], yb = T.YY[iy + 1];
-    const double dg2bdy_ya = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, ya);
-    const double dg2bdy_yb = accessv_s<UNI>(T, false, false, 0, iy, QG2, true, 0, cb, yb);
     const double ddg2bdydy = (dg2bdy_yb - dg2bdy_ya) / (T.YY[1] - T.YY[0]);
     D.ddg2dydy = ax * 0.0 + (1.0 - ax) * ddg2bdydy;
     const double ddg1adydy = (TXr(G3Ady, i2) - TXr(G3Bdy, i2) - (TXr(G3Ady, iy_tay) - TXr(G3Bdy, iy_tay)))
       / (TXr(TYY, i2) - TXr(TYY, iy_tay));
-    const double dg3bdy_ya = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, ya);
-    const double dg3bdy_yb = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 0, cb, yb);
     const double ddg3bdydy = (dg3bdy_yb - dg3bdy_ya) / (T.YY[1] - T.YY[0]);
     const double ddg1bdydy = ddg3bdydy - (y * ddg2bdydy + 2.0 * dg2bdy) * cb / x;
     D.ddg1dydy = ax * ddg1adydy + (1.0 - ax) * ddg1bdydy;
