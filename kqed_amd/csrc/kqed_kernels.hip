@@ -241,63 +241,73 @@ DEV void clen_step(double &ya, double &yb, double c, double twox, double PA, dou
   ya = t;
 }
 
-/* Packed-path Clenshaw over the arrays of two grid nodes of one x node (y nodes iy and iy+1) for
- * orders O0 .. O0+NO-1 of both the value (.x) and d/dy (.y) arrays in one pass.  nk is wave-uniform
- * (the largest length among the wave's nodes) and rounded up to whole blocks of U steps: the arrays
- * are zero past each node's own length (up to ks), and zero coefficients leave every chain at exact
- * zero, so a node's sums do not depend on how far above its length the chain starts.  A block's
- * U coefficient loads are issued together, so one memory latency covers U steps. */
+/* Packed-path sums (reference simd.c:79-118, 182-336) over the arrays of two grid nodes of one x
+ * node (y nodes iy and iy+1) for orders O0 .. O0+NO-1 of both the value (.x) and d/dy (.y) arrays
+ * in one pass.  The reference evaluates sum_k c_k phi^d_k(cos beta) by Clenshaw's backward
+ * recurrence, b_k = c_k + 2x PA^d_k b_{k+1} - PB^d_k b_{k+2} (PA, PB: the ratio table; order 0:
+ * PA = PB = 1), result f_d b_d.  Here the same sums are formed directly: the polynomials come
+ * from the forward recurrence that these coefficients define,
+ *   phi_d = f_d, phi_{d-1} = 0, phi_{k+1} = 2x PA_k phi_k - PB_{k-1} phi_{k-1}   (f = 1, 2, 8, 48;
+ * phi^1_k = U'_k, ...), one set per order shared by the four chains, and every chain is one
+ * fused multiply-add per step into its own sum.  The two forms agree to rounding; the
+ * accumulations are independent where Clenshaw's chains are serial, which the scheduler uses.
+ * nk is wave-uniform (the largest length among the wave's nodes) and rounded up to whole blocks
+ * of U steps: the arrays are zero past each node's own length (up to ks), so the extra steps add
+ * exact zeros.  A block's U coefficient loads are issued together, one memory latency per block. */
 template <int O0, int NO, int U, typename P1>
 DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
                     const double *__restrict__ LUTg, double r1[NO][2], double r2[NO][2])
 {
   static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
   const cdouble *LUT = lut_ptr(LUTg);
-  double ya[2][NO][2], yb[2][NO][2];
+  double acc[2][NO][2], ph[NO], pp[NO];
 #pragma unroll
-  for (int i = 0; i < NO; i++)
+  for (int i = 0; i < NO; i++) {
+    ph[i] = 0.0;
+    pp[i] = 0.0;
 #pragma unroll
-    for (int a = 0; a < 2; a++) { ya[a][i][0] = ya[a][i][1] = yb[a][i][0] = yb[a][i][1] = 0.0; }
+    for (int a = 0; a < 2; a++) { acc[a][i][0] = acc[a][i][1] = 0.0; }
+  }
   const int nsteps = nk - O0;
-  const int kt = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0) - 1;
-  for (int kb = kt; kb >= O0; kb -= U) {
+  const int kend = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0);
+  for (int kb = O0; kb < kend; kb += U) {
     double2 cc[U][2];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       /* c1, c2 point at double2 arrays read as pairs of doubles (value, d/dy) */
-      cc[u][0] = make_double2(c1[2 * (kb - u)], c1[2 * (kb - u) + 1]);
-      cc[u][1] = make_double2(c2[2 * (kb - u)], c2[2 * (kb - u) + 1]);
+      cc[u][0] = make_double2(c1[2 * (kb + u)], c1[2 * (kb + u) + 1]);
+      cc[u][1] = make_double2(c2[2 * (kb + u)], c2[2 * (kb + u) + 1]);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int k = kb - u;
+      const int k = kb + u;
 #pragma unroll
       for (int i = 0; i < NO; i++) {
         const int d = O0 + i;
-        if (d > 0 && k < d) continue;
-        double PA = 0.0, PB = 0.0;
-        if (d > 0) { PA = LUT[8 * k + 2 * d - 1] * twox; PB = LUT[8 * k + 2 * d]; }
+        if (k < d) continue;
+        if (k == d) {
+          ph[i] = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
+          pp[i] = 0.0;
+        } else {
+          const double nw = d == 0 ? fma(twox, ph[i], -pp[i])
+                                   : fma(twox * LUT[8 * (k - 1) + 2 * d - 1], ph[i], -(LUT[8 * (k - 2) + 2 * d] * pp[i]));
+          pp[i] = ph[i];
+          ph[i] = nw;
+        }
 #pragma unroll
         for (int a = 0; a < 2; a++) {
-          if (d == 0) {
-            clen_step<0>(ya[a][i][0], yb[a][i][0], cc[u][a].x, twox, PA, PB);
-            clen_step<0>(ya[a][i][1], yb[a][i][1], cc[u][a].y, twox, PA, PB);
-          } else {
-            clen_step<1>(ya[a][i][0], yb[a][i][0], cc[u][a].x, twox, PA, PB);
-            clen_step<1>(ya[a][i][1], yb[a][i][1], cc[u][a].y, twox, PA, PB);
-          }
+          acc[a][i][0] = fma(cc[u][a].x, ph[i], acc[a][i][0]);
+          acc[a][i][1] = fma(cc[u][a].y, ph[i], acc[a][i][1]);
         }
       }
     }
   }
 #pragma unroll
   for (int i = 0; i < NO; i++) {
-    const int d = O0 + i;
-    const double f = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
-    r1[i][0] = d == 0 ? yb[0][i][0] : yb[0][i][0] * f;
-    r1[i][1] = d == 0 ? yb[0][i][1] : yb[0][i][1] * f;
-    r2[i][0] = d == 0 ? yb[1][i][0] : yb[1][i][0] * f;
-    r2[i][1] = d == 0 ? yb[1][i][1] : yb[1][i][1] * f;
+    r1[i][0] = acc[0][i][0];
+    r1[i][1] = acc[0][i][1];
+    r2[i][0] = acc[1][i][0];
+    r2[i][1] = acc[1][i][1];
   }
 }
 
