@@ -311,37 +311,37 @@ DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
   }
 }
 
-/* two packed chains of one order over plain double arrays (same wave-uniform length, blocks of U) */
-
+/* two packed chains of one order over plain double arrays (same wave-uniform length, blocks of U),
+ * as direct sums over the polynomials of order d (see clen_node2) */
 template <int U, typename P1>
 DEV void clen_two(P1 __restrict__ q1, P1 __restrict__ q2, int nk, int d, double twox,
                   const double *__restrict__ LUTg, double &o1, double &o2)
 {
   static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
   const cdouble *LUT = lut_ptr(LUTg);
-  double ya1 = 0.0, yb1 = 0.0, ya2 = 0.0, yb2 = 0.0;
+  double s1 = 0.0, s2 = 0.0, pp = 0.0;
+  double ph = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0; /* phi_d */
   const int nsteps = nk - d;
-  const int kt = d + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0) - 1;
-  for (int kb = kt; kb >= d; kb -= U) {
+  const int kend = d + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0);
+  for (int kb = d; kb < kend; kb += U) {
     double a[U], b[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) { a[u] = q1[kb - u]; b[u] = q2[kb - u]; }
+    for (int u = 0; u < U; u++) { a[u] = q1[kb + u]; b[u] = q2[kb + u]; }
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int k = kb - u;
-      if (d == 0) {
-        clen_step<0>(ya1, yb1, a[u], twox, 0.0, 0.0);
-        clen_step<0>(ya2, yb2, b[u], twox, 0.0, 0.0);
-      } else {
-        const double PA = LUT[8 * k + 2 * d - 1] * twox, PB = LUT[8 * k + 2 * d];
-        clen_step<1>(ya1, yb1, a[u], twox, PA, PB);
-        clen_step<1>(ya2, yb2, b[u], twox, PA, PB);
+      const int k = kb + u;
+      if (k > d) {
+        const double nw = d == 0 ? fma(twox, ph, -pp)
+                                 : fma(twox * LUT[8 * (k - 1) + 2 * d - 1], ph, -(LUT[8 * (k - 2) + 2 * d] * pp));
+        pp = ph;
+        ph = nw;
       }
+      s1 = fma(a[u], ph, s1);
+      s2 = fma(b[u], ph, s2);
     }
   }
-  const double f = d == 0 ? 1.0 : d == 1 ? 2.0 : d == 2 ? 8.0 : 48.0;
-  o1 = d == 0 ? yb1 : yb1 * f;
-  o2 = d == 0 ? yb2 : yb2 * f;
+  o1 = s1;
+  o2 = s2;
 }
 
 /* scalar-path Clenshaw with divisions (reference cheby.c:13-91); comp selects .x or .y */
@@ -378,47 +378,36 @@ DEV double interpol3(double y, double y1, double y2, double dy, double f1, doubl
           (dy * dy * dy));
 }
 
-/* Scalar-path Clenshaw sums (reference cheby.c:13-91) of order D over the value and d/dy arrays of
- * one or two grid nodes in one pass: r = {a.x, a.y, b.x, b.y}.  The step factors (j+1)/j,
- * (j+3)/(j+1), ... of D >= 1 are read from the ratio table instead of dividing at every step, and
- * every step is two fused multiply-adds per chain (the forms agree to rounding). */
+/* Scalar-path sums (reference cheby.c:13-91, which uses Clenshaw's recurrence with a division per
+ * step) of order D over the value and d/dy arrays of one or two grid nodes in one pass:
+ * r = {a.x, a.y, b.x, b.y}.  Formed directly over the polynomials of order D, phi^0_k = U_k,
+ * phi^D_k from the forward recurrence with the ratio table (see clen_node2): one set of
+ * polynomials for the up-to-four chains, one fused multiply-add per chain and step (the forms
+ * agree to rounding). */
 template <int D, typename P1>
 DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const double *LUTg, double r[4])
 {
   /* a, b: double2 arrays (value, d/dy) read as pairs of doubles */
   const cdouble *LUT = lut_ptr(LUTg);
   const int nc = b ? 4 : 2;
-  double ya[4] = { 0.0, 0.0, 0.0, 0.0 }, yb[4] = { 0.0, 0.0, 0.0, 0.0 };
+  double s[4] = { 0.0, 0.0, 0.0, 0.0 };
   const double twox = 2.0 * x;
-  const int jmin = D == 0 ? 1 : D;
-  for (int j = nk - 1; j >= jmin; j--) {
+  double ph = D == 0 ? 1.0 : D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0, pp = 0.0; /* phi_D */
+  for (int j = D; j < nk; j++) {
+    if (j > D) {
+      const double nw = D == 0 ? fma(twox, ph, -pp)
+                               : fma(twox * LUT[8 * (j - 1) + 2 * D - 1], ph, -(LUT[8 * (j - 2) + 2 * D] * pp));
+      pp = ph;
+      ph = nw;
+    }
     const double c[4] = { a[2 * j], a[2 * j + 1], b ? b[2 * j] : 0.0, b ? b[2 * j + 1] : 0.0 };
-    if (D == 0) {
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if (i >= nc) break;
-        const double t = yb[i];
-        yb[i] = fma(twox, yb[i], c[i] - ya[i]);
-        ya[i] = t;
-      }
-    } else {
-      const double PA = twox * LUT[8 * j + 2 * D - 1], PB = LUT[8 * j + 2 * D];
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        if (i >= nc) break;
-        const double t = yb[i];
-        yb[i] = fma(PA, yb[i], fma(-PB, ya[i], c[i]));
-        ya[i] = t;
-      }
+    for (int i = 0; i < 4; i++) {
+      if (i >= nc) break;
+      s[i] = fma(c[i], ph, s[i]);
     }
   }
-  if (D == 0) {
-    const double c[4] = { a[0], a[1], b ? b[0] : 0.0, b ? b[1] : 0.0 };
-    for (int i = 0; i < 4; i++) r[i] = -ya[i] + twox * yb[i] + c[i];
-  } else {
-    const double f = D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0;
-    for (int i = 0; i < 4; i++) r[i] = f * yb[i];
-  }
+  for (int i = 0; i < 4; i++) r[i] = s[i];
 }
 
 /* getff2 + accessv (reference getff-new.c:104-194) over the precomputed scalar arrays, in two
