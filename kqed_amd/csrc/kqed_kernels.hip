@@ -14,6 +14,7 @@
  * reference's AVX+FMA build fuses (simd.c:79-118, 424-435, 54-63; getff-new.c:69-90).
  */
 #include <stdint.h>
+#include <type_traits>
 
 #include "../../include/kqed.h"
 #include "kqed_device.h"
@@ -253,31 +254,36 @@ DEV void clen_step(double &ya, double &yb, double c, double twox, double PA, dou
  * accumulations are independent where Clenshaw's chains are serial, which the scheduler uses.
  * nk is wave-uniform (the largest length among the wave's nodes) and rounded up to whole blocks
  * of U steps: the arrays are zero past each node's own length (up to ks), so the extra steps add
- * exact zeros.  A block's U coefficient loads are issued together, one memory latency per block. */
-template <int O0, int NO, int U, typename P1>
-DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
-                    const double *__restrict__ LUTg, double r1[NO][2], double r2[NO][2])
+ * exact zeros.  A block's U coefficient loads are issued together, one memory latency per block.
+ * NT tables of one x node (equal lengths) run in one loop and share the polynomials. */
+template <int O0, int NO, int NT, int U, typename P1>
+DEV void clen_nodes(P1 const (&c)[NT], int ns, int nk, double twox, const double *__restrict__ LUTg,
+                    double r[NT][2][NO][2])
 {
   static_assert(U <= KQED_CLEN_BLOCK, "block longer than the table padding");
   const cdouble *LUT = lut_ptr(LUTg);
-  double acc[2][NO][2], ph[NO], pp[NO];
+  double acc[NT][2][NO][2], ph[NO], pp[NO];
 #pragma unroll
   for (int i = 0; i < NO; i++) {
     ph[i] = 0.0;
     pp[i] = 0.0;
 #pragma unroll
-    for (int a = 0; a < 2; a++) { acc[a][i][0] = acc[a][i][1] = 0.0; }
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) { acc[t][a][i][0] = acc[t][a][i][1] = 0.0; }
   }
   const int nsteps = nk - O0;
   const int kend = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0);
   for (int kb = O0; kb < kend; kb += U) {
-    double2 cc[U][2];
+    double2 cc[U][NT][2];
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      /* c1, c2 point at double2 arrays read as pairs of doubles (value, d/dy) */
-      cc[u][0] = make_double2(c1[2 * (kb + u)], c1[2 * (kb + u) + 1]);
-      cc[u][1] = make_double2(c2[2 * (kb + u)], c2[2 * (kb + u) + 1]);
-    }
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        /* double2 arrays (value, d/dy) read as pairs of doubles; node iy+1 is ns doubles on */
+        cc[u][t][0] = make_double2(c[t][2 * (kb + u)], c[t][2 * (kb + u) + 1]);
+        cc[u][t][1] = make_double2(c[t][ns + 2 * (kb + u)], c[t][ns + 2 * (kb + u) + 1]);
+      }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int k = kb + u;
@@ -295,24 +301,25 @@ DEV void clen_node2(P1 __restrict__ c1, P1 __restrict__ c2, int nk, double twox,
           ph[i] = nw;
         }
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-          acc[a][i][0] = fma(cc[u][a].x, ph[i], acc[a][i][0]);
-          acc[a][i][1] = fma(cc[u][a].y, ph[i], acc[a][i][1]);
-        }
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+          for (int a = 0; a < 2; a++) {
+            acc[t][a][i][0] = fma(cc[u][t][a].x, ph[i], acc[t][a][i][0]);
+            acc[t][a][i][1] = fma(cc[u][t][a].y, ph[i], acc[t][a][i][1]);
+          }
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < NO; i++) {
-    r1[i][0] = acc[0][i][0];
-    r1[i][1] = acc[0][i][1];
-    r2[i][0] = acc[1][i][0];
-    r2[i][1] = acc[1][i][1];
-  }
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int i = 0; i < NO; i++) { r[t][a][i][0] = acc[t][a][i][0]; r[t][a][i][1] = acc[t][a][i][1]; }
 }
 
 /* two packed chains of one order over plain double arrays (same wave-uniform length, blocks of U),
- * as direct sums over the polynomials of order d (see clen_node2) */
+ * as direct sums over the polynomials of order d (see clen_nodes) */
 template <int U, typename P1>
 DEV void clen_two(P1 __restrict__ q1, P1 __restrict__ q2, int nk, int d, double twox,
                   const double *__restrict__ LUTg, double &o1, double &o2)
@@ -381,7 +388,7 @@ DEV double interpol3(double y, double y1, double y2, double dy, double f1, doubl
 /* Scalar-path sums (reference cheby.c:13-91, which uses Clenshaw's recurrence with a division per
  * step) of order D over the value and d/dy arrays of one or two grid nodes in one pass:
  * r = {a.x, a.y, b.x, b.y}.  Formed directly over the polynomials of order D, phi^0_k = U_k,
- * phi^D_k from the forward recurrence with the ratio table (see clen_node2): one set of
+ * phi^D_k from the forward recurrence with the ratio table (see clen_nodes): one set of
  * polynomials for the up-to-four chains, one fused multiply-add per chain and step (the forms
  * agree to rounding). */
 template <int D, typename P1>
@@ -525,27 +532,38 @@ DEV void interpol5(double F[4], const Ip &P, const double f[12])
   t = f[0] * P.A; t = fma(f[1], P.B, t); t = fma(f[2], P.C1, t); t = fma(f[3], P.C2, t); F[3] = t * P.D;
 }
 
-/* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of one table at NO orders.
- * nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
+/* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of NT tables of equal mm at NO
+ * orders.  nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
+template <int O0, int NO, int NT, bool UNI>
+DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const Inv &I, double twox, double H[NT][NO],
+                      double Lv[NT][NO])
+{
+  constexpr int U = NT * NO <= 2 ? 8 : 4;
+  const int nk = nfw + mm_of(nm[0]);
+  double r[NT][2][NO][2];
+  /* UNI: one grid cell for every active lane, so the coefficient arrays are read by scalar loads */
+  typedef typename std::conditional<UNI, const cdouble *, const double *>::type P1;
+  P1 c[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) c[t] = (P1)(T.P + node(T, nm[t], ixn, I.INVy.idx));
+  clen_nodes<O0, NO, NT, U>(c, 2 * T.ks, nk, twox, T.LUT, r);
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int i = 0; i < NO; i++) {
+      H[t][i] = hy(I.INVy, r[t][0][i][0], r[t][1][i][0], r[t][0][i][1], r[t][1][i][1]);
+      Lv[t][i] = ly(I.INVy, r[t][0][i][1], r[t][1][i][1]);
+    }
+}
+
 template <int O0, int NO, bool UNI>
 DEV void table_xnode(const DT &T, int nm, int ixn, int nfw, const Inv &I, double twox, double H[NO], double Lv[NO])
 {
-  const int nk = nfw + mm_of(nm);
-  double r1[NO][2], r2[NO][2];
-  const double2 *c1 = T.P + node(T, nm, ixn, I.INVy.idx);
-  if (UNI) {
-    /* one grid cell for every active lane: the coefficient arrays are read by scalar loads */
-    const cdouble *u1 = (const cdouble *)c1;
-    clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(u1, u1 + 2 * T.ks, nk, twox, T.LUT, r1, r2);
-  } else {
-    const double *g1 = (const double *)c1;
-    clen_node2<O0, NO, (NO > 2 ? 4 : 8)>(g1, g1 + 2 * T.ks, nk, twox, T.LUT, r1, r2);
-  }
+  const int n1[1] = { nm };
+  double H1[1][NO], L1[1][NO];
+  table_xnodes<O0, NO, 1, UNI>(T, n1, ixn, nfw, I, twox, H1, L1);
 #pragma unroll
-  for (int i = 0; i < NO; i++) {
-    H[i] = hy(I.INVy, r1[i][0], r2[i][0], r1[i][1], r2[i][1]);
-    Lv[i] = ly(I.INVy, r1[i][1], r2[i][1]);
-  }
+  for (int i = 0; i < NO; i++) { H[i] = H1[0][i]; Lv[i] = L1[0][i]; }
 }
 
 /* derivative-only variant (accessv6 with ndy, simd.c:616-650) + x step: extractff(nm, ndy=true) */
@@ -588,12 +606,12 @@ template <int D, bool UNI>
 DEV void simple_group(const DT &T, int nm, const Inv &I, const int nfw[2], double twox, double F[4])
 {
   double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
+  const int nms[2] = { nm, nm + 6 }; /* a table and its x derivative: equal mm */
   for (int xn = 0; xn < 2; xn++) {
-    double H[2], L[2];
-    table_xnode<D, 2, UNI>(T, nm, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
-    Hn[xn][0] = H[0]; Hn[xn][1] = H[1]; Ln[xn] = L[0];
-    table_xnode<D, 2, UNI>(T, nm + 6, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
-    Hd[xn][0] = H[0]; Hd[xn][1] = H[1]; Ld[xn] = L[0];
+    double H[2][2], L[2][2];
+    table_xnodes<D, 2, 2, UNI>(T, nms, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
+    Hn[xn][0] = H[0][0]; Hn[xn][1] = H[0][1]; Ln[xn] = L[0][0];
+    Hd[xn][0] = H[1][0]; Hd[xn][1] = H[1][1]; Ld[xn] = L[1][0];
   }
   request(I.INVx, Hn, Ln, Hd, Ld, F);
 }
@@ -977,9 +995,11 @@ DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, cons
   /* chnr_dV, QG2 family: tables QG2 (orders 1-3), dxQG2 (1-3), d2xQG2 (1-2) */
   {
     double H2[2][3], L2[2][3], H8[2][3], L8[2][3], H12[2][2], L12[2][2];
+    const int n28[2] = { QG2, dxQG2 };
     for (int xn = 0; xn < 2; xn++) {
-      table_xnode<1, 3, UNI>(T, QG2, I.INVx.idx + xn, nfw[xn], I, twox, H2[xn], L2[xn]);
-      table_xnode<1, 3, UNI>(T, dxQG2, I.INVx.idx + xn, nfw[xn], I, twox, H8[xn], L8[xn]);
+      double H[2][3], L[2][3];
+      table_xnodes<1, 3, 2, UNI>(T, n28, I.INVx.idx + xn, nfw[xn], I, twox, H, L);
+      for (int i = 0; i < 3; i++) { H2[xn][i] = H[0][i]; L2[xn][i] = L[0][i]; H8[xn][i] = H[1][i]; L8[xn][i] = L[1][i]; }
       table_xnode<1, 2, UNI>(T, d2xQG2, I.INVx.idx + xn, nfw[xn], I, twox, H12[xn], L12[xn]);
     }
     double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
