@@ -966,11 +966,6 @@ DEV int surr_bin(const DT &T, double x, double y)
   return bu * KQED_SURR_NB + bv;
 }
 
-/* Scalar-load variant of the packed path (one cell per pass of a waterfall loop, every table read
- * a scalar load).  Correct, but measured 3x slower than the vector-load form on the benchmark
- * workload (passes over several cells per wavefront, scalar-load latency per Clenshaw block), so
- * it is off. */
-#define KQED_PACKED_SCALAR_LOADS 0
 
 /* The packed path (simd.c getff6/getff7 + accessv6/7 + extractff/extractff2, FMA build) for one
  * pair: every weight function of chnr_dS/dT/dV into st.  UNI: I's cell (INVx.idx, INVy.idx) and
@@ -1114,36 +1109,23 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
   const bool flag_hx = (I.x >= T.XX[T.nstpx - 1]);
   const bool flag_hy = (I.y >= T.YY[T.nstpy - 1]);
   double F[4];
-  if (CLS == 1 && KQED_PACKED_SCALAR_LOADS) {
-    /* one grid cell at a time per wavefront (cell order makes this one pass almost always): the
-     * cell's indices and Clenshaw lengths are uniform, so every table read is a scalar load */
-    /* Waterfall over the wavefront's cells with an explicit mask of lanes still to serve: each pass
-     * takes the first remaining lane's cell (readlane: uniform) and serves the lanes of that cell.
-     * The mask is loop-carried on purpose: a loop on readfirstlane of a loop-invariant value alone
-     * lets the compiler hoist the readfirstlane and drop the loop. */
-    const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
-    unsigned long long todo = __ballot(1);
-    while (todo) {
-      const int first = (int)__builtin_ctzll(todo);
-      const int cu = __builtin_amdgcn_readlane(cell, first);
-      const unsigned long long same = __ballot(cell == cu);
-      if (cell == cu) {
-        /* the first lane's indices (read straight into scalar registers: indices computed with
-         * vector instructions would turn the table reads back into vector loads) */
-        Inv Iu = I;
-        Iu.INVx.idx = __builtin_amdgcn_readlane(I.INVx.idx, first);
-        Iu.INVy.idx = __builtin_amdgcn_readlane(I.INVy.idx, first);
-        const int nfw[2] = { __builtin_amdgcn_readlane(T.nfx[Iu.INVx.idx], first),
-                             __builtin_amdgcn_readlane(T.nfx[Iu.INVx.idx + 1], first) };
-        packed_section<CLS, true>(T, Iu, st, nfw);
-      }
-      todo &= ~same;
-    }
-  } else if (CLS == 1 || (!flag_hx && !flag_hy)) {
-    /* Clenshaw lengths are made wave-uniform (zero padding keeps each node's sums exact), so the
+  if (CLS == 1 || (!flag_hx && !flag_hy)) {
+    /* sum lengths are made wave-uniform (zero padding keeps each node's sums exact), so the
      * loops and ratio-table reads are scalar */
     const int nfw[2] = { wave_max_active(T.nfx[I.INVx.idx]), wave_max_active(T.nfx[I.INVx.idx + 1]) };
-    packed_section<CLS, false>(T, I, st, nfw);
+    /* the sort keys the packed path by cell, so most wavefronts hold one cell only: then the
+     * cell's indices are wave-uniform and every table read is a scalar load (the other
+     * wavefronts read through vector loads) */
+    const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
+    const int cu = __builtin_amdgcn_readfirstlane(cell);
+    if (CLS == 1 && __ballot(cell != cu) == 0) {
+      Inv Iu = I;
+      Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx);
+      Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
+      packed_section<CLS, true>(T, Iu, st, nfw);
+    } else {
+      packed_section<CLS, false>(T, I, st, nfw);
+    }
   } else if (CLS == 0) {
     /* grid-edge fallback: every extractff2 on the scalar path */
     extractff2_s(T, QG0, 0, I, F); st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
