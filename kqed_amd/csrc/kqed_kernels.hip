@@ -1523,27 +1523,47 @@ __constant__ unsigned char HDESC[19][7] = {
   { HYH, HR(7), HC2, HR(8), HX, HZERO, HONE },     { HYH, HR(10), HC2, HR(11), HX, HZERO, HONE },
   { HYH, HR(13), HC2, HR(14), HX, HZERO, HONE } };
 
+/* Two pairs' values side by side (the two-pair assembly keeps every LDS slot as one D2): the
+ * operators act on both components with the scalar code's operations and order, so a D2
+ * evaluation gives each pair exactly its scalar result. */
+struct __align__(16) D2 { double x, y; };
+DEV D2 operator+(D2 a, D2 b) { return { a.x + b.x, a.y + b.y }; }
+DEV D2 operator-(D2 a, D2 b) { return { a.x - b.x, a.y - b.y }; }
+DEV D2 operator*(D2 a, D2 b) { return { a.x * b.x, a.y * b.y }; }
+DEV D2 operator+(double a, D2 b) { return { a + b.x, a + b.y }; }
+DEV D2 operator+(D2 a, double b) { return { a.x + b, a.y + b }; }
+DEV D2 operator-(double a, D2 b) { return { a - b.x, a - b.y }; }
+DEV D2 operator-(D2 a, double b) { return { a.x - b, a.y - b }; }
+DEV D2 operator*(double a, D2 b) { return { a * b.x, a * b.y }; }
+DEV D2 operator*(D2 a, double b) { return { a.x * b, a.y * b }; }
+DEV D2 operator/(D2 a, double b) { return { a.x / b, a.y / b }; }
+DEV D2 operator+(D2 a) { return a; }
+DEV D2 &operator+=(D2 &a, D2 b) { a = a + b; return a; }
+DEV D2 fma(double a, D2 b, D2 c) { return { fma(a, b.x, c.x), fma(a, b.y, c.y) }; }
+
 /* d: this helper's descriptor packed as bytes U|a|V|b (d.x) and W|c|s (d.y); R: the record in LDS */
-DEV double std_helper(const double *R, int j, uint2 d)
+template <typename V>
+DEV V std_helper(const V *R, int j, uint2 d)
 {
   if (j < 76) {
     const int i = j & 3;
-    const double u = R[(d.x & 0xFF) + i] * R[(d.x >> 8) & 0xFF];
-    const double v = R[((d.x >> 16) & 0xFF) + i] * R[d.x >> 24];
-    const double w = R[(d.y & 0xFF) + i] * R[(d.y >> 8) & 0xFF];
+    const V u = R[(d.x & 0xFF) + i] * R[(d.x >> 8) & 0xFF];
+    const V v = R[((d.x >> 16) & 0xFF) + i] * R[d.x >> 24];
+    const V w = R[(d.y & 0xFF) + i] * R[(d.y >> 8) & 0xFF];
     return ((u + v) + w) * R[(d.y >> 16) & 0xFF];
   }
-  const double *xv = R + KQED_REC_X, *yv = R + KQED_REC_Y;
-  const double rx = R[HRX], ry = R[HRY], cb = R[HR(17)];
+  const V *xv = R + KQED_REC_X, *yv = R + KQED_REC_Y;
+  const V rx = R[HRX], ry = R[HRY], cb = R[HR(17)];
   const int a = (j - 76) >> 2, b = (j - 76) & 3;
-  const double xa = xv[a] * rx, xb = xv[b] * rx, ya = yv[a] * ry, yb = yv[b] * ry;
-  const int d_ab = (a == b);
-  const double xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
+  const V xa = xv[a] * rx, xb = xv[b] * rx, ya = yv[a] * ry, yb = yv[b] * ry;
+  const double d_ab = (a == b);
+  const V xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
   return ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) * rx + (d_ab + xayb * cb - xaxb - yayb) * ry);
 }
 
 /* S/T/V entry c of the standard path: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
-DEV double std_entry(const double *P, const double *xv, const double *yv, const double *h, int c)
+template <typename V>
+DEV V std_entry(const V *P, const V *xv, const V *yv, const V *h, int c)
 {
   if (c >= 192) {
     const int b = c & 3;
@@ -1551,21 +1571,21 @@ DEV double std_entry(const double *P, const double *xv, const double *yv, const 
     return h[4 + b] * P[1] + h[8 + b] * P[2];
   }
   const int alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
-  const int d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
+  const double d_ab = (alf == bet), d_bd = (bet == dta), d_ad = (alf == dta);
   if (c >= 64) {
-    const double xsq = P[18], ysq = P[19], xdoty = P[20];
-    const double t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
-    const double t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
-    const double t3 = xv[alf] * xv[dta] - d_ad * xsq / 4;
-    const double t4 = yv[alf] * yv[dta] - d_ad * ysq / 4;
-    const double t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * xdoty * d_ad;
+    const V xsq = P[18], ysq = P[19], xdoty = P[20];
+    const V t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
+    const V t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
+    const V t3 = xv[alf] * xv[dta] - d_ad * xsq / 4;
+    const V t4 = yv[alf] * yv[dta] - d_ad * ysq / 4;
+    const V t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * xdoty * d_ad;
     if (c < 128) return +t2 * P[3] + t1 * P[5] + t3 * h[52 + bet] + t4 * h[56 + bet] + t5 * h[60 + bet];
     return +t1 * P[4] + t2 * P[5] + t3 * h[64 + bet] + t4 * h[68 + bet] + t5 * h[72 + bet];
   }
-  const double xa = h[alf], xb = h[bet], ya = h[4 + alf], yb = h[4 + bet];
-  const double c4a = h[12 + alf], c2c4 = h[8 + bet] + h[12 + bet];
-  const double xaxb = xa * xb, xayb = xa * yb;
-  double f = d_bd * h[44 + alf];
+  const V xa = h[alf], xb = h[bet], ya = h[4 + alf], yb = h[4 + bet];
+  const V c4a = h[12 + alf], c2c4 = h[8 + bet] + h[12 + bet];
+  const V xaxb = xa * xb, xayb = xa * yb;
+  V f = d_bd * h[44 + alf];
   f += d_ad * h[48 + bet];
   f += h[16 + dta] * xaxb + h[20 + dta] * xayb;
   f += (h[24 + dta] * xb + h[28 + dta] * yb) * c4a;
@@ -1663,6 +1683,25 @@ DEV R wm_dot(const unsigned *__restrict__ e, int lane, const R *Tv)
 }
 
 
+/* wm_dot for two pairs side by side: one entry decode and one 16-byte tensor read per term */
+template <int KB>
+DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const D2 *Tv)
+{
+  unsigned w[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
+  D2 t[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + 2 * (w[k] & 0xFFFFu));
+  D2 s = D2{ 0.0, 0.0 };
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const double c = __hiloint2double((int)(w[k] & 0xFFFF0000u), 0);
+    s = fma(c, t[k], s);
+  }
+  return s;
+}
+
 /* FP64 sum over the 64 lanes of a wavefront (every lane active), result in every lane */
 DEV double wave_sum(double v)
 {
@@ -1681,9 +1720,17 @@ DEV double wave_sum(double v)
 template <typename MT>
 __global__ void __launch_bounds__(64 * KQED_ASM_WAVES)
 k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__restrict__ kinds, void *__restrict__ outv,
-           int layout, int nslots, size_t ntot, const void *__restrict__ Tc, const double *__restrict__ Wc)
+           int layout, int nslots, size_t ntot, const void *__restrict__ Tc, const double *__restrict__ Wc,
+           const unsigned *__restrict__ split, size_t c0)
 {
   typedef MT R_t;
+  /* split: only the records before sorted position split[0] (the special pairs; the two-pair
+   * kernel takes the rest) */
+  if (split) {
+    const size_t sp = split[0] > c0 ? split[0] - c0 : 0;
+    n = sp < n ? sp : n;
+  }
+  if ((size_t)blockIdx.x * KQED_ASM_WAVES >= n) return;
   __shared__ unsigned sE[KQED_WM_MAX];
   __shared__ unsigned short sRow[3 * 384];
   __shared__ double sW[KQED_ASM_WAVES][WREC + WH + 201 + WT];
@@ -1961,9 +2008,120 @@ __global__ void k_combine(int ny, const double *__restrict__ sigma, const double
 }
 
 template __global__ void k_assemble<double>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
-                                            const void *, const double *);
+                                            const void *, const double *, const unsigned *, size_t);
 template __global__ void k_assemble<float>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
-                                           const void *, const double *);
+                                           const void *, const double *, const unsigned *, size_t);
+
+/* Two pairs per wavefront: the standard-path assembly of one evaluation slot (L0) with FP64
+ * output, for the records from sorted position split[0] on (the packed path and the Taylor-x
+ * region, whose kinds are only KIND_STD or KIND_NONE).  Records, helpers, tensor entries and
+ * outputs of the two pairs lie side by side in LDS (D2), so every LDS access, map-entry decode and
+ * index computation serves both pairs; each pair's arithmetic is the one-pair kernel's, in the
+ * same order, so the outputs are bit-identical to it.  The output area aliases the helper and
+ * tensor areas (they are dead once the rows are formed).  layout: 0 out[q][384], 1 out[384][ntot]. */
+template <int AW>
+__global__ void __launch_bounds__(64 * AW)
+k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
+            const unsigned *__restrict__ kinds, double *__restrict__ out, int layout, size_t ntot)
+{
+  __shared__ unsigned sE[64 * KQED_WM_PER_LANE]; /* the standard map (map 0 comes first) */
+  __shared__ unsigned short sRow[384];
+  __shared__ D2 sW[AW][KQED_REC_AUX + WT];
+  size_t lo = split[0] > c0 ? split[0] - c0 : 0;
+  lo = lo < n ? lo : n;
+  const size_t np = (n - lo + 1) / 2; /* record pairs */
+  if ((size_t)blockIdx.x * AW >= np) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 64 * KQED_WM_PER_LANE; i += blockDim.x) sE[i] = T.wmE[i];
+  for (int i = threadIdx.x; i < 384; i += blockDim.x) sRow[i] = T.wmRow[i];
+  __syncthreads();
+  D2 *R = sW[w], *H = R + KQED_REC_AUX, *O = H, *Tv = H + WH;
+  const size_t stride = (size_t)gridDim.x * AW;
+  auto hdesc = [](int g) {
+    const unsigned char *e = HDESC[g < 19 ? g : 0];
+    return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
+  };
+  const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  /* the next two records (header and main payload: 80 doubles) wait in registers */
+  double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+  unsigned ka = KIND_NONE, kb = KIND_NONE;
+  auto load = [&](size_t qq) {
+    const size_t ia = lo + 2 * qq, ib = ia + 1;
+    const double *ra = rec + ia * WREC;
+    a0 = ra[lane];
+    a1 = lane < KQED_REC_AUX - 64 ? ra[lane + 64] : 0.0;
+    ka = kinds[ia];
+    if (ib < n) {
+      const double *rb = ra + WREC;
+      b0 = rb[lane];
+      b1 = lane < KQED_REC_AUX - 64 ? rb[lane + 64] : 0.0;
+      kb = kinds[ib];
+    } else {
+      b0 = b1 = 0.0;
+      kb = KIND_NONE;
+    }
+  };
+  size_t q = (size_t)blockIdx.x * AW + w;
+  if (q < np) load(q);
+  /* outputs wait in O until the next records are staged (loads and stores share one counter) */
+  size_t pa = 0, pb = 0;
+  bool pending = false, pbv = false;
+  auto flush = [&]() {
+    for (int i = lane; i < 384; i += 64) {
+      const D2 o = O[i];
+      if (layout & 1) {
+        out[(size_t)i * ntot + pa] = o.x;
+        if (pbv) out[(size_t)i * ntot + pb] = o.y;
+      } else {
+        out[pa * 384 + i] = o.x;
+        if (pbv) out[pb * 384 + i] = o.y;
+      }
+    }
+  };
+  for (; q < np; q += stride) {
+    R[lane] = D2{ a0, b0 };
+    if (lane < KQED_REC_AUX - 64) R[lane + 64] = D2{ a1, b1 };
+    const unsigned kda = __builtin_amdgcn_readfirstlane(ka) & 7u, kdb = __builtin_amdgcn_readfirstlane(kb) & 7u;
+    const bool hasb = lo + 2 * q + 1 < n;
+    if (pending) flush();
+    if (q + stride < np) load(q + stride);
+    wave_sync();
+    if (lane == 0) Tv[200] = D2{ 0.0, 0.0 };
+    const size_t qa = (size_t)R[KQED_REC_P].x, qb = (size_t)R[KQED_REC_P].y;
+    H[lane] = std_helper(R, lane, hd0);
+    if (lane + 64 < WH) H[lane + 64] = std_helper(R, lane + 64, hd1);
+    wave_sync();
+    for (int c = lane; c < 200; c += 64) Tv[c] = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+    wave_sync();
+    D2 v[6];
+    {
+      const unsigned *e = sE;
+      v[0] = wm_dot2<13>(e, lane, Tv);
+      v[1] = wm_dot2<9>(e + 64 * 13, lane, Tv);
+      v[2] = wm_dot2<6>(e + 64 * 22, lane, Tv);
+      v[3] = wm_dot2<6>(e + 64 * 28, lane, Tv);
+      v[4] = wm_dot2<4>(e + 64 * 34, lane, Tv);
+      v[5] = wm_dot2<3>(e + 64 * 38, lane, Tv);
+    }
+    wave_sync(); /* every lane is done with the helper and tensor areas, which O overlays */
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      D2 o = v[m];
+      if (kda != KIND_STD) o.x = 0.0;
+      if (kdb != KIND_STD) o.y = 0.0;
+      O[sRow[64 * m + lane]] = o;
+    }
+    pa = qa;
+    pb = qb;
+    pbv = hasb;
+    pending = true;
+    wave_sync();
+  }
+  if (pending) flush();
+}
+
+template __global__ void k_assemble2<KQED_ASM2_WAVES>(DT, size_t, const unsigned *, size_t, const double *,
+                                                      const unsigned *, double *, int, size_t);
 
 /* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,
