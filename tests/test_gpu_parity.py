@@ -100,6 +100,31 @@ def test_multi_pass_batch(gpu_full, oracle_full):
     assert normwise(got[idx[:300]], ref).max() <= TOL
 
 
+def test_two_pair_assembly_bit_identical(gpu_full):
+    """Sorted one-slot FP64 batches assemble the packed and Taylor classes two pairs per wavefront
+    (the special pairs keep the one-pair kernel); every output equals the one-pair kernel's on the
+    unsorted batch bit for bit: AoS and SoA, odd sizes, special pairs mixed in, one pass and
+    several passes of odd length."""
+    x, y = log_uniform(21, 5001)
+    x[:3] = 0.0
+    y[3:6] = 0.0
+    x[6:9] = y[6:9]
+    old = gpu_full.get_tune()
+    try:
+        gpu_full.set_tune(sort_min=1 << 30)  # never sorted: every pair through the one-pair kernel
+        ref = gpu_full.batch_host(0, x, y)
+        gpu_full.set_tune(old)
+        got = gpu_full.batch_host(0, x, y)
+        got_s = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_SOA)
+        gpu_full.set_tune(chunk=1001)
+        got_p = gpu_full.batch_host(0, x, y)
+    finally:
+        gpu_full.set_tune(old)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got_s, ref.T)
+    assert np.array_equal(got_p, ref)
+
+
 def test_deterministic_and_order_independent(gpu_full):
     """Each pair's result is independent of the batch around it: the same batch twice, and the batch
     in a shuffled order, give identical per-pair results."""
