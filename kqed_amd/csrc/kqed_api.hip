@@ -500,7 +500,7 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       const unsigned char *smask = sur ? sur->mask : nullptr;
       hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
-      hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
+      hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(KQED_INTERP_THREADS), KQED_INTERP_THREADS / 64 * KQED_LDS_ARRAYS * T.ks * sizeof(double2), s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
                          which, M, w.rec, w.kinds, smask, fuse);
       hipLaunchKernelGGL(k_interp<2>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);

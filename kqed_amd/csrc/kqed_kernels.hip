@@ -532,6 +532,11 @@ DEV void interpol5(double F[4], const Ip &P, const double f[12])
   t = f[0] * P.A; t = fma(f[1], P.B, t); t = fma(f[2], P.C1, t); t = fma(f[3], P.C2, t); F[3] = t * P.D;
 }
 
+/* LDS of the packed-path kernel (dynamic, sized at launch): per wavefront KQED_LDS_ARRAYS node
+ * arrays of ks slots, the staging area of its wave-uniform table reads */
+extern __shared__ double2 kqed_interp_lds[];
+DEV void wave_sync();
+
 /* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of NT tables of equal mm at NO
  * orders.  nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
 template <int O0, int NO, int NT, bool UNI>
@@ -541,12 +546,36 @@ DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const 
   constexpr int U = NT * NO <= 2 ? 8 : 4;
   const int nk = nfw + mm_of(nm[0]);
   double r[NT][2][NO][2];
-  /* UNI: one grid cell for every active lane, so the coefficient arrays are read by scalar loads */
-  typedef typename std::conditional<UNI, const cdouble *, const double *>::type P1;
-  P1 c[NT];
+  if constexpr (UNI) {
+    /* one grid cell for every active lane: the wavefront copies the node arrays it needs (slots
+     * [0, kend) of both y nodes of each table) into its LDS area with coalesced loads, then every
+     * lane reads them there at the same address (broadcast), one LDS latency per block of steps */
+    const int nsteps = nk - O0;
+    const int kend = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0);
+    double2 *wl = kqed_interp_lds + (threadIdx.x >> 6) * (KQED_LDS_ARRAYS * T.ks);
+    const unsigned long long act = __ballot(1);
+    const int na = __popcll(act);
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(act >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)act, 0));
+    wave_sync(); /* the previous call's reads of the area come first */
 #pragma unroll
-  for (int t = 0; t < NT; t++) c[t] = (P1)(T.P + node(T, nm[t], ixn, I.INVy.idx));
-  clen_nodes<O0, NO, NT, U>(c, 2 * T.ks, nk, twox, T.LUT, r);
+    for (int t = 0; t < NT; t++) {
+      const double2 *g = T.P + node(T, nm[t], ixn, I.INVy.idx);
+      for (int j = rank; j < kend; j += na) {
+        wl[2 * t * T.ks + j] = g[j];
+        wl[(2 * t + 1) * T.ks + j] = g[T.ks + j];
+      }
+    }
+    wave_sync();
+    const double *c[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) c[t] = (const double *)(wl + 2 * t * T.ks);
+    clen_nodes<O0, NO, NT, U>(c, 2 * T.ks, nk, twox, T.LUT, r);
+  } else {
+    const double *c[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) c[t] = (const double *)(T.P + node(T, nm[t], ixn, I.INVy.idx));
+    clen_nodes<O0, NO, NT, U>(c, 2 * T.ks, nk, twox, T.LUT, r);
+  }
 #pragma unroll
   for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -1118,14 +1147,17 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
      * wavefronts read through vector loads) */
     const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
     const int cu = __builtin_amdgcn_readfirstlane(cell);
-    if (CLS == 1 && __ballot(cell != cu) == 0) {
-      Inv Iu = I;
-      Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx);
-      Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
-      packed_section<CLS, true>(T, Iu, st, nfw);
-    } else {
-      packed_section<CLS, false>(T, I, st, nfw);
+    bool uni = false;
+    if constexpr (CLS == 1) {
+      if (__ballot(cell != cu) == 0) {
+        Inv Iu = I;
+        Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx);
+        Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
+        packed_section<CLS, true>(T, Iu, st, nfw);
+        uni = true;
+      }
     }
+    if (!uni) packed_section<CLS, false>(T, I, st, nfw);
   } else if (CLS == 0) {
     /* grid-edge fallback: every extractff2 on the scalar path */
     extractff2_s(T, QG0, 0, I, F); st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
