@@ -321,6 +321,12 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     T.nstpy = h->nstpy;
     T.nfmax = h->nfmax;
     T.ks = h->nfmax + 2 + KQED_CLEN_BLOCK - 1;
+    /* the packed-path kernel stages node arrays in LDS: KQED_LDS_ARRAYS of ks double2 per wavefront */
+    if ((size_t)KQED_INTERP_THREADS / 64 * KQED_LDS_ARRAYS * T.ks * sizeof(double2) > 64 * 1024) {
+      set_err("table nfmax %d too large for the packed-path LDS staging (at most %d)", h->nfmax,
+              (int)(64 * 1024 / (KQED_INTERP_THREADS / 64 * KQED_LDS_ARRAYS * sizeof(double2))) - 2 - KQED_CLEN_BLOCK + 1);
+      goto fail;
+    }
     const size_t nodes = (size_t)h->nstpx * h->nstpy;
     const size_t nraw = (size_t)KQED_NFFA * nodes * h->nfmax;
     double *XX = (double *)dev_alloc(d, 8 * h->nstpx), *YY = (double *)dev_alloc(d, 8 * h->nstpy);
