@@ -1806,12 +1806,12 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       if (lane == 0) ((double *)outv)[pq] = pdot;
     } else if (layout & 2) {
       float *out = (float *)outv;
-      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + pq] = (float)O[i];
-      else for (int i = lane; i < 384; i += 64) out[pq * 384 + i] = (float)O[i];
+      if (layout & 1) for (int i = lane; i < 384; i += 64) __builtin_nontemporal_store((float)O[i], out + (size_t)i * ntot + pq);
+      else for (int i = lane; i < 384; i += 64) __builtin_nontemporal_store((float)O[i], out + pq * 384 + i);
     } else {
       double *out = (double *)outv;
-      if (layout & 1) for (int i = lane; i < 384; i += 64) out[(size_t)i * ntot + pq] = O[i];
-      else for (int i = lane; i < 384; i += 64) out[pq * 384 + i] = O[i];
+      if (layout & 1) for (int i = lane; i < 384; i += 64) __builtin_nontemporal_store(O[i], out + (size_t)i * ntot + pq);
+      else for (int i = lane; i < 384; i += 64) __builtin_nontemporal_store(O[i], out + pq * 384 + i);
     }
   };
   for (; p < n; p += stride) {
@@ -1822,9 +1822,9 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
     if (pending) flush();
     if (p + stride < n) {
       const double *rp = rec + (p + stride) * WREC;
-      nx0 = rp[lane];
-      if (lane + 64 < used) nx1 = rp[lane + 64];
-      if (lane + 128 < used) nx2 = rp[lane + 128];
+      nx0 = __builtin_nontemporal_load(rp + lane);
+      if (lane + 64 < used) nx1 = __builtin_nontemporal_load(rp + lane + 64);
+      if (lane + 128 < used) nx2 = __builtin_nontemporal_load(rp + lane + 128);
       nk = kinds[p + stride];
     }
     if (lane == 0) Tv[200] = 0.0;
@@ -2080,13 +2080,13 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   auto load = [&](size_t qq) {
     const size_t ia = lo + 2 * qq, ib = ia + 1;
     const double *ra = rec + ia * WREC;
-    a0 = ra[lane];
-    a1 = lane < KQED_REC_AUX - 64 ? ra[lane + 64] : 0.0;
+    a0 = __builtin_nontemporal_load(ra + lane);
+    a1 = lane < KQED_REC_AUX - 64 ? __builtin_nontemporal_load(ra + lane + 64) : 0.0;
     ka = kinds[ia];
     if (ib < n) {
       const double *rb = ra + WREC;
-      b0 = rb[lane];
-      b1 = lane < KQED_REC_AUX - 64 ? rb[lane + 64] : 0.0;
+      b0 = __builtin_nontemporal_load(rb + lane);
+      b1 = lane < KQED_REC_AUX - 64 ? __builtin_nontemporal_load(rb + lane + 64) : 0.0;
       kb = kinds[ib];
     } else {
       b0 = b1 = 0.0;
@@ -2095,18 +2095,20 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   };
   size_t q = (size_t)blockIdx.x * AW + w;
   if (q < np) load(q);
-  /* outputs wait in O until the next records are staged (loads and stores share one counter) */
+  /* outputs wait in O until the next records are staged (loads and stores share one counter); the
+   * outputs and records are touched once, so they are stored and loaded non-temporally (they would
+   * only push the interpolation's table nodes out of the caches) */
   size_t pa = 0, pb = 0;
   bool pending = false, pbv = false;
   auto flush = [&]() {
     for (int i = lane; i < 384; i += 64) {
       const D2 o = O[i];
       if (layout & 1) {
-        out[(size_t)i * ntot + pa] = o.x;
-        if (pbv) out[(size_t)i * ntot + pb] = o.y;
+        __builtin_nontemporal_store(o.x, out + (size_t)i * ntot + pa);
+        if (pbv) __builtin_nontemporal_store(o.y, out + (size_t)i * ntot + pb);
       } else {
-        out[pa * 384 + i] = o.x;
-        if (pbv) out[pb * 384 + i] = o.y;
+        __builtin_nontemporal_store(o.x, out + pa * 384 + i);
+        if (pbv) __builtin_nontemporal_store(o.y, out + pb * 384 + i);
       }
     }
   };
