@@ -543,7 +543,8 @@ template <int O0, int NO, int NT, bool UNI>
 DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const Inv &I, double twox, double H[NT][NO],
                       double Lv[NT][NO])
 {
-  constexpr int U = NT * NO <= 2 ? 8 : 4;
+  /* steps per block: LDS-staged arrays (UNI) in blocks of 4, vector loads 8 (4 with more chains) */
+  constexpr int U = UNI ? 4 : (NT * NO <= 2 ? 8 : 4);
   const int nk = nfw + mm_of(nm[0]);
   double r[NT][2][NO][2];
   if constexpr (UNI) {
