@@ -62,9 +62,9 @@ __global__ void k_surr_binerr(size_t n, const double *a, const double *b, const 
 template <typename R>
 __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, void *out, int layout,
                            int nslots, size_t ntot, const void *Tc, const double *Wc, const unsigned *split, size_t c0);
-template <int AW>
+template <int AW, typename MT>
 __global__ void k_assemble2(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
-                            const unsigned *kinds, double *out, int layout, size_t ntot);
+                            const unsigned *kinds, void *out, int layout, size_t ntot);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
 __global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
                                    double *W);
@@ -526,21 +526,26 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
     const unsigned grid = (unsigned)(nblk < (size_t)tu.asm_grid ? nblk : (size_t)tu.asm_grid);
     prof_begin(d, s, ev, ST_ASSEMBLE);
-    /* one FP64 evaluation slot in sorted order: the records past the special pairs (kinds
-     * KIND_STD / KIND_NONE only) go to the two-pair kernel, the special ones to the general one */
-    const bool two = perm && nslots[which] == 1 && (layout == KQED_AOS || layout == KQED_SOA) && !Wc;
+    /* one evaluation slot in sorted order: the records past the special pairs (kinds KIND_STD /
+     * KIND_NONE only) go to the two-pair kernel, the special ones to the general one */
+    const bool f32 = layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
+    const bool two = perm && nslots[which] == 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc;
     const unsigned *split = two ? w.count + 1 : nullptr;
-    if (layout == KQED_AOS_F32 || layout == KQED_SOA_F32)
+    if (f32)
       hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
-                         nslots[which], n, Tc, Wc, (const unsigned *)nullptr, (size_t)0);
+                         nslots[which], n, Tc, Wc, split, c0);
     else
       hipLaunchKernelGGL(k_assemble<double>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
                          nslots[which], n, Tc, Wc, split, c0);
     if (two) {
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2_WAVES - 1) / KQED_ASM2_WAVES;
       const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
-      hipLaunchKernelGGL(k_assemble2<KQED_ASM2_WAVES>, dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split, c0, w.rec,
-                         w.kinds, (double *)out_dev, layout, n);
+      if (f32)
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, float>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split, c0,
+                           w.rec, w.kinds, out_dev, layout, n);
+      else
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split,
+                           c0, w.rec, w.kinds, out_dev, layout, n);
     }
     prof_end(d, s, ev);
   }

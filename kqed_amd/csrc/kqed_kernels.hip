@@ -1560,6 +1560,7 @@ __constant__ unsigned char HDESC[19][7] = {
  * operators act on both components with the scalar code's operations and order, so a D2
  * evaluation gives each pair exactly its scalar result. */
 struct __align__(16) D2 { double x, y; };
+struct __align__(8) F2 { float x, y; };
 DEV D2 operator+(D2 a, D2 b) { return { a.x + b.x, a.y + b.y }; }
 DEV D2 operator-(D2 a, D2 b) { return { a.x - b.x, a.y - b.y }; }
 DEV D2 operator*(D2 a, D2 b) { return { a.x * b.x, a.y * b.y }; }
@@ -1733,6 +1734,26 @@ DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const D2 *Tv)
     s = fma(c, t[k], s);
   }
   return s;
+}
+
+/* the same in FP32 (the map of the FP32 layouts): 8-byte elements, so the byte offset is col * 8 */
+template <int KB>
+DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const F2 *Tv)
+{
+  unsigned w[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
+  F2 t[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) t[k] = *(const F2 *)((const char *)Tv + (w[k] & 0xFFFFu));
+  float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const float c = (float)__hiloint2double((int)(w[k] & 0xFFFF0000u), 0);
+    sx = fmaf(c, t[k].x, sx);
+    sy = fmaf(c, t[k].y, sy);
+  }
+  return D2{ (double)sx, (double)sy };
 }
 
 /* FP64 sum over the 64 lanes of a wavefront (every lane active), result in every lane */
@@ -2052,10 +2073,10 @@ template __global__ void k_assemble<float>(DT, size_t, const double *, const uns
  * index computation serves both pairs; each pair's arithmetic is the one-pair kernel's, in the
  * same order, so the outputs are bit-identical to it.  The output area aliases the helper and
  * tensor areas (they are dead once the rows are formed).  layout: 0 out[q][384], 1 out[384][ntot]. */
-template <int AW>
+template <int AW, typename MT>
 __global__ void __launch_bounds__(64 * AW)
 k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
-            const unsigned *__restrict__ kinds, double *__restrict__ out, int layout, size_t ntot)
+            const unsigned *__restrict__ kinds, void *__restrict__ outv, int layout, size_t ntot)
 {
   __shared__ unsigned sE[64 * KQED_WM_PER_LANE]; /* the standard map (map 0 comes first) */
   __shared__ unsigned short sRow[384];
@@ -2068,7 +2089,10 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   for (int i = threadIdx.x; i < 64 * KQED_WM_PER_LANE; i += blockDim.x) sE[i] = T.wmE[i];
   for (int i = threadIdx.x; i < 384; i += blockDim.x) sRow[i] = T.wmRow[i];
   __syncthreads();
-  D2 *R = sW[w], *H = R + KQED_REC_AUX, *O = H, *Tv = H + WH;
+  /* MT: arithmetic of the sparse map (the FP32 layouts run it in FP32, as the one-pair kernel) */
+  typedef typename std::conditional<sizeof(MT) == 8, D2, F2>::type TV;
+  D2 *R = sW[w], *H = R + KQED_REC_AUX, *O = H;
+  TV *Tv = (TV *)(H + WH);
   const size_t stride = (size_t)gridDim.x * AW;
   auto hdesc = [](int g) {
     const unsigned char *e = HDESC[g < 19 ? g : 0];
@@ -2104,12 +2128,13 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   auto flush = [&]() {
     for (int i = lane; i < 384; i += 64) {
       const D2 o = O[i];
+      MT *out = (MT *)outv;
       if (layout & 1) {
-        __builtin_nontemporal_store(o.x, out + (size_t)i * ntot + pa);
-        if (pbv) __builtin_nontemporal_store(o.y, out + (size_t)i * ntot + pb);
+        __builtin_nontemporal_store((MT)o.x, out + (size_t)i * ntot + pa);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + (size_t)i * ntot + pb);
       } else {
-        __builtin_nontemporal_store(o.x, out + pa * 384 + i);
-        if (pbv) __builtin_nontemporal_store(o.y, out + pb * 384 + i);
+        __builtin_nontemporal_store((MT)o.x, out + pa * 384 + i);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + pb * 384 + i);
       }
     }
   };
@@ -2121,12 +2146,15 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     if (pending) flush();
     if (q + stride < np) load(q + stride);
     wave_sync();
-    if (lane == 0) Tv[200] = D2{ 0.0, 0.0 };
+    if (lane == 0) Tv[200] = TV{ 0, 0 };
     const size_t qa = (size_t)R[KQED_REC_P].x, qb = (size_t)R[KQED_REC_P].y;
     H[lane] = std_helper(R, lane, hd0);
     if (lane + 64 < WH) H[lane + 64] = std_helper(R, lane + 64, hd1);
     wave_sync();
-    for (int c = lane; c < 200; c += 64) Tv[c] = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+    for (int c = lane; c < 200; c += 64) {
+      const D2 t = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+      Tv[c] = TV{ (MT)t.x, (MT)t.y };
+    }
     wave_sync();
     D2 v[6];
     {
@@ -2155,8 +2183,10 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   if (pending) flush();
 }
 
-template __global__ void k_assemble2<KQED_ASM2_WAVES>(DT, size_t, const unsigned *, size_t, const double *,
-                                                      const unsigned *, double *, int, size_t);
+template __global__ void k_assemble2<KQED_ASM2_WAVES, double>(DT, size_t, const unsigned *, size_t, const double *,
+                                                              const unsigned *, void *, int, size_t);
+template __global__ void k_assemble2<KQED_ASM2_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
+                                                             const unsigned *, void *, int, size_t);
 
 /* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,

@@ -101,10 +101,10 @@ def test_multi_pass_batch(gpu_full, oracle_full):
 
 
 def test_two_pair_assembly_bit_identical(gpu_full):
-    """Sorted one-slot FP64 batches assemble the packed and Taylor classes two pairs per wavefront
-    (the special pairs keep the one-pair kernel); every output equals the one-pair kernel's on the
-    unsorted batch bit for bit: AoS and SoA, odd sizes, special pairs mixed in, one pass and
-    several passes of odd length."""
+    """Sorted one-slot batches assemble the packed and Taylor classes two pairs per wavefront (the
+    special pairs keep the one-pair kernel); every output equals the one-pair kernel's on the
+    unsorted batch bit for bit: FP64 and FP32, AoS and SoA, odd sizes, special pairs mixed in, one
+    pass and several passes of odd length."""
     x, y = log_uniform(21, 5001)
     x[:3] = 0.0
     y[3:6] = 0.0
@@ -113,9 +113,12 @@ def test_two_pair_assembly_bit_identical(gpu_full):
     try:
         gpu_full.set_tune(sort_min=1 << 30)  # never sorted: every pair through the one-pair kernel
         ref = gpu_full.batch_host(0, x, y)
+        ref32 = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_AOS_F32)
         gpu_full.set_tune(old)
         got = gpu_full.batch_host(0, x, y)
         got_s = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_SOA)
+        got32 = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_AOS_F32)
+        got32_s = gpu_full.batch_host(0, x, y, layout=kqed_amd.KQED_SOA_F32)
         gpu_full.set_tune(chunk=1001)
         got_p = gpu_full.batch_host(0, x, y)
     finally:
@@ -123,6 +126,8 @@ def test_two_pair_assembly_bit_identical(gpu_full):
     assert np.array_equal(got, ref)
     assert np.array_equal(got_s, ref.T)
     assert np.array_equal(got_p, ref)
+    assert got32.dtype == np.float32 and np.array_equal(got32, ref32)
+    assert np.array_equal(got32_s, ref32.T)
 
 
 def test_deterministic_and_order_independent(gpu_full):
