@@ -7,7 +7,7 @@ Workload (the plain-FP64 GPU configuration): 2^24 (x,y) pairs, |x| and |y| log-u
 QED_kernel_L0 over all pairs (inputs resident in HBM, 384 FP64 outputs per pair written to HBM).
 L1, L2, L3 and M-L2 (M = 0.5) rates are measured on the same inputs and reported beside it.
 
-Prints one JSON line.  --gpus N > 1 is refused: this build runs on a single GPU.
+Prints one JSON line.  --gpus N > 1 is refused: the library runs on a single GPU.
 """
 import argparse
 import json
@@ -270,7 +270,7 @@ def main():
                     help="also run one device-to-device copy of MIB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus})", file=sys.stderr)
+        print(f"bench.py: the library runs on one GPU only (asked for --gpus {a.gpus})", file=sys.stderr)
         sys.exit(2)
 
     import torch

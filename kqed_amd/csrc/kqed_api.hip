@@ -1849,7 +1849,7 @@ extern "C" int kqed_surrogate_load(kqed_surrogate *g, const char *path)
             fread(g->h_mask.data(), 1, g->h_mask.size(), f) == g->h_mask.size() && fread(&R, sizeof R, 1, f) == 1 &&
             fread(&val, 4, 1, f) == 1;
   fclose(f);
-  if (!ok) { set_err("%s is not a surrogate file of this build", path); return -1; }
+  if (!ok) { set_err("%s is not a surrogate file of this library", path); return -1; }
   g->R = R;
   g->validated = val != 0;
   if (hipMemcpy(g->P, P.data(), 4 * SURR_NPARAM, hipMemcpyHostToDevice) != hipSuccess ||
