@@ -400,18 +400,31 @@ DEV void cheb_r(int nk, double x, P1 __restrict__ a, P1 __restrict__ b, const do
   double s[4] = { 0.0, 0.0, 0.0, 0.0 };
   const double twox = 2.0 * x;
   double ph = D == 0 ? 1.0 : D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0, pp = 0.0; /* phi_D */
-  for (int j = D; j < nk; j++) {
-    if (j > D) {
-      const double nw = D == 0 ? fma(twox, ph, -pp)
-                               : fma(twox * LUT[8 * (j - 1) + 2 * D - 1], ph, -(LUT[8 * (j - 2) + 2 * D] * pp));
-      pp = ph;
-      ph = nw;
-    }
-    const double c[4] = { a[2 * j], a[2 * j + 1], b ? b[2 * j] : 0.0, b ? b[2 * j + 1] : 0.0 };
+  /* steps in blocks of U with their loads issued together; steps past nk read the zero padding
+   * of the arrays (ks slots, nk + U - 1 < ks) and add exact zeros */
+  constexpr int U = 4;
+  for (int j0 = D; j0 < nk; j0 += U) {
+    double c[U][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      if (i >= nc) break;
-      s[i] = fma(c[i], ph, s[i]);
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + u;
+      c[u][0] = a[2 * j]; c[u][1] = a[2 * j + 1];
+      c[u][2] = b ? b[2 * j] : 0.0; c[u][3] = b ? b[2 * j + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + u;
+      if (j > D) {
+        const double nw = D == 0 ? fma(twox, ph, -pp)
+                                 : fma(twox * LUT[8 * (j - 1) + 2 * D - 1], ph, -(LUT[8 * (j - 2) + 2 * D] * pp));
+        pp = ph;
+        ph = nw;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        if (i >= nc) break;
+        s[i] = fma(c[u][i], ph, s[i]);
+      }
     }
   }
   for (int i = 0; i < 4; i++) r[i] = s[i];
