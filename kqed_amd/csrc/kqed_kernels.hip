@@ -710,9 +710,9 @@ DEV void V_algebra(Vt &D, const Inv &I)
 
 /* Taylor-x branch of chnr_dS (reference chnr_dS.c:10-45) */
 template <bool UNI>
-DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, const Inv &I, int iy)
+DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, const Inv &I, int iy, int ity)
 {
-  const int iy_tay = bsrch(T.TX + TYY * NYTAY, I.y, 0, NYTAY);
+  const int iy_tay = ity;
   if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T.TX;
@@ -732,10 +732,10 @@ DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, cons
 
 /* Taylor-x branch of chnr_dT (reference chnr_dT.c:15-143) */
 template <bool UNI>
-DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy)
+DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy, int ity)
 {
   const double ysq = I.flag2 ? I.xmysq : I.ysq;
-  const int iy_tay = bsrch(T_.TX + TYY * NYTAY, I.y, 0, NYTAY);
+  const int iy_tay = ity;
   if (iy_tay == NYTAY - 1 || iy == T_.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T_.TX;
@@ -791,9 +791,9 @@ DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy)
 
 /* Taylor-x branch of chnr_dV (reference chnr_dV.c:18-174) */
 template <bool UNI>
-DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2, double cb, int iy)
+DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2, double cb, int iy, int ity)
 {
-  const int iy_tay = bsrch(T.TX + TYY * NYTAY, y, 0, NYTAY);
+  const int iy_tay = ity;
   if (iy_tay == NYTAY - 1 || iy == T.nstpy - 1) return 1;
   const int i2 = iy_tay + 1;
   const double *TX = T.TX;
@@ -1129,14 +1129,15 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     /* the sort keys the Taylor region by y node, so a wavefront's pairs nearly always share it:
      * then every coefficient array is read by scalar loads */
     const int iyu = __builtin_amdgcn_readfirstlane(iy);
+    const int ity = bsrch(T.TX + TYY * NYTAY, I.y, 0, NYTAY); /* the Taylor table's y row, once for S, T, V */
     if (CLS == 2 && __ballot(iy != iyu) == 0) {
-      if (S_taylor<true>(T, dg0dy, dg0dx, dg0dcb, I, iyu)) return 1;
-      if (T_taylor<true>(T, Q, I, iyu)) return 1;
-      if (V_taylor<true>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iyu)) return 1;
+      if (S_taylor<true>(T, dg0dy, dg0dx, dg0dcb, I, iyu, ity)) return 1;
+      if (T_taylor<true>(T, Q, I, iyu, ity)) return 1;
+      if (V_taylor<true>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iyu, ity)) return 1;
     } else {
-      if (S_taylor<false>(T, dg0dy, dg0dx, dg0dcb, I, iy)) return 1;
-      if (T_taylor<false>(T, Q, I, iy)) return 1;
-      if (V_taylor<false>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iy)) return 1;
+      if (S_taylor<false>(T, dg0dy, dg0dx, dg0dcb, I, iy, ity)) return 1;
+      if (T_taylor<false>(T, Q, I, iy, ity)) return 1;
+      if (V_taylor<false>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iy, ity)) return 1;
     }
     st->dg0dx = dg0dx; st->dg0dy = dg0dy; st->dg0dcb = dg0dcb;
     st->T = Q;
