@@ -1750,6 +1750,22 @@ DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const D2 *Tv)
   return s;
 }
 
+/* wm_dot2 with the entries already in registers */
+template <int KB>
+DEV D2 wm_dot2r(const unsigned *w, const D2 *Tv)
+{
+  D2 t[KB];
+#pragma unroll
+  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + 2 * (w[k] & 0xFFFFu));
+  D2 s = D2{ 0.0, 0.0 };
+#pragma unroll
+  for (int k = 0; k < KB; k++) {
+    const double c = __hiloint2double((int)(w[k] & 0xFFFF0000u), 0);
+    s = fma(c, t[k], s);
+  }
+  return s;
+}
+
 /* the same in FP32 (the map of the FP32 layouts): 8-byte elements, so the byte offset is col * 8 */
 template <int KB>
 DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const F2 *Tv)
@@ -2113,6 +2129,10 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  /* the first row slot's 13 map entries stay in registers (a longer set spills) */
+  unsigned er[13];
+#pragma unroll
+  for (int k = 0; k < 13; k++) er[k] = sE[64 * k + lane];
   /* the next two records (header and main payload: 80 doubles) wait in registers */
   double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
   unsigned ka = KIND_NONE, kb = KIND_NONE;
@@ -2173,7 +2193,8 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     D2 v[6];
     {
       const unsigned *e = sE;
-      v[0] = wm_dot2<13>(e, lane, Tv);
+      if constexpr (sizeof(MT) == 8) v[0] = wm_dot2r<13>(er, (const D2 *)Tv);
+      else v[0] = wm_dot2<13>(e, lane, Tv);
       v[1] = wm_dot2<9>(e + 64 * 13, lane, Tv);
       v[2] = wm_dot2<6>(e + 64 * 22, lane, Tv);
       v[3] = wm_dot2<6>(e + 64 * 28, lane, Tv);
