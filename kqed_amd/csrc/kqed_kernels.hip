@@ -2096,13 +2096,14 @@ template __global__ void k_assemble<double>(DT, size_t, const double *, const un
 template __global__ void k_assemble<float>(DT, size_t, const double *, const unsigned *, void *, int, int, size_t,
                                            const void *, const double *, const unsigned *, size_t);
 
-/* Two pairs per wavefront: the standard-path assembly of one evaluation slot (L0) with FP64
- * output, for the records from sorted position split[0] on (the packed path and the Taylor-x
+/* Two pairs per wavefront: the standard-path assembly of one evaluation slot (L0) with FP64 or
+ * FP32 output (MT), for the records from sorted position split[0] on (the packed path and the Taylor-x
  * region, whose kinds are only KIND_STD or KIND_NONE).  Records, helpers, tensor entries and
  * outputs of the two pairs lie side by side in LDS (D2), so every LDS access, map-entry decode and
  * index computation serves both pairs; each pair's arithmetic is the one-pair kernel's, in the
  * same order, so the outputs are bit-identical to it.  The output area aliases the helper and
- * tensor areas (they are dead once the rows are formed).  layout: 0 out[q][384], 1 out[384][ntot]. */
+ * tensor areas (they are dead once the rows are formed).  layout bit 0: out[384][ntot] (else
+ * out[q][384]); the element type is MT. */
 template <int AW, typename MT>
 __global__ void __launch_bounds__(64 * AW)
 k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
