@@ -183,6 +183,13 @@ def test_longest_chebyshev_rows(which):
         got = t.batch_host(which, x, y)
         ref = o.batch(which, x, y)
         assert normwise(got, ref).max() < 1e-12
+        # a sorted batch: cell-uniform wavefronts stage these long node arrays in LDS
+        x2, y2 = log_uniform(18, 6000)
+        print(f"wide table sorted which={which} n={len(x2)}", flush=True)
+        got2 = t.batch_host(which, x2, y2)
+        idx = np.random.default_rng(19).choice(len(x2), 300, replace=False)
+        ref2 = o.batch(which, x2[idx], y2[idx])
+        assert normwise(got2[idx], ref2).max() < 1e-12
     finally:
         t.free()
         o.close()
