@@ -552,7 +552,7 @@ DEV void wave_sync();
 
 /* Per x node, the y-stepped values H (Hermite) and L (d/dy lerp) of NT tables of equal mm at NO
  * orders.  nfw: wave-uniform upper bound of nfx over the wave's pairs at this x-node position. */
-template <int O0, int NO, int NT, bool UNI>
+template <int O0, int NO, int NT, int UNI>
 DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const Inv &I, double twox, double H[NT][NO],
                       double Lv[NT][NO])
 {
@@ -560,7 +560,40 @@ DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const 
   constexpr int U = UNI ? 4 : (NT * NO <= 2 ? 8 : 4);
   const int nk = nfw + mm_of(nm[0]);
   double r[NT][2][NO][2];
-  if constexpr (UNI) {
+  if constexpr (UNI == 2) {
+    /* two grid cells among the active lanes (A: the first lane's, B): the arrays of both are staged,
+     * A's at arrays [0, 2NT), B's at [2NT, 4NT); each lane reads its own cell's (two addresses per
+     * LDS read) */
+    static_assert(4 * NT <= KQED_LDS_ARRAYS, "LDS staging area");
+    const int iy = I.INVy.idx;
+    const int ixa = __builtin_amdgcn_readfirstlane(ixn), iya = __builtin_amdgcn_readfirstlane(iy);
+    const bool inb = ixn != ixa || iy != iya;
+    const unsigned long long mb = __ballot(inb);
+    const int fb = mb ? (int)__builtin_ctzll(mb) : 0;
+    const int ixb = __builtin_amdgcn_readlane(ixn, fb), iyb = __builtin_amdgcn_readlane(iy, fb);
+    const int nsteps = nk - O0;
+    const int kend = O0 + (nsteps > 0 ? (nsteps + U - 1) / U * U : 0);
+    double2 *wl = kqed_interp_lds + (threadIdx.x >> 6) * (KQED_LDS_ARRAYS * T.ks);
+    const unsigned long long act = __ballot(1);
+    const int na = __popcll(act);
+    const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(act >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)act, 0));
+    wave_sync(); /* the previous call's reads of the area come first */
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const double2 *ga = T.P + node(T, nm[t], ixa, iya), *gb = T.P + node(T, nm[t], ixb, iyb);
+      for (int j = rank; j < kend; j += na) {
+        wl[2 * t * T.ks + j] = ga[j];
+        wl[(2 * t + 1) * T.ks + j] = ga[T.ks + j];
+        wl[(2 * NT + 2 * t) * T.ks + j] = gb[j];
+        wl[(2 * NT + 2 * t + 1) * T.ks + j] = gb[T.ks + j];
+      }
+    }
+    wave_sync();
+    const double *c[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) c[t] = (const double *)(wl + ((inb ? 2 * NT : 0) + 2 * t) * T.ks);
+    clen_nodes<O0, NO, NT, U>(c, 2 * T.ks, nk, twox, T.LUT, r);
+  } else if constexpr (UNI == 1) {
     /* one grid cell for every active lane: the wavefront copies the node arrays it needs (slots
      * [0, kend) of both y nodes of each table) into its LDS area with coalesced loads, then every
      * lane reads them there at the same address (broadcast), one LDS latency per block of steps */
@@ -599,7 +632,7 @@ DEV void table_xnodes(const DT &T, const int (&nm)[NT], int ixn, int nfw, const 
     }
 }
 
-template <int O0, int NO, bool UNI>
+template <int O0, int NO, int UNI>
 DEV void table_xnode(const DT &T, int nm, int ixn, int nfw, const Inv &I, double twox, double H[NO], double Lv[NO])
 {
   const int n1[1] = { nm };
@@ -645,7 +678,7 @@ DEV void request(const Ip &INVx, const double Hn[2][2], const double Ln[2], cons
 }
 
 /* Simple groups: one request (nm, nm+6) at orders d, d+1 (d+1 < 4 always here). */
-template <int D, bool UNI>
+template <int D, int UNI>
 DEV void simple_group(const DT &T, int nm, const Inv &I, const int nfw[2], double twox, double F[4])
 {
   double Hn[2][2], Ln[2], Hd[2][2], Ld[2];
@@ -1013,7 +1046,7 @@ DEV int surr_bin(const DT &T, double x, double y)
 /* The packed path (simd.c getff6/getff7 + accessv6/7 + extractff/extractff2, FMA build) for one
  * pair: every weight function of chnr_dS/dT/dV into st.  UNI: I's cell (INVx.idx, INVy.idx) and
  * nfw are the same in every active lane, so table addresses are scalar. */
-template <int CLS, bool UNI>
+template <int CLS, int UNI>
 DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, const int nfw[2])
 {
   Tt &Q = st->T;
@@ -1088,13 +1121,13 @@ DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, cons
     D.dg3dcb = F[3]; D.ddg3dydcb = F[2]; D.ddg3dcbdcb = F[0];
   }
   /* extractff(QG3, ndy, d0cb): derivative-only arrays (QG3 = 2, dxQG3 = 3) */
-  const double dg3dy = ndy_packed<UNI>(T, 2, 3, QG3, I, nfw, twox, 0);
+  const double dg3dy = ndy_packed<UNI == 1>(T, 2, 3, QG3, I, nfw, twox, 0);
   D.dg3dy = dg3dy;
   if (I.flag2) {
     /* reference chnr_dV.c:444-466: both finite-difference ends reuse the stencil weights of y,
      * so they coincide unless the upper end sits on the top y node */
     const int iy1 = I.INVy.idx;
-    const double fq1 = ndy_packed<UNI>(T, 0, 1, QG2, I, nfw, twox, 1);
+    const double fq1 = ndy_packed<UNI == 1>(T, 0, 1, QG2, I, nfw, twox, 1);
     double fq2 = fq1, gq2 = dg3dy;
     if (CLS == 0 && iy1 + 1 == T.nstpy - 1) {
       Inv I2 = I;
@@ -1164,15 +1197,23 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     const int cu = __builtin_amdgcn_readfirstlane(cell);
     bool uni = false;
     if constexpr (CLS == 1) {
-      if (__ballot(cell != cu) == 0) {
+      const unsigned long long other = __ballot(cell != cu);
+      if (other == 0) {
         Inv Iu = I;
         Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx);
         Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
-        packed_section<CLS, true>(T, Iu, st, nfw);
+        packed_section<CLS, 1>(T, Iu, st, nfw);
         uni = true;
+      } else {
+        /* two cells (the sorted order's cell boundary inside the wavefront): both staged in LDS */
+        const int c2 = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(other));
+        if (__ballot(cell != cu && cell != c2) == 0) {
+          packed_section<CLS, 2>(T, I, st, nfw);
+          uni = true;
+        }
       }
     }
-    if (!uni) packed_section<CLS, false>(T, I, st, nfw);
+    if (!uni) packed_section<CLS, 0>(T, I, st, nfw);
   } else if (CLS == 0) {
     /* grid-edge fallback: every extractff2 on the scalar path */
     extractff2_s(T, QG0, 0, I, F); st->dg0dcb = F[0]; st->dg0dx = F[1]; st->dg0dy = F[2];
