@@ -1590,7 +1590,8 @@ template __global__ void k_interp<2>(DT, size_t, const double *, int, const doub
  *   0 xh, 1 yh, 2 c2, 3 c4, 4..8 x P[26+2k] + y P[27+2k], 9..10 (x P[21+2k] + y P[22+2k])/|x|,
  *   11 (P21+P22) xh + (P23+P24) c4, 12 P21 xh + P25 yh + P23 e12,
  *   13..15 xh P[6+3k] + c4 P[8+3k], 16..18 yh P[7+3k] + c2 P[8+3k],
- * then 16 quadratic ones, j = 76 + 4a + b.  Groups 0..18 are all ((U_i a + V_i b) + W_i c) s with
+ * then 16 quadratic ones, j = 76 + 4a + b, which std_entry forms where it uses them (from the
+ * unit vectors it reads anyway: no divergent helper branch).  Groups 0..18 are all ((U_i a + V_i b) + W_i c) s with
  * the vectors U, V, W and scalars a, b, c, s read from the record (offsets from its start, HDESC),
  * so the 64 lanes run one instruction stream; the vectors xh, yh, c2, c4, e12 and the sums come
  * from the algebra kernel, and every helper keeps its expression's order of operations. */
@@ -1630,24 +1631,16 @@ DEV D2 operator+(D2 a) { return a; }
 DEV D2 &operator+=(D2 &a, D2 b) { a = a + b; return a; }
 DEV D2 fma(double a, D2 b, D2 c) { return { fma(a, b.x, c.x), fma(a, b.y, c.y) }; }
 
-/* d: this helper's descriptor packed as bytes U|a|V|b (d.x) and W|c|s (d.y); R: the record in LDS */
+/* group helper j < 76; d: its descriptor packed as bytes U|a|V|b (d.x) and W|c|s (d.y); R: the
+ * record in LDS */
 template <typename V>
 DEV V std_helper(const V *R, int j, uint2 d)
 {
-  if (j < 76) {
-    const int i = j & 3;
-    const V u = R[(d.x & 0xFF) + i] * R[(d.x >> 8) & 0xFF];
-    const V v = R[((d.x >> 16) & 0xFF) + i] * R[d.x >> 24];
-    const V w = R[(d.y & 0xFF) + i] * R[(d.y >> 8) & 0xFF];
-    return ((u + v) + w) * R[(d.y >> 16) & 0xFF];
-  }
-  const V *xv = R + KQED_REC_X, *yv = R + KQED_REC_Y;
-  const V rx = R[HRX], ry = R[HRY], cb = R[HR(17)];
-  const int a = (j - 76) >> 2, b = (j - 76) & 3;
-  const V xa = xv[a] * rx, xb = xv[b] * rx, ya = yv[a] * ry, yb = yv[b] * ry;
-  const double d_ab = (a == b);
-  const V xaxb = xa * xb, yayb = ya * yb, xayb = xa * yb, yaxb = ya * xb;
-  return ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) * rx + (d_ab + xayb * cb - xaxb - yayb) * ry);
+  const int i = j & 3;
+  const V u = R[(d.x & 0xFF) + i] * R[(d.x >> 8) & 0xFF];
+  const V v = R[((d.x >> 16) & 0xFF) + i] * R[d.x >> 24];
+  const V w = R[(d.y & 0xFF) + i] * R[(d.y >> 8) & 0xFF];
+  return ((u + v) + w) * R[(d.y >> 16) & 0xFF];
 }
 
 /* S/T/V entry c of the standard path: 0-63 Vv, 64-127 Txv, 128-191 Tyv, 192-195 Sxv, 196-199 Syv */
@@ -1680,7 +1673,12 @@ DEV V std_entry(const V *P, const V *xv, const V *yv, const V *h, int c)
   f += (h[24 + dta] * xb + h[28 + dta] * yb) * c4a;
   f += (h[24 + dta] * xa + h[32 + dta] * c4a) * c2c4;
   f += h[36 + dta] * (d_ab - xaxb);
-  return f + h[40 + dta] * h[76 + 4 * alf + bet];
+  /* the quadratic helper (alf, bet) from the values at hand: the unit vectors x/|x|, y/|y| at alf
+   * and bet are h[0..8), so this is the helper's own expression with its own operands */
+  const V cb = P[17], rx = P[KQED_HELP_BASE + 22], ry = P[KQED_HELP_BASE + 23];
+  const V yayb = ya * yb, yaxb = ya * xb;
+  const V quad = ((cb * (3.0 * xaxb - d_ab) - xayb - yaxb) * rx + (d_ab + xayb * cb - xaxb - yayb) * ry);
+  return f + h[40 + dta] * quad;
 }
 
 /* Tabd_xeq0 tensors (reference Tabd.c:71-111); h = y/|y|.  c: 0-63 tI, 64-127 tII, 128-191 tIII */
@@ -1740,6 +1738,7 @@ DEV double yeq0_entry(const double *P, const double *h, int c)
  * as 384 contiguous doubles (6 coalesced 512-byte stores per lane set). */
 #define WREC KQED_REC_LEN
 #define WH 92
+#define WHG 76 /* helpers formed in the helper phase; the 16 quadratic ones are formed where used */
 #define WT 384
 
 DEV void wave_sync()
@@ -1962,7 +1961,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       const double *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
       if (kd == KIND_STD) {
         H[lane] = std_helper(R, lane, hd0);
-        if (lane + 64 < WH) H[lane + 64] = std_helper(R, lane + 64, hd1);
+        if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
       } else if (lane < 4) {
         H[lane] = pl[lane] / pl[4];
       }
@@ -2225,7 +2224,7 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     if (lane == 0) Tv[200] = TV{ 0, 0 };
     const size_t qa = (size_t)R[KQED_REC_P].x, qb = (size_t)R[KQED_REC_P].y;
     H[lane] = std_helper(R, lane, hd0);
-    if (lane + 64 < WH) H[lane + 64] = std_helper(R, lane + 64, hd1);
+    if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
     wave_sync();
     for (int c = lane; c < 200; c += 64) {
       const D2 t = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
