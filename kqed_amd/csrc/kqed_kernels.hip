@@ -463,6 +463,71 @@ DEV void sums_s(const DT &T, bool flag_hy, int ix, int iy, int nm, int ndcb, dou
   }
 }
 
+/* sums_s for NT tables of equal mm at once (Taylor-x region: x node 0, y nodes iy and iy+1): one
+ * set of polynomials for the 4 NT chains, the tables' coefficients loaded U steps at a time; each
+ * chain's sum is sums_s's bit for bit (the extra steps past nk read zero padding) */
+template <int D, int NT, int U, typename P1>
+DEV void cheb_rn(int nk, double x, P1 const (&a)[NT], int ns, const double *LUTg, double r[NT][4])
+{
+  const cdouble *LUT = lut_ptr(LUTg);
+  double s[NT][4];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) s[t][i] = 0.0;
+  const double twox = 2.0 * x;
+  double ph = D == 0 ? 1.0 : D == 1 ? 2.0 : D == 2 ? 8.0 : 48.0, pp = 0.0;
+  for (int j0 = D; j0 < nk; j0 += U) {
+    double c[U][NT][4];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int j = j0 + u;
+        c[u][t][0] = a[t][2 * j]; c[u][t][1] = a[t][2 * j + 1];
+        c[u][t][2] = a[t][ns + 2 * j]; c[u][t][3] = a[t][ns + 2 * j + 1];
+      }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int j = j0 + u;
+      if (j > D) {
+        const double nw = D == 0 ? fma(twox, ph, -pp)
+                                 : fma(twox * LUT[8 * (j - 1) + 2 * D - 1], ph, -(LUT[8 * (j - 2) + 2 * D] * pp));
+        pp = ph;
+        ph = nw;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) s[t][i] = fma(c[u][t][i], ph, s[t][i]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[t][i] = s[t][i];
+}
+
+template <int NT, bool UNI>
+DEV void tsums(const DT &T, const int (&nm)[NT], int ndcb, int iy, double cb, double r[NT][4])
+{
+  constexpr int U = NT == 1 ? 4 : 2;
+  const int mm = mm_of(nm[0]);
+  const int nk = T.nfx[0] + mm;
+  const int ord = mm + ndcb;
+  typedef typename std::conditional<UNI, const cdouble *, const double *>::type P1;
+  P1 a[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) a[t] = (P1)(T.S + node(T, nm[t], 0, iy));
+  const int ns = 2 * T.ks;
+  switch (ord) {
+  case 0: cheb_rn<0, NT, U>(nk, cb, a, ns, T.LUT, r); break;
+  case 1: cheb_rn<1, NT, U>(nk, cb, a, ns, T.LUT, r); break;
+  case 2: cheb_rn<2, NT, U>(nk, cb, a, ns, T.LUT, r); break;
+  default: cheb_rn<3, NT, U>(nk, cb, a, ns, T.LUT, r); break;
+  }
+}
+
 DEV double ysum_s(const DT &T, bool flag_hy, bool use_y_derivs, int iy, bool ndy, const double r[4], double y)
 {
   const double r10 = ndy ? r[1] : r[0], r11 = r[1], r20 = ndy ? r[3] : r[2], r21 = r[3];
@@ -752,13 +817,17 @@ DEV int S_taylor(const DT &T, double &dg0dy, double &dg0dx, double &dg0dcb, cons
 #define TXr(r, i) TX[(r) * NYTAY + (i)]
   const double ay = (TXr(TYY, i2) - I.y) / (TXr(TYY, i2) - TXr(TYY, iy_tay));
   const double ax = (T.XX[0] - I.x) / (T.XX[0]);
+  double r0[2][4], r1[1][4];
+  const int n0[2] = { dxQG0, QG0 }, n1[1] = { QG0 };
+  tsums<2, UNI>(T, n0, 0, iy, I.cb, r0);
+  tsums<1, UNI>(T, n1, 1, iy, I.cb, r1);
   double f1 = I.cb * lerp(ay, TXr(G0dx, iy_tay), TXr(G0dx, i2));
-  double f2 = accessv_s<UNI>(T, false, true, 0, iy, dxQG0, false, 0, I.cb, I.y);
+  double f2 = ysum_s(T, false, true, iy, false, r0[0], I.y);
   dg0dx = lerp(ax, f1, f2);
   f1 = lerp(ay, TXr(G0dy, iy_tay), TXr(G0dy, i2));
-  f2 = accessv_s<UNI>(T, false, false, 0, iy, QG0, true, 0, I.cb, I.y);
+  f2 = ysum_s(T, false, false, iy, true, r0[1], I.y);
   dg0dy = lerp(ax, f1, f2);
-  f2 = accessv_s<UNI>(T, false, true, 0, iy, QG0, false, 1, I.cb, I.y);
+  f2 = ysum_s(T, false, true, iy, false, r1[0], I.y);
   dg0dcb = lerp(ax, 0.0, f2);
   return 0;
 }
@@ -776,29 +845,34 @@ DEV int T_taylor(const DT &T_, Tt &T, const Inv &I, int iy, int ity)
   const double ax = (T_.XX[0] - I.x) / (T_.XX[0]);
   const double xb = T_.XX[0], xbsq = xb * xb, cb = I.cb, y = I.y;
   const double ell2a = lerp(ay, TXr(Gl2, iy_tay), TXr(Gl2, i2));
-  double r[4];
-  /* value and d/dy of QL2, QG1 and QL4 come from the same node sums */
-  sums_s<UNI>(T_, false, 0, iy, QL2, 0, cb, r);
-  const double ell2b = ysum_s(T_, false, true, iy, false, r, y);
-  const double dell2bdy = ysum_s(T_, false, false, iy, true, r, y);
+  /* node sums: a table and its x derivative (equal mm) in one loop; value and d/dy from the same sums */
+  double r2[2][4], r1[1][4];
+  const int nl2[2] = { QL2, dxQL2 }, ng1[2] = { QG1, dxQG1 }, nl4[2] = { QL4, dxQL4 };
+  const int ol2[1] = { QL2 }, og1[1] = { QG1 }, ol4[1] = { QL4 };
+  tsums<2, UNI>(T_, nl2, 0, iy, cb, r2);
+  const double ell2b = ysum_s(T_, false, true, iy, false, r2[0], y);
+  const double dell2bdy = ysum_s(T_, false, false, iy, true, r2[0], y);
+  const double dell2bdx = ysum_s(T_, false, true, iy, false, r2[1], y);
   T.ell2 = lerp(ax, ell2a, ell2b);
   const double dell2adx = lerp(ay, TXr(Gl21, iy_tay), TXr(Gl21, i2)) * cb;
-  const double dell2bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL2, false, 0, cb, y);
   T.dell2dx = lerp(ax, dell2adx, dell2bdx);
   const double dell2ady = lerp(ay, TXr(Gl2dy, iy_tay), TXr(Gl2dy, i2));
   T.dell2dy = lerp(ax, dell2ady, dell2bdy);
-  const double dell2bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL2, false, 1, cb, y);
+  tsums<1, UNI>(T_, ol2, 1, iy, cb, r1);
+  const double dell2bdcb = ysum_s(T_, false, true, iy, false, r1[0], y);
   T.dell2dcb = lerp(ax, 0.0, dell2bdcb);
-  sums_s<UNI>(T_, false, 0, iy, QG1, 0, cb, r);
-  const double v1b = ysum_s(T_, false, true, iy, false, r, y);
-  const double dv1bdy = ysum_s(T_, false, false, iy, true, r, y);
-  const double dv1bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQG1, false, 0, cb, y);
-  const double dv1bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QG1, false, 1, cb, y);
-  sums_s<UNI>(T_, false, 0, iy, QL4, 0, cb, r);
-  const double ell4b = ysum_s(T_, false, true, iy, false, r, y);
-  const double dell4bdy = ysum_s(T_, false, false, iy, true, r, y);
-  const double dell4bdx = accessv_s<UNI>(T_, false, true, 0, iy, dxQL4, false, 0, cb, y);
-  const double dell4bdcb = accessv_s<UNI>(T_, false, true, 0, iy, QL4, false, 1, cb, y);
+  tsums<2, UNI>(T_, ng1, 0, iy, cb, r2);
+  const double v1b = ysum_s(T_, false, true, iy, false, r2[0], y);
+  const double dv1bdy = ysum_s(T_, false, false, iy, true, r2[0], y);
+  const double dv1bdx = ysum_s(T_, false, true, iy, false, r2[1], y);
+  tsums<1, UNI>(T_, og1, 1, iy, cb, r1);
+  const double dv1bdcb = ysum_s(T_, false, true, iy, false, r1[0], y);
+  tsums<2, UNI>(T_, nl4, 0, iy, cb, r2);
+  const double ell4b = ysum_s(T_, false, true, iy, false, r2[0], y);
+  const double dell4bdy = ysum_s(T_, false, false, iy, true, r2[0], y);
+  const double dell4bdx = ysum_s(T_, false, true, iy, false, r2[1], y);
+  tsums<1, UNI>(T_, ol4, 1, iy, cb, r1);
+  const double dell4bdcb = ysum_s(T_, false, true, iy, false, r1[0], y);
   const double ell3a = lerp(ay, TXr(Gl3, iy_tay), TXr(Gl3, i2));
   const double ell3b = ell4b / (2.0 * xbsq * ysq) - y * cb * ell2b / xb;
   T.ell3 = lerp(ax, ell3a, ell3b);
@@ -834,45 +908,48 @@ DEV int V_taylor(const DT &T, Vt &D, double x, double y, double xsq, bool flag2,
   const double ax = (T.XX[0] - x) / (T.XX[0]);
   const double xb = T.XX[0], xbsq = xb * xb;
   /* calls that differ only in ndy (and, with flag2, in the interpolation point) share node sums */
-  double r[4], rg2[4];
-  sums_s<UNI>(T, false, 0, iy, QG2, 0, cb, rg2);
-  const double g2b = ysum_s(T, false, true, iy, false, rg2, y);
-  const double dg2bdy = ysum_s(T, false, false, iy, true, rg2, y);
+  double r3[3][4], r2[2][4], r1[1][4];
+  const int ng2[3] = { QG2, dxQG2, d2xQG2 }, ng2c[2] = { dxQG2, QG2 }, og2[1] = { QG2 };
+  tsums<3, UNI>(T, ng2, 0, iy, cb, r3);
+  const double g2b = ysum_s(T, false, true, iy, false, r3[0], y);
+  const double dg2bdy = ysum_s(T, false, false, iy, true, r3[0], y);
   /* flag2: the same d/dy sums at the two y nodes (finite difference in y below) */
-  const double dg2bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, rg2, T.YY[iy]) : 0.0;
-  const double dg2bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, rg2, T.YY[iy + 1]) : 0.0;
+  const double dg2bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, r3[0], T.YY[iy]) : 0.0;
+  const double dg2bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, r3[0], T.YY[iy + 1]) : 0.0;
+  const double dg2bdx = ysum_s(T, false, true, iy, false, r3[1], y);
+  const double ddg2bdxdy = ysum_s(T, false, false, iy, true, r3[1], y);
+  const double ddg2bdxdx = ysum_s(T, false, true, iy, false, r3[2], y);
+  tsums<2, UNI>(T, ng2c, 1, iy, cb, r2);
+  const double ddg2bdxdcb = ysum_s(T, false, true, iy, false, r2[0], y);
+  const double dg2bdcb = ysum_s(T, false, true, iy, false, r2[1], y);
+  const double ddg2bdydcb = ysum_s(T, false, false, iy, true, r2[1], y);
+  tsums<1, UNI>(T, og2, 2, iy, cb, r1);
+  const double ddg2bdcbdcb = ysum_s(T, false, true, iy, false, r1[0], y);
   D.g2 = lerp(ax, 0.0, g2b);
   const double ddg2adxdcb = lerp(ay, TXr(G21, iy_tay), TXr(G21, i2));
-  const double ddg2bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG2, false, 1, cb, y);
   D.ddg2dxdcb = lerp(ax, ddg2adxdcb, ddg2bdxdcb);
-  sums_s<UNI>(T, false, 0, iy, dxQG2, 0, cb, r);
-  const double dg2bdx = ysum_s(T, false, true, iy, false, r, y);
-  const double ddg2bdxdy = ysum_s(T, false, false, iy, true, r, y);
   D.dg2dx = D.ddg2dxdcb * cb;
-  sums_s<UNI>(T, false, 0, iy, QG2, 1, cb, r);
-  const double dg2bdcb = ysum_s(T, false, true, iy, false, r, y);
-  const double ddg2bdydcb = ysum_s(T, false, false, iy, true, r, y);
   D.dg2dcb = D.ddg2dxdcb * x;
   D.dg2dy = lerp(ax, 0.0, dg2bdy);
   const double ddg2adxdy = cb * lerp(ay, TXr(G21dy, iy_tay), TXr(G21dy, i2));
   D.ddg2dxdy = lerp(ax, ddg2adxdy, ddg2bdxdy);
   const double ddg2adxdx = 2.0 * (lerp(ay, TXr(G22A, iy_tay), TXr(G22A, i2))
                                   + lerp(ay, TXr(G22B, iy_tay), TXr(G22B, i2)) * (0.4 + 0.6 * (2.0 * cb * cb - 1)));
-  const double ddg2bdxdx = accessv_s<UNI>(T, false, true, 0, iy, d2xQG2, false, 0, cb, y);
   D.ddg2dxdx = lerp(ax, ddg2adxdx, ddg2bdxdx);
-  const double ddg2bdcbdcb = accessv_s<UNI>(T, false, true, 0, iy, QG2, false, 2, cb, y);
   D.ddg2dcbdcb = ddg2bdcbdcb * xsq / xbsq;
   D.ddg2dydcb = lerp(ax, 0.0, ddg2bdydcb);
-  double rg3[4];
-  sums_s<UNI>(T, false, 0, iy, QG3, 0, cb, rg3);
-  const double dg3bdy = ysum_s(T, false, false, iy, true, rg3, y);
-  const double dg3bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, rg3, T.YY[iy]) : 0.0;
-  const double dg3bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, rg3, T.YY[iy + 1]) : 0.0;
-  const double ddg3bdxdy = accessv_s<UNI>(T, false, false, 0, iy, dxQG3, true, 0, cb, y);
-  const double ddg3bdxdcb = accessv_s<UNI>(T, false, true, 0, iy, dxQG3, false, 1, cb, y);
-  const double ddg3bdxdx = accessv_s<UNI>(T, false, true, 0, iy, d2xQG3, false, 0, cb, y);
-  const double ddg3bdcbdcb = accessv_s<UNI>(T, false, true, 0, iy, QG3, false, 2, cb, y);
-  const double ddg3bdydcb = accessv_s<UNI>(T, false, false, 0, iy, QG3, true, 1, cb, y);
+  const int ng3[3] = { QG3, dxQG3, d2xQG3 }, ng3c[2] = { dxQG3, QG3 }, og3[1] = { QG3 };
+  tsums<3, UNI>(T, ng3, 0, iy, cb, r3);
+  const double dg3bdy = ysum_s(T, false, false, iy, true, r3[0], y);
+  const double dg3bdy_ya = flag2 ? ysum_s(T, false, false, iy, true, r3[0], T.YY[iy]) : 0.0;
+  const double dg3bdy_yb = flag2 ? ysum_s(T, false, false, iy, true, r3[0], T.YY[iy + 1]) : 0.0;
+  const double ddg3bdxdy = ysum_s(T, false, false, iy, true, r3[1], y);
+  const double ddg3bdxdx = ysum_s(T, false, true, iy, false, r3[2], y);
+  tsums<2, UNI>(T, ng3c, 1, iy, cb, r2);
+  const double ddg3bdxdcb = ysum_s(T, false, true, iy, false, r2[0], y);
+  const double ddg3bdydcb = ysum_s(T, false, false, iy, true, r2[1], y);
+  tsums<1, UNI>(T, og3, 2, iy, cb, r1);
+  const double ddg3bdcbdcb = ysum_s(T, false, true, iy, false, r1[0], y);
   const double dg1bdy = dg3bdy - cb / xb * g2b - (y / xb) * cb * dg2bdy;
   const double ddg1bdxdx = ddg3bdxdx - 2.0 * y / (xbsq * xb) * cb * g2b + y / xbsq * cb * dg2bdx + y / xbsq * cb * dg2bdx
     - y / xb * cb * ddg2bdxdx;
