@@ -1636,8 +1636,9 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   r[KQED_REC_COEF + 2] = s2;
   r[KQED_REC_COEF + 3] = s3;
   r[KQED_REC_M] = M;
-  r[KQED_REC_GX] = exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.);
-  r[KQED_REC_GY] = exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.);
+  /* the M-L2 damping factors (exp(-0) = 1 exactly, so M = 0 skips the exponentials) */
+  r[KQED_REC_GX] = M != 0.0 ? exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.) : 1.0;
+  r[KQED_REC_GY] = M != 0.0 ? exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.) : 1.0;
   r[KQED_REC_P] = (double)p;
   kinds[ri] = kd;
   /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
