@@ -65,6 +65,9 @@ __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const
 template <int AW, typename MT>
 __global__ void k_assemble2(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
                             const unsigned *kinds, void *out, int layout, size_t ntot);
+template <int AW, typename MT>
+__global__ void k_assemble2m(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
+                             const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
 __global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
                                    double *W);
@@ -530,7 +533,11 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
      * KIND_NONE only) go to the two-pair kernel, the special ones to the general one */
     const bool f32 = layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
     const bool two = perm && nslots[which] == 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc;
-    const unsigned *split = two ? w.count + 1 : nullptr;
+    /* several slots (L1..L3; not M-L2, whose slots gather over rows): the multi-slot two-pair
+     * kernel, which also forms the contraction dot */
+    const bool twom = perm && nslots[which] > 1 && which != 4 && !Wc &&
+                      (f32 || layout == KQED_AOS || layout == KQED_SOA || layout == LAYOUT_DOT || layout == LAYOUT_DOT64);
+    const unsigned *split = two || twom ? w.count + 1 : nullptr;
     if (f32)
       hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
                          nslots[which], n, Tc, Wc, split, c0);
@@ -546,6 +553,16 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       else
         hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split,
                            c0, w.rec, w.kinds, out_dev, layout, n);
+    }
+    if (twom) {
+      const size_t nb2 = ((c + 1) / 2 + KQED_ASM2M_WAVES - 1) / KQED_ASM2M_WAVES;
+      const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
+      if (f32)
+        hipLaunchKernelGGL((k_assemble2m<KQED_ASM2M_WAVES, float>), dim3(grid2), dim3(64 * KQED_ASM2M_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc);
+      else
+        hipLaunchKernelGGL((k_assemble2m<KQED_ASM2M_WAVES, double>), dim3(grid2), dim3(64 * KQED_ASM2M_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc);
     }
     prof_end(d, s, ev);
   }

@@ -58,6 +58,7 @@
 #define KQED_LDS_ARRAYS 8       /* node arrays per wavefront staged in LDS by the packed-path kernel */
 #define KQED_INTERP_THREADS 128 /* interpolation block: two wavefronts (256 measured 1.5-2% slower) */
 #define KQED_ASM2_WAVES 12  /* wavefronts per two-pair assembly block (LDS 100 KB; 16 measured 18 % slower: VGPR-bound) */
+#define KQED_ASM2M_WAVES 8 /* multi-slot two-pair assembly: LDS 139 KB (all three maps, record, O) */
 #define KQED_ASM_WAVES 16   /* wavefronts per assembly block: one block (LDS 136 KB) fills a CU */
 
 enum {

@@ -1708,6 +1708,8 @@ DEV D2 operator/(D2 a, double b) { return { a.x / b, a.y / b }; }
 DEV D2 operator+(D2 a) { return a; }
 DEV D2 &operator+=(D2 &a, D2 b) { a = a + b; return a; }
 DEV D2 fma(double a, D2 b, D2 c) { return { fma(a, b.x, c.x), fma(a, b.y, c.y) }; }
+DEV D2 fma(D2 a, D2 b, D2 c) { return { fma(a.x, b.x, c.x), fma(a.y, b.y, c.y) }; }
+DEV D2 operator/(D2 a, D2 b) { return { a.x / b.x, a.y / b.y }; }
 
 /* group helper j < 76; d: its descriptor packed as bytes U|a|V|b (d.x) and W|c|s (d.y); R: the
  * record in LDS */
@@ -1759,54 +1761,57 @@ DEV V std_entry(const V *P, const V *xv, const V *yv, const V *h, int c)
   return f + h[40 + dta] * quad;
 }
 
-/* Tabd_xeq0 tensors (reference Tabd.c:71-111); h = y/|y|.  c: 0-63 tI, 64-127 tII, 128-191 tIII */
-DEV double xeq0_entry(const double *P, const double *h, int c)
+/* Tabd_xeq0 tensors (reference Tabd.c:71-111); h = y/|y|.  c: 0-63 tI, 64-127 tII, 128-191 tIII.
+ * V: double, or D2 for two pairs side by side */
+template <typename V>
+DEV V xeq0_entry(const V *P, const V *h, int c)
 {
-  const double *yv = P;
-  const double y = P[4], ysq = P[5], fx = P[6], fy = P[7], ell2 = P[8], dell2adx = P[9], dell2ady = P[10];
-  const double ell3a = P[11], dg2dx = P[12], ddg2dxdy = P[13], dg1dx = P[14], dg1dy = P[15], phi1 = P[16], phi2 = P[17];
+  const V *yv = P;
+  const V y = P[4], ysq = P[5], fx = P[6], fy = P[7], ell2 = P[8], dell2adx = P[9], dell2ady = P[10];
+  const V ell3a = P[11], dg2dx = P[12], ddg2dxdy = P[13], dg1dx = P[14], dg1dy = P[15], phi1 = P[16], phi2 = P[17];
   const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
-  const double ha = h[alf], hb = h[bet], hd = h[dta];
+  const V ha = h[alf], hb = h[bet], hd = h[dta];
   const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
-  const double ysq_4 = ysq / 4.;
-  const double E = (ell3a + ell2), D2 = (dell2adx + dell2ady), D1 = (dg1dy + dg1dx), D12 = (dg1dx + dg2dx),
-               D3 = (y * ddg2dxdy - dg2dx);
+  const V ysq_4 = ysq / 4.;
+  const V E = (ell3a + ell2), Dd2 = (dell2adx + dell2ady), Dd1 = (dg1dy + dg1dx), Dd12 = (dg1dx + dg2dx),
+          Dd3 = (y * ddg2dxdy - dg2dx);
   if (t == 2) {
-    const double dgSUM = ((hb * fx / 2) + (hb * fy / 2));
-    const double tIIIsum = dgSUM - 0.5 * (yv[bet] * E) - ysq_4 * (hb * D2);
-    return (yv[alf] * yv[dta] * (hb * D2) + dab * (yv[dta] * E) + dbd * (yv[alf] * E) + dad * tIIIsum);
+    const V dgSUM = ((hb * fx / 2) + (hb * fy / 2));
+    const V tIIIsum = dgSUM - 0.5 * (yv[bet] * E) - ysq_4 * (hb * Dd2);
+    return (yv[alf] * yv[dta] * (hb * Dd2) + dab * (yv[dta] * E) + dbd * (yv[alf] * E) + dad * tIIIsum);
   }
   if (t == 1) {
-    const double yalf2dx = ha * (dell2adx);
-    const double tIIsum = (ha * fx / 2) - 0.5 * (yv[alf] * ell3a) - ysq_4 * yalf2dx;
+    const V yalf2dx = ha * (dell2adx);
+    const V tIIsum = (ha * fx / 2) - 0.5 * (yv[alf] * ell3a) - ysq_4 * yalf2dx;
     return (yv[bet] * yv[dta] * yalf2dx + dbd * (tIIsum) + (dab * (yv[dta] * ell3a) + dad * (yv[bet] * ell3a)));
   }
-  const double tIsum = 2 * (dab * phi1 + ha * hb * phi2);
-  return (dbd * (ha * D12) + (hd * dg2dx) * dab + (hb * D3) * ha * hd + (hb * D1) * dad + yv[dta] * tIsum);
+  const V tIsum = 2 * (dab * phi1 + ha * hb * phi2);
+  return (dbd * (ha * Dd12) + (hd * dg2dx) * dab + (hb * Dd3) * ha * hd + (hb * Dd1) * dad + yv[dta] * tIsum);
 }
 
 /* Tabd_yeq0 tensors (reference Tabd.c:190-217, loop form); h = x/|x| */
-DEV double yeq0_entry(const double *P, const double *h, int c)
+template <typename V>
+DEV V yeq0_entry(const V *P, const V *h, int c)
 {
-  const double *xv = P;
-  const double xsq = P[5], fx = P[6], fy = P[7], ell1 = P[8], ell3 = P[9], dell1dx = P[10];
-  const double dell1dycb = P[11], dg1dx = P[12], ddg1dxdx = P[13], dg2dx = P[14], dg1dycb = P[15], ddg1dxdycb = P[16];
+  const V *xv = P;
+  const V xsq = P[5], fx = P[6], fy = P[7], ell1 = P[8], ell3 = P[9], dell1dx = P[10];
+  const V dell1dycb = P[11], dg1dx = P[12], ddg1dxdx = P[13], dg2dx = P[14], dg1dycb = P[15], ddg1dxdycb = P[16];
   const int t = c >> 6, alf = (c >> 4) & 3, bet = (c >> 2) & 3, dta = c & 3;
-  const double ha = h[alf], hb = h[bet], hd = h[dta];
+  const V ha = h[alf], hb = h[bet], hd = h[dta];
   const double dab = alf == bet, dbd = bet == dta, dad = alf == dta;
-  const double xsq_4 = xsq / 4., E = ell1 + ell3, D1 = dell1dx + dell1dycb, D2 = (ddg1dxdx + ddg1dxdycb),
-               D3 = (dg1dycb - dg2dx), D4 = (dg1dx + dg1dycb);
+  const V xsq_4 = xsq / 4., E = ell1 + ell3, Dd1 = dell1dx + dell1dycb, Dd2 = (ddg1dxdx + ddg1dxdycb),
+          Dd3 = (dg1dycb - dg2dx), Dd4 = (dg1dx + dg1dycb);
   if (t == 2) {
-    const double Dg0SUM = ((hb * fx / 2) + (hb * fy / 2) - 0.5 * (xv[bet] * E));
-    const double xad = (xv[alf] * xv[dta] - xsq_4 * dad);
-    return ((xad * hb * D1 + dab * (xv[dta] * E) + dbd * (xv[alf] * E) + Dg0SUM * dad));
+    const V Dg0SUM = ((hb * fx / 2) + (hb * fy / 2) - 0.5 * (xv[bet] * E));
+    const V xad = (xv[alf] * xv[dta] - xsq_4 * dad);
+    return ((xad * hb * Dd1 + dab * (xv[dta] * E) + dbd * (xv[alf] * E) + Dg0SUM * dad));
   }
   if (t == 1) {
-    const double xhd1dx = ha * (dell1dx);
-    const double tIIsum = ((ha * fx / 2) - 0.5 * (xv[alf] * ell1));
+    const V xhd1dx = ha * (dell1dx);
+    const V tIIsum = ((ha * fx / 2) - 0.5 * (xv[alf] * ell1));
     return ((xv[bet] * xv[dta] - xsq_4 * dbd) * xhd1dx + dab * (xv[dta] * ell1) + dad * (xv[bet] * ell1) + dbd * (tIIsum));
   }
-  return ((dad * hb + dbd * ha + dab * hd - ha * hb * hd) * D4 + ha * (hb * xv[dta] * D2 - dbd * D3));
+  return ((dad * hb + dbd * ha + dab * hd - ha * hb * hd) * Dd4 + ha * (hb * xv[dta] * Dd2 - dbd * Dd3));
 }
 
 /* ------------------------------------------------------------------ assembly kernel */
@@ -2341,6 +2346,201 @@ template __global__ void k_assemble2<KQED_ASM2_WAVES, double>(DT, size_t, const 
                                                               const unsigned *, void *, int, size_t);
 template __global__ void k_assemble2<KQED_ASM2_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
                                                              const unsigned *, void *, int, size_t);
+
+/* Several evaluation slots (L1, L2, L3) in sorted order past the special pairs, two records per
+ * wavefront as in k_assemble2: a slot is evaluated once for both records when their kinds agree
+ * (past the special pairs they do, unless a slot of one record is KIND_NONE, whose component is then
+ * left unchanged), and once per record when they differ.  Each record's arithmetic is the one-pair
+ * kernel's, in the same order (slot 0 stored, later slots fma'd with their coefficient, the dot
+ * layouts summed over O in lane order), so the outputs are bit-identical to it.  The M-L2 slots
+ * (KIND_XEQ0_M / KIND_YEQ0_M) are not handled here: the host never routes M-L2 to this kernel.
+ * Layouts: 0/1 out[q][384] / out[384][ntot] of MT, 16 / 64 the contraction dot with an FP32 /
+ * FP64 site tensor Tc (MT double). */
+template <int AW, typename MT>
+__global__ void __launch_bounds__(64 * AW)
+k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
+             const unsigned *__restrict__ kinds, void *__restrict__ outv, int layout, int nslots, size_t ntot,
+             const void *__restrict__ Tc)
+{
+  __shared__ unsigned sE[KQED_WM_MAX];
+  __shared__ unsigned short sRow[3 * 384];
+  __shared__ D2 sW[AW][WREC + WH + 201 + WT];
+  size_t lo = split[0] > c0 ? split[0] - c0 : 0;
+  lo = lo < n ? lo : n;
+  const size_t np = (n - lo + 1) / 2; /* record pairs */
+  if ((size_t)blockIdx.x * AW >= np) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < T.wmTotal; i += blockDim.x) sE[i] = T.wmE[i];
+  for (int i = threadIdx.x; i < 3 * 384; i += blockDim.x) sRow[i] = T.wmRow[i];
+  __syncthreads();
+  typedef typename std::conditional<sizeof(MT) == 8, D2, F2>::type TV;
+  D2 *R = sW[w], *H = R + WREC, *O = H + WH + 201;
+  TV *Tv = (TV *)(H + WH);
+  const size_t stride = (size_t)gridDim.x * AW;
+  const bool dotl = layout == 16 || layout == 64;
+  auto hdesc = [](int g) {
+    const unsigned char *e = HDESC[g < 19 ? g : 0];
+    return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
+  };
+  const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  unsigned er[13]; /* map 0's first row slot, kept in registers */
+#pragma unroll
+  for (int k = 0; k < 13; k++) er[k] = sE[64 * k + lane];
+  static_assert(WREC > 128 && WREC <= 192, "record staging covers three doubles per lane");
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+  unsigned ka = KIND_NONE, kb = KIND_NONE;
+  auto load = [&](size_t qq) {
+    const size_t ia = lo + 2 * qq, ib = ia + 1;
+    const double *ra = rec + ia * WREC;
+    a0 = __builtin_nontemporal_load(ra + lane);
+    a1 = __builtin_nontemporal_load(ra + lane + 64);
+    a2 = lane + 128 < WREC ? __builtin_nontemporal_load(ra + lane + 128) : 0.0;
+    ka = kinds[ia];
+    if (ib < n) {
+      const double *rb = ra + WREC;
+      b0 = __builtin_nontemporal_load(rb + lane);
+      b1 = __builtin_nontemporal_load(rb + lane + 64);
+      b2 = lane + 128 < WREC ? __builtin_nontemporal_load(rb + lane + 128) : 0.0;
+      kb = kinds[ib];
+    } else {
+      b0 = b1 = b2 = 0.0;
+      kb = KIND_NONE;
+    }
+  };
+  size_t q = (size_t)blockIdx.x * AW + w;
+  if (q < np) load(q);
+  size_t pa = 0, pb = 0;
+  double pda = 0.0, pdb = 0.0;
+  bool pending = false, pbv = false;
+  auto flush = [&]() {
+    if (dotl) {
+      if (lane == 0) {
+        ((double *)outv)[pa] = pda;
+        if (pbv) ((double *)outv)[pb] = pdb;
+      }
+      return;
+    }
+    MT *out = (MT *)outv;
+    for (int i = lane; i < 384; i += 64) {
+      const D2 o = O[i];
+      if (layout & 1) {
+        __builtin_nontemporal_store((MT)o.x, out + (size_t)i * ntot + pa);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + (size_t)i * ntot + pb);
+      } else {
+        __builtin_nontemporal_store((MT)o.x, out + pa * 384 + i);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + pb * 384 + i);
+      }
+    }
+  };
+  for (; q < np; q += stride) {
+    R[lane] = D2{ a0, b0 };
+    R[lane + 64] = D2{ a1, b1 };
+    if (lane + 128 < WREC) R[lane + 128] = D2{ a2, b2 };
+    const unsigned kda = __builtin_amdgcn_readfirstlane(ka), kdb = __builtin_amdgcn_readfirstlane(kb);
+    const bool hasb = lo + 2 * q + 1 < n;
+    if (pending) flush();
+    if (q + stride < np) load(q + stride);
+    for (int i = lane; i < 384; i += 64) O[i] = D2{ 0.0, 0.0 };
+    if (lane == 0) Tv[200] = TV{ 0, 0 };
+    wave_sync();
+    const size_t qa = (size_t)R[KQED_REC_P].x, qb = hasb ? (size_t)R[KQED_REC_P].y : qa;
+    for (int slot = 0; slot < nslots; slot++) {
+      const unsigned sa = (kda >> (3 * slot)) & 7u, sb = (kdb >> (3 * slot)) & 7u;
+      for (int pass = 0; pass < 2; pass++) {
+        /* pass 0: the kind of record a (of b when a's slot is empty); pass 1: b's differing kind */
+        unsigned kd;
+        if (pass == 0) kd = sa != KIND_NONE ? sa : sb;
+        else if (sa != KIND_NONE && sb != KIND_NONE && sa != sb) kd = sb;
+        else break;
+        if (kd == KIND_NONE) break;
+        const bool ua = pass == 0 && sa == kd, ub = sb == kd;
+        const D2 *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
+        if (kd == KIND_STD) {
+          H[lane] = std_helper(R, lane, hd0);
+          if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
+        } else if (lane < 4) {
+          H[lane] = pl[lane] / pl[4];
+        }
+        wave_sync();
+        if (kd == KIND_STD) {
+          for (int c = lane; c < 200; c += 64) {
+            const D2 t = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+            Tv[c] = TV{ (MT)t.x, (MT)t.y };
+          }
+        } else if (kd == KIND_XEQ0 || kd == KIND_XEQ0_M) {
+          for (int c = lane; c < 192; c += 64) {
+            const D2 t = xeq0_entry(pl, H, c);
+            Tv[c] = TV{ (MT)t.x, (MT)t.y };
+          }
+        } else {
+          for (int c = lane; c < 192; c += 64) {
+            const D2 t = yeq0_entry(pl, H, c);
+            Tv[c] = TV{ (MT)t.x, (MT)t.y };
+          }
+        }
+        wave_sync();
+        const int mq = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
+        const unsigned *e = sE + mq * 64 * KQED_WM_PER_LANE;
+        D2 v[6];
+        if constexpr (sizeof(MT) == 8) {
+          if (mq == 0) v[0] = wm_dot2r<13>(er, (const D2 *)Tv);
+          else v[0] = wm_dot2<13>(e, lane, Tv);
+        } else {
+          v[0] = wm_dot2<13>(e, lane, Tv);
+        }
+        v[1] = wm_dot2<9>(e + 64 * 13, lane, Tv);
+        v[2] = wm_dot2<6>(e + 64 * 22, lane, Tv);
+        v[3] = wm_dot2<6>(e + 64 * 28, lane, Tv);
+        v[4] = wm_dot2<4>(e + 64 * 34, lane, Tv);
+        v[5] = wm_dot2<3>(e + 64 * 38, lane, Tv);
+        const D2 cf = R[KQED_REC_COEF + slot];
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+          const int r = sRow[mq * 384 + 64 * m + lane];
+          D2 o = O[r];
+          if (ua) o.x = slot == 0 ? v[m].x : fma(v[m].x, cf.x, o.x); /* atomic_kernel_Saxpy, kernels.c:52-78 */
+          if (ub) o.y = slot == 0 ? v[m].y : fma(v[m].y, cf.y, o.y);
+          O[r] = o;
+        }
+        wave_sync();
+      }
+    }
+    double da = 0.0, db = 0.0;
+    if (dotl) {
+      double sa = 0.0, sb = 0.0;
+      if (layout == 16) {
+        const float *ta = (const float *)Tc + qa * 384, *tb = (const float *)Tc + qb * 384;
+        for (int i = lane; i < 384; i += 64) {
+          const D2 o = O[i];
+          sa = fma(o.x, (double)ta[i], sa);
+          sb = fma(o.y, (double)tb[i], sb);
+        }
+      } else {
+        const double *ta = (const double *)Tc + qa * 384, *tb = (const double *)Tc + qb * 384;
+        for (int i = lane; i < 384; i += 64) {
+          const D2 o = O[i];
+          sa = fma(o.x, ta[i], sa);
+          sb = fma(o.y, tb[i], sb);
+        }
+      }
+      da = wave_sum(sa);
+      db = wave_sum(sb);
+    }
+    pa = qa;
+    pb = qb;
+    pda = da;
+    pdb = db;
+    pbv = hasb;
+    pending = true;
+    wave_sync();
+  }
+  if (pending) flush();
+}
+
+template __global__ void k_assemble2m<KQED_ASM2M_WAVES, double>(DT, size_t, const unsigned *, size_t, const double *,
+                                                                const unsigned *, void *, int, int, size_t, const void *);
+template __global__ void k_assemble2m<KQED_ASM2M_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
+                                                               const unsigned *, void *, int, int, size_t, const void *);
 
 /* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,

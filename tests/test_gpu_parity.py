@@ -130,6 +130,52 @@ def test_two_pair_assembly_bit_identical(gpu_full):
     assert np.array_equal(got32_s, ref32.T)
 
 
+def test_two_pair_multislot_assembly_bit_identical(gpu_full):
+    """Sorted L1, L2 and L3 batches assemble the pairs past the special ones two per wavefront with
+    every slot (L0(x,x), L0(0,y), L0(x,0), ...): each output equals the one-pair kernel's on the
+    unsorted batch bit for bit, FP64 and FP32, AoS and SoA, an odd number of pairs with special
+    pairs mixed in, several passes of odd length, and the FP64 per-pair dot of the lepton loop.
+    M-L2 keeps the one-pair kernel (checked unchanged alongside)."""
+    import torch
+    x, y = log_uniform(23, 4001)
+    x[:3] = 0.0
+    y[3:6] = 0.0
+    x[6:9] = y[6:9]
+    x[9:12] = -y[9:12]
+    dev = torch.device("cuda:0")
+    xd, yd = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+
+    def loop(which, M):
+        out = torch.zeros(len(x), dtype=torch.float64, device=dev)
+        gpu_full.lepton_loop_device(which, len(x), xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=M, Mv=1.5,
+                                    stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    old = gpu_full.get_tune()
+    cases = ((1, 0.0), (2, 0.0), (3, 0.0), (4, 0.4))
+    try:
+        gpu_full.set_tune(sort_min=1 << 30)
+        ref = {w: gpu_full.batch_host(w, x, y, M=M) for w, M in cases}
+        ref32 = {w: gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_AOS_F32) for w, M in cases}
+        refd = {w: loop(w, M) for w, M in cases}
+        gpu_full.set_tune(old)
+        gpu_full.set_tune(sort_min=0)
+        for w, M in cases:
+            print(f"multi-slot two-pair which={w}", flush=True)
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M), ref[w]), w
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA), ref[w].T), w
+            got32 = gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_AOS_F32)
+            assert got32.dtype == np.float32 and np.array_equal(got32, ref32[w]), w
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA_F32), ref32[w].T), w
+            assert np.array_equal(loop(w, M).view(np.int64), refd[w].view(np.int64)), w  # NaN at x = 0 too
+        gpu_full.set_tune(chunk=999)
+        for w, M in cases[:3]:
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M), ref[w]), w
+    finally:
+        gpu_full.set_tune(old)
+
+
 def test_deterministic_and_order_independent(gpu_full):
     """Each pair's result is independent of the batch around it: the same batch twice, and the batch
     in a shuffled order, give identical per-pair results."""
