@@ -533,9 +533,9 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
      * KIND_NONE only) go to the two-pair kernel, the special ones to the general one */
     const bool f32 = layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
     const bool two = perm && nslots[which] == 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc;
-    /* several slots (L1..L3; not M-L2, whose slots gather over rows): the multi-slot two-pair
-     * kernel, which also forms the contraction dot */
-    const bool twom = perm && nslots[which] > 1 && which != 4 && !Wc &&
+    /* several slots (L1..L3, M-L2): the multi-slot two-pair kernel, which also forms the
+     * contraction dot */
+    const bool twom = perm && nslots[which] > 1 && !Wc &&
                       (f32 || layout == KQED_AOS || layout == KQED_SOA || layout == LAYOUT_DOT || layout == LAYOUT_DOT64);
     const unsigned *split = two || twom ? w.count + 1 : nullptr;
     if (f32)

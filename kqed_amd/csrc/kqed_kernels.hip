@@ -2347,13 +2347,13 @@ template __global__ void k_assemble2<KQED_ASM2_WAVES, double>(DT, size_t, const 
 template __global__ void k_assemble2<KQED_ASM2_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
                                                              const unsigned *, void *, int, size_t);
 
-/* Several evaluation slots (L1, L2, L3) in sorted order past the special pairs, two records per
+/* Several evaluation slots (L1, L2, L3, M-L2) in sorted order past the special pairs, two records per
  * wavefront as in k_assemble2: a slot is evaluated once for both records when their kinds agree
  * (past the special pairs they do, unless a slot of one record is KIND_NONE, whose component is then
  * left unchanged), and once per record when they differ.  Each record's arithmetic is the one-pair
  * kernel's, in the same order (slot 0 stored, later slots fma'd with their coefficient, the dot
- * layouts summed over O in lane order), so the outputs are bit-identical to it.  The M-L2 slots
- * (KIND_XEQ0_M / KIND_YEQ0_M) are not handled here: the host never routes M-L2 to this kernel.
+ * layouts summed over O in lane order, the M-L2 slots' alpha sums gathered in four rounds), so the
+ * outputs are bit-identical to it.
  * Layouts: 0/1 out[q][384] / out[384][ntot] of MT, 16 / 64 the contraction dot with an FP32 /
  * FP64 site tensor Tc (MT double). */
 template <int AW, typename MT>
@@ -2493,14 +2493,50 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
         v[3] = wm_dot2<6>(e + 64 * 28, lane, Tv);
         v[4] = wm_dot2<4>(e + 64 * 34, lane, Tv);
         v[5] = wm_dot2<3>(e + 64 * 38, lane, Tv);
-        const D2 cf = R[KQED_REC_COEF + slot];
+        if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
+          /* the M-L2 term (reference kernels.c:292-323) with the one-pair kernel's alpha sums: both
+           * records use the same map, so each lane's rows and sums are the same for both */
+          const bool xm = kd == KIND_XEQ0_M;
+          const D2 Mv = R[KQED_REC_M];
+          const D2 g = xm ? R[KQED_REC_GX] : R[KQED_REC_GY];
+          const D2 *vv = xm ? R + KQED_REC_X : R + KQED_REC_Y;
+          D2 *S = (D2 *)Tv; /* 96 sums in the tensor area (201 D2 slots in both element types) */
+          wave_sync();
+          for (int i = lane; i < 96; i += 64) S[i] = D2{ 0.0, 0.0 };
+          wave_sync();
+          for (int a = 0; a < 4; a++) {
 #pragma unroll
-        for (int m = 0; m < 6; m++) {
-          const int r = sRow[mq * 384 + 64 * m + lane];
-          D2 o = O[r];
-          if (ua) o.x = slot == 0 ? v[m].x : fma(v[m].x, cf.x, o.x); /* atomic_kernel_Saxpy, kernels.c:52-78 */
-          if (ub) o.y = slot == 0 ? v[m].y : fma(v[m].y, cf.y, o.y);
-          O[r] = o;
+            for (int m = 0; m < 6; m++) {
+              const int r = sRow[mq * 384 + 64 * m + lane];
+              const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
+              if ((xm ? mu : nu) == a) {
+                D2 *sp = S + 16 * rs + 4 * (xm ? nu : mu) + la;
+                *sp = *sp + vv[a] * v[m];
+              }
+            }
+            wave_sync();
+          }
+#pragma unroll
+          for (int m = 0; m < 6; m++) {
+            const int r = sRow[mq * 384 + 64 * m + lane];
+            const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
+            const D2 sum = S[16 * rs + 4 * (xm ? nu : mu) + la];
+            const D2 t = O[r] - g * (v[m] - Mv * vv[xm ? mu : nu] * sum);
+            D2 o = O[r];
+            if (ua) o.x = t.x;
+            if (ub) o.y = t.y;
+            O[r] = o;
+          }
+        } else {
+          const D2 cf = R[KQED_REC_COEF + slot];
+#pragma unroll
+          for (int m = 0; m < 6; m++) {
+            const int r = sRow[mq * 384 + 64 * m + lane];
+            D2 o = O[r];
+            if (ua) o.x = slot == 0 ? v[m].x : fma(v[m].x, cf.x, o.x); /* atomic_kernel_Saxpy, kernels.c:52-78 */
+            if (ub) o.y = slot == 0 ? v[m].y : fma(v[m].y, cf.y, o.y);
+            O[r] = o;
+          }
         }
         wave_sync();
       }
