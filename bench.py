@@ -253,6 +253,9 @@ def surrogate_leg(t, dev, stream, n_train, epochs, n_infer, n_plain_rate):
             "max_rel_err_vs_fp64": float(err.max()), "median_rel_err_vs_fp64": float(np.median(err))}
 
 
+KNAMES = ("L0", "L1", "L2", "L3", "ML2_M0.5")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +265,9 @@ def main():
     ap.add_argument("--table", default="full")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the L1/L2/L3/M-L2 rates and the contraction")
+    ap.add_argument("--kernel", type=int, default=0, choices=range(5),
+                    help="kernel of the timed workload: 0 L0 (the headline), 1 L1, 2 L2, 3 L3, 4 M-L2 (M = 0.5); "
+                         "kernels other than L0 time the GPU only (no CPU baseline or roofline)")
     ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
     ap.add_argument("--surrogate", type=int, default=1, help="run the surrogate + FP32 leg (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=1 << 24, help="training pairs of the surrogate leg")
@@ -286,8 +292,11 @@ def main():
     out = torch.empty((a.n, 384), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
 
-    def step(which=0, M=0.0, n=a.n):
+    def step(which=a.kernel, M=0.5 if a.kernel == 4 else 0.0, n=a.n):
         t.batch_device(which, n, xd.data_ptr(), yd.data_ptr(), out.data_ptr(), M=M, stream=stream)
+
+    if a.kernel != 0:
+        a.no_cpu = True
 
     if a.copy_calibration:
         src = torch.empty(a.copy_calibration << 18, dtype=torch.float32, device=dev).fill_(1.0)
@@ -365,8 +374,8 @@ def main():
         "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": dt / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "QED_kernel_L0, 2^24 pairs, |x|,|y| log-uniform [1e-3,4], isotropic, FP64",
-                   "pairs": a.n, "table": a.table, "kernel": "L0", "path": "plain FP64"},
+        "config": {"workload": f"QED_kernel_{KNAMES[a.kernel]}, 2^24 pairs, |x|,|y| log-uniform [1e-3,4], isotropic, FP64",
+                   "pairs": a.n, "table": a.table, "kernel": KNAMES[a.kernel], "path": "plain FP64"},
         "stages_ms_per_step": {"sort": ms_s / a.steps, "interp": ms_i / a.steps, "assemble": ms_a / a.steps},
         "dominant_kernel": dom,
         "roofline": roof,

@@ -2462,21 +2462,24 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
           H[lane] = pl[lane] / pl[4];
         }
         wave_sync();
+        /* Tabd tensors: a lane's three entries c = lane + 64 k are formed before any is stored (no
+         * LDS store lies between their reads of the record, so the per-pair values are read and
+         * formed once) */
         if (kd == KIND_STD) {
           for (int c = lane; c < 200; c += 64) {
             const D2 t = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
             Tv[c] = TV{ (MT)t.x, (MT)t.y };
           }
         } else if (kd == KIND_XEQ0 || kd == KIND_XEQ0_M) {
-          for (int c = lane; c < 192; c += 64) {
-            const D2 t = xeq0_entry(pl, H, c);
-            Tv[c] = TV{ (MT)t.x, (MT)t.y };
-          }
+          const D2 t0 = xeq0_entry(pl, H, lane), t1 = xeq0_entry(pl, H, lane + 64), t2 = xeq0_entry(pl, H, lane + 128);
+          Tv[lane] = TV{ (MT)t0.x, (MT)t0.y };
+          Tv[lane + 64] = TV{ (MT)t1.x, (MT)t1.y };
+          Tv[lane + 128] = TV{ (MT)t2.x, (MT)t2.y };
         } else {
-          for (int c = lane; c < 192; c += 64) {
-            const D2 t = yeq0_entry(pl, H, c);
-            Tv[c] = TV{ (MT)t.x, (MT)t.y };
-          }
+          const D2 t0 = yeq0_entry(pl, H, lane), t1 = yeq0_entry(pl, H, lane + 64), t2 = yeq0_entry(pl, H, lane + 128);
+          Tv[lane] = TV{ (MT)t0.x, (MT)t0.y };
+          Tv[lane + 64] = TV{ (MT)t1.x, (MT)t1.y };
+          Tv[lane + 128] = TV{ (MT)t2.x, (MT)t2.y };
         }
         wave_sync();
         const int mq = kd == KIND_STD ? 0 : kd == KIND_YEQ0_MULAM ? 2 : 1;
