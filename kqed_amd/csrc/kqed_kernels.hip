@@ -2441,7 +2441,6 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
     if (pending) flush();
     if (q + stride < np) load(q + stride);
     for (int i = lane; i < 384; i += 64) O[i] = D2{ 0.0, 0.0 };
-    if (lane == 0) Tv[200] = TV{ 0, 0 };
     wave_sync();
     const size_t qa = (size_t)R[KQED_REC_P].x, qb = hasb ? (size_t)R[KQED_REC_P].y : qa;
     for (int slot = 0; slot < nslots; slot++) {
@@ -2455,6 +2454,9 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
         if (kd == KIND_NONE) break;
         const bool ua = pass == 0 && sa == kd, ub = sb == kd;
         const D2 *pl = slot == 0 ? R + KQED_REC_MAIN : R + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
+        /* the maps' zero column, set for every slot: an M-L2 slot's sums overwrite it in the FP32
+         * layouts, whose tensor entries are 8 bytes */
+        if (lane == 0) Tv[200] = TV{ 0, 0 };
         if (kd == KIND_STD) {
           H[lane] = std_helper(R, lane, hd0);
           if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
@@ -2497,34 +2499,40 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
         v[4] = wm_dot2<4>(e + 64 * 34, lane, Tv);
         v[5] = wm_dot2<3>(e + 64 * 38, lane, Tv);
         if (kd == KIND_XEQ0_M || kd == KIND_YEQ0_M) {
-          /* the M-L2 term (reference kernels.c:292-323) with the one-pair kernel's alpha sums: both
-           * records use the same map, so each lane's rows and sums are the same for both */
+          /* the M-L2 term (reference kernels.c:292-323).  Sum (rs, nu, la) (x = 0 term; (rs, mu, la)
+           * for y = 0) is ((0 + p_0) + p_1) + p_2) + p_3 with p_a = v_a(row) vv[a], as the one-pair
+           * kernel's four rounds form it: each row stores its p_a once (a < 3: into buffer a, 96
+           * sums each, in the free helper and tensor areas), then the row with a = 3 adds the three
+           * and its own p_3 and leaves the sum in buffer 0.  Both records use the same map, so each
+           * lane's rows and sums are the same for both. */
           const bool xm = kd == KIND_XEQ0_M;
           const D2 Mv = R[KQED_REC_M];
           const D2 g = xm ? R[KQED_REC_GX] : R[KQED_REC_GY];
           const D2 *vv = xm ? R + KQED_REC_X : R + KQED_REC_Y;
-          D2 *S = (D2 *)Tv; /* 96 sums in the tensor area (201 D2 slots in both element types) */
-          wave_sync();
-          for (int i = lane; i < 96; i += 64) S[i] = D2{ 0.0, 0.0 };
-          wave_sync();
-          for (int a = 0; a < 4; a++) {
-#pragma unroll
-            for (int m = 0; m < 6; m++) {
-              const int r = sRow[mq * 384 + 64 * m + lane];
-              const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
-              if ((xm ? mu : nu) == a) {
-                D2 *sp = S + 16 * rs + 4 * (xm ? nu : mu) + la;
-                *sp = *sp + vv[a] * v[m];
-              }
-            }
-            wave_sync();
-          }
+          static_assert(3 * 96 <= WH + 201, "three sum buffers fit the helper and tensor areas");
+          D2 *B = H;
+          int ia[6], ix[6];
 #pragma unroll
           for (int m = 0; m < 6; m++) {
             const int r = sRow[mq * 384 + 64 * m + lane];
             const int rs = r >> 6, mu = (r >> 4) & 3, nu = (r >> 2) & 3, la = r & 3;
-            const D2 sum = S[16 * rs + 4 * (xm ? nu : mu) + la];
-            const D2 t = O[r] - g * (v[m] - Mv * vv[xm ? mu : nu] * sum);
+            ia[m] = xm ? mu : nu;
+            ix[m] = 16 * rs + 4 * (xm ? nu : mu) + la;
+          }
+          wave_sync(); /* every lane is done reading the helpers and tensors */
+#pragma unroll
+          for (int m = 0; m < 6; m++)
+            if (ia[m] < 3) B[96 * ia[m] + ix[m]] = vv[ia[m]] * v[m];
+          wave_sync();
+#pragma unroll
+          for (int m = 0; m < 6; m++)
+            if (ia[m] == 3) B[ix[m]] = ((0.0 + B[ix[m]]) + B[96 + ix[m]]) + B[192 + ix[m]] + vv[3] * v[m];
+          wave_sync();
+#pragma unroll
+          for (int m = 0; m < 6; m++) {
+            const int r = sRow[mq * 384 + 64 * m + lane];
+            const D2 sum = B[ix[m]];
+            const D2 t = O[r] - g * (v[m] - Mv * vv[ia[m]] * sum);
             D2 o = O[r];
             if (ua) o.x = t.x;
             if (ub) o.y = t.y;
