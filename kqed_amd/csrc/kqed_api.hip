@@ -62,9 +62,9 @@ __global__ void k_surr_binerr(size_t n, const double *a, const double *b, const 
 template <typename R>
 __global__ void k_assemble(kqed_dev_tables T, size_t n, const double *rec, const unsigned *kinds, void *out, int layout,
                            int nslots, size_t ntot, const void *Tc, const double *Wc, const unsigned *split, size_t c0);
-template <int AW, typename MT>
+template <int AW, typename MT, bool WDOT>
 __global__ void k_assemble2(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
-                            const unsigned *kinds, void *out, int layout, size_t ntot);
+                            const unsigned *kinds, void *out, int layout, size_t ntot, const double *Wc);
 template <int AW, typename MT>
 __global__ void k_assemble2m(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
                              const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
@@ -530,9 +530,11 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     const unsigned grid = (unsigned)(nblk < (size_t)tu.asm_grid ? nblk : (size_t)tu.asm_grid);
     prof_begin(d, s, ev, ST_ASSEMBLE);
     /* one evaluation slot in sorted order: the records past the special pairs (kinds KIND_STD /
-     * KIND_NONE only) go to the two-pair kernel, the special ones to the general one */
+     * KIND_NONE only) go to the two-pair kernel (outputs, or the contraction shortcut's dot), the
+     * special ones to the general one */
     const bool f32 = layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
-    const bool two = perm && nslots[which] == 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc;
+    const bool two = perm && nslots[which] == 1 &&
+                     ((f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc || (Wc && layout == LAYOUT_DOT));
     /* several slots (L1..L3, M-L2): the multi-slot two-pair kernel, which also forms the
      * contraction dot */
     const bool twom = perm && nslots[which] > 1 && !Wc &&
@@ -548,11 +550,14 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2_WAVES - 1) / KQED_ASM2_WAVES;
       const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
       if (f32)
-        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, float>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split, c0,
-                           w.rec, w.kinds, out_dev, layout, n);
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, float, false>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, n, (const double *)nullptr);
+      else if (Wc)
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double, true>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, n, Wc);
       else
-        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c, split,
-                           c0, w.rec, w.kinds, out_dev, layout, n);
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double, false>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, n, (const double *)nullptr);
     }
     if (twom) {
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2M_WAVES - 1) / KQED_ASM2M_WAVES;

@@ -2227,10 +2227,11 @@ template __global__ void k_assemble<float>(DT, size_t, const double *, const uns
  * same order, so the outputs are bit-identical to it.  The output area aliases the helper and
  * tensor areas (they are dead once the rows are formed).  layout bit 0: out[384][ntot] (else
  * out[q][384]); the element type is MT. */
-template <int AW, typename MT>
+template <int AW, typename MT, bool WDOT>
 __global__ void __launch_bounds__(64 * AW)
 k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
-            const unsigned *__restrict__ kinds, void *__restrict__ outv, int layout, size_t ntot)
+            const unsigned *__restrict__ kinds, void *__restrict__ outv, int layout, size_t ntot,
+            const double *__restrict__ Wc)
 {
   __shared__ unsigned sE[64 * KQED_WM_PER_LANE]; /* the standard map (map 0 comes first) */
   __shared__ unsigned short sRow[384];
@@ -2282,8 +2283,16 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
    * outputs and records are touched once, so they are stored and loaded non-temporally (they would
    * only push the interpolation's table nodes out of the caches) */
   size_t pa = 0, pb = 0;
+  double pda = 0.0, pdb = 0.0;
   bool pending = false, pbv = false;
   auto flush = [&]() {
+    if constexpr (WDOT) {
+      if (lane == 0) {
+        ((double *)outv)[pa] = pda;
+        if (pbv) ((double *)outv)[pb] = pdb;
+      }
+      return;
+    }
     for (int i = lane; i < 384; i += 64) {
       const D2 o = O[i];
       MT *out = (MT *)outv;
@@ -2314,6 +2323,27 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
       Tv[c] = TV{ (MT)t.x, (MT)t.y };
     }
     wave_sync();
+    if constexpr (WDOT) {
+      /* contraction shortcut (layout 16, FP64 map type): dot of the 200 tensor entries with the
+       * site's pulled-back weights, as the one-pair kernel forms it; an empty slot gives 0 */
+      const double *wa = Wc + qa * 200, *wb = Wc + qb * 200;
+      double sa = 0.0, sb = 0.0;
+      for (int c = lane; c < 200; c += 64) {
+        const TV t = Tv[c];
+        sa = fma((double)t.x, wa[c], sa);
+        sb = fma((double)t.y, wb[c], sb);
+      }
+      sa = wave_sum(sa);
+      sb = wave_sum(sb);
+      pda = kda == KIND_STD ? sa : 0.0;
+      pdb = kdb == KIND_STD ? sb : 0.0;
+      pa = qa;
+      pb = qb;
+      pbv = hasb;
+      pending = true;
+      wave_sync();
+      continue;
+    }
     D2 v[6];
     {
       const unsigned *e = sE;
@@ -2342,10 +2372,12 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   if (pending) flush();
 }
 
-template __global__ void k_assemble2<KQED_ASM2_WAVES, double>(DT, size_t, const unsigned *, size_t, const double *,
-                                                              const unsigned *, void *, int, size_t);
-template __global__ void k_assemble2<KQED_ASM2_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
-                                                             const unsigned *, void *, int, size_t);
+template __global__ void k_assemble2<KQED_ASM2_WAVES, double, false>(DT, size_t, const unsigned *, size_t, const double *,
+                                                                     const unsigned *, void *, int, size_t, const double *);
+template __global__ void k_assemble2<KQED_ASM2_WAVES, float, false>(DT, size_t, const unsigned *, size_t, const double *,
+                                                                    const unsigned *, void *, int, size_t, const double *);
+template __global__ void k_assemble2<KQED_ASM2_WAVES, double, true>(DT, size_t, const unsigned *, size_t, const double *,
+                                                                    const unsigned *, void *, int, size_t, const double *);
 
 /* Several evaluation slots (L1, L2, L3, M-L2) in sorted order past the special pairs, two records per
  * wavefront as in k_assemble2: a slot is evaluated once for both records when their kinds agree

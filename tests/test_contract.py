@@ -100,3 +100,24 @@ def test_contract_through_combination_multi_pass(gpu_full, oracle_full):
     assert np.array_equal(s1, s2)
     ref, a = oracle_full.contract_comb(kqed_amd.KQED_COMB_CON_V2, x, T, y)
     assert (np.abs(s1 - ref) / a).max() <= 1e-12
+
+
+def test_contract_two_pair_bit_identical(gpu_full):
+    """A cell-sorted contraction (the two-pair kernels: the shortcut's dot of the tensor entries with
+    the site weights for L0..L3, the full per-pair dot of M-L2) gives the same sums, bit for bit, as
+    the unsorted one (one-pair kernel): the per-pair dots are identical and the site sums run in site
+    order either way."""
+    x, T, y = _lattice()
+    y = y[:3]
+    old = gpu_full.get_tune()
+    cases = ((0, 0.0), (1, 0.0), (2, 0.0), (3, 0.0), (kqed_amd.KQED_ML2, 0.5))
+    try:
+        gpu_full.set_tune(sort_min=1 << 30)
+        ref = [gpu_full.contract(w, x, T, y, M=M) for w, M in cases]
+        gpu_full.set_tune(old)
+        gpu_full.set_tune(sort_min=0)
+        got = [gpu_full.contract(w, x, T, y, M=M) for w, M in cases]
+    finally:
+        gpu_full.set_tune(old)
+    for (w, _), a, b in zip(cases, got, ref):
+        assert np.array_equal(a, b), w
