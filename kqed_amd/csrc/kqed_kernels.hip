@@ -1463,6 +1463,7 @@ k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *_
            unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
 {
   __shared__ unsigned sk[256], sp[256], sbase[256];
+  __shared__ unsigned long long swb[4];
   const int t = threadIdx.x;
   const size_t i = (size_t)blockIdx.x * 256 + t;
   unsigned k = 0xFFFFFFFFu; /* past the end of the batch: sorts last, never counted */
@@ -1494,18 +1495,33 @@ k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *_
       }
       __syncthreads();
     }
-  /* run starts: one atomic per run of equal keys for the run's length */
+  /* runs of equal keys: one atomic per run for the run's length.  The run boundaries of each
+   * wavefront are one ballot mask; a run's start is the last boundary at or before t and its end
+   * the first boundary after t (in this wavefront's mask or the next ones'). */
   const unsigned kt = sk[t];
-  const bool start = kt != 0xFFFFFFFFu && (t == 0 || sk[t - 1] != kt);
-  if (start) {
-    int e = t + 1;
-    while (e < 256 && sk[e] == kt) e++;
-    sbase[t] = atomicAdd(&count[kt], (unsigned)(e - t));
+  const bool bound = t == 0 || sk[t - 1] != kt;
+  const unsigned long long bm = __ballot(bound);
+  const int lane = t & 63, wv = t >> 6;
+  if (lane == 0) swb[wv] = bm;
+  __syncthreads();
+  const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1; /* bits 0..lane */
+  int e = 256;
+  if (bm & ~upto) {
+    e = wv * 64 + __builtin_ctzll(bm & ~upto);
+  } else {
+    for (int w2 = wv + 1; w2 < 4; w2++)
+      if (swb[w2]) { e = w2 * 64 + __builtin_ctzll(swb[w2]); break; }
   }
+  if (kt != 0xFFFFFFFFu && bound) sbase[t] = atomicAdd(&count[kt], (unsigned)(e - t));
   __syncthreads();
   if (kt != 0xFFFFFFFFu) {
-    int s0 = t;
-    while (s0 > 0 && sk[s0 - 1] == kt) s0--;
+    int s0 = 0;
+    if (bm & upto) {
+      s0 = wv * 64 + 63 - __builtin_clzll(bm & upto);
+    } else {
+      for (int w2 = wv - 1; w2 >= 0; w2--)
+        if (swb[w2]) { s0 = w2 * 64 + 63 - __builtin_clzll(swb[w2]); break; }
+    }
     rank[(size_t)blockIdx.x * 256 + sp[t]] = sbase[s0] + (unsigned)(t - s0);
   }
 }
