@@ -2400,8 +2400,8 @@ template __global__ void k_assemble2<KQED_ASM2_WAVES, double, true>(DT, size_t, 
  * (past the special pairs they do, unless a slot of one record is KIND_NONE, whose component is then
  * left unchanged), and once per record when they differ.  Each record's arithmetic is the one-pair
  * kernel's, in the same order (slot 0 stored, later slots fma'd with their coefficient, the dot
- * layouts summed over O in lane order, the M-L2 slots' alpha sums gathered in four rounds), so the
- * outputs are bit-identical to it.
+ * layouts summed over O in lane order, the M-L2 slots' alpha sums added in the order a = 0..3), so
+ * the outputs are bit-identical to it.
  * Layouts: 0/1 out[q][384] / out[384][ntot] of MT, 16 / 64 the contraction dot with an FP32 /
  * FP64 site tensor Tc (MT double). */
 template <int AW, typename MT>
