@@ -246,6 +246,17 @@ int kqed_surrogate_validate(kqed_surrogate *s, size_t n, const double *x_dev, co
 int kqed_batch_surrogate(const struct QED_kernel_temps *t, const kqed_surrogate *s, int which, double M,
                          size_t n, const double *x_dev, const double *y_dev, void *out_dev, int layout,
                          void *stream);
+/* L0 of n device pairs with the packed-path weight functions given by the caller instead of the
+ * tables: w_dev[n][34] per input pair, in the order g0: dcb dx dy | l2: dx dy val dcb | l4: dx dy
+ * val dcb | v1: dx dy val dcb | g2: val dy | dcb dydcb dcbdcb | dxdx dx dxdy dxdcb | g3: dxdx dx dxdy
+ * dxdcb | dcb dydcb dcbdcb | dy | swap regime only: g2 dydy, g3 dydy, each a derivative in the
+ * post-swap invariants (|x|, y', cb') of the weight function chnr_dS.c:50-95, chnr_dT.c:185-293 and
+ * chnr_dV.c:363-479 read from extractff2/extractff.  Pairs not on the packed path (Taylor region,
+ * special points, grid edges) ignore their row.  The chain-rule algebra and assembly then run
+ * unchanged (the step the surrogate feeds).  No reference counterpart: it exposes that step for
+ * checking.  Enqueued on `stream`; 0 or a negative error code. */
+int kqed_batch_inject(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
+                      const double *w_dev, void *out_dev, int layout, void *stream);
 /* One forward/backward pass on a host batch (inputs X[B][3], targets and weights [B][34]): loss and
  * the FP32 gradient of every parameter (kqed_surrogate_nparam() floats), for checking. */
 int kqed_surrogate_gradients(kqed_surrogate *s, int B, const float *X, const float *target,

@@ -103,6 +103,7 @@ def lib():
     L.kqed_surrogate_validate.argtypes = [vp, ctypes.c_size_t, vp, vp, ctypes.c_double, ctypes.c_int, arr]
     L.kqed_batch_surrogate.argtypes = [P, vp, ctypes.c_int, ctypes.c_double, ctypes.c_size_t, vp, vp, vp, ctypes.c_int,
                                        vp]
+    L.kqed_batch_inject.argtypes = [P, ctypes.c_size_t, vp, vp, vp, vp, ctypes.c_int, vp]
     L.kqed_surrogate_save.argtypes = [vp, ctypes.c_char_p]
     L.kqed_surrogate_load.argtypes = [vp, ctypes.c_char_p]
     L.kqed_comb_kernels.argtypes = [ctypes.c_int]
@@ -409,6 +410,16 @@ class QEDKernelTemps:
                               ctypes.c_void_p(stream or 0))
         if rc != 0:
             raise RuntimeError("kqed_batch failed: " + lib().kqed_last_error().decode())
+
+
+    def batch_inject_device(self, n, x_ptr, y_ptr, w_ptr, out_ptr, layout=KQED_AOS, stream=None):
+        """L0 of n device pairs with the packed-path weight functions taken from w_ptr ([n][34] FP64,
+        order of kqed_batch_inject) instead of the tables."""
+        rc = lib().kqed_batch_inject(ctypes.byref(self._t), int(n), ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr),
+                                     ctypes.c_void_p(w_ptr), ctypes.c_void_p(out_ptr), int(layout),
+                                     ctypes.c_void_p(stream or 0))
+        if rc != 0:
+            raise RuntimeError("kqed_batch_inject failed: " + lib().kqed_last_error().decode())
 
 
 class RaggedGraph:

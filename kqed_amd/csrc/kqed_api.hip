@@ -30,6 +30,7 @@ __global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs,
 __global__ void k_scan(unsigned *count, unsigned nb);
 __global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm);
 __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
+__global__ void k_inject_records(size_t n, double *rec, const unsigned *kinds, const double *w);
 template <int CLS>
 __global__ void k_interp(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys,
                          const unsigned *perm, const unsigned *bounds, int ilo, int ihi, size_t c0, size_t c1, int which,
@@ -112,6 +113,7 @@ static const int LAYOUT_SCALARS = 32;
 struct SurrUse {
   SurrNet net;
   const unsigned char *mask;
+  const double *inject = nullptr; /* weight functions from the caller instead of the network */
 };
 
 struct Workspace {
@@ -152,6 +154,8 @@ struct kqed_device {
   double *csc_vals = nullptr;
   /* trace tables of the perturbative lepton loop (built on first use) */
   signed char *pert_tab = nullptr;
+  /* every surrogate bin set: routes all packed-path pairs through the substitution step */
+  unsigned char *allmask = nullptr;
   std::mutex mu;
   std::unordered_map<void *, Workspace> ws;
   bool profiling = false;
@@ -224,6 +228,7 @@ extern "C" void free_QED_temps(struct QED_kernel_temps *t)
     (void)hipDeviceSynchronize();
     for (auto &kv : d->ws) free_ws(kv.second);
     (void)hipFree(d->pert_tab);
+    (void)hipFree(d->allmask);
     for (int i = 0; i < d->nalloc; i++) (void)hipFree(d->allocs[i]);
     free(d->maps);
     delete d;
@@ -513,7 +518,9 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
                          which, M, w.rec, w.kinds, smask, fuse);
       hipLaunchKernelGGL(k_interp<2>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
-      if (smask)
+      if (smask && sur->inject)
+        hipLaunchKernelGGL(k_inject_records, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds, sur->inject);
+      else if (smask)
         hipLaunchKernelGGL(k_surr_records, dim3((unsigned)((c + 63) / 64)), dim3(256), 0, s, c, w.rec, w.kinds, sur->net);
     } else {
       hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, -1, c0, c1,
@@ -1836,6 +1843,34 @@ extern "C" int kqed_batch_surrogate(const struct QED_kernel_temps *t, const kqed
   if (d->tune.sort_mode == KQED_SORT_NONE) d->tune.sort_mode = KQED_SORT_CELL;
   const int rc = run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
                            nullptr, &su);
+  d->tune = keep;
+  return rc;
+}
+
+/* L0 with the packed-path weight functions supplied by the caller (w_dev[n][34]): the pairs
+ * interpolated on the packed path (|x| in [XX[0], XX[last]), post-swap |y| below YY[last]) take
+ * w_dev's values instead of the tables and run the chain-rule algebra and assembly unchanged; every
+ * other pair is evaluated from the tables.  No counterpart in the reference: it checks the algebra
+ * on given inputs, the step the surrogate feeds. */
+extern "C" int kqed_batch_inject(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
+                                 const double *w_dev, void *out_dev, int layout, void *stream)
+{
+  if (!t || !t->dev || !w_dev || !layout_ok(layout)) { set_err("kqed_batch_inject: bad argument"); return -1; }
+  if (n == 0) return 0;
+  if (n > 0xFFFFFFFFull) { set_err("kqed_batch_inject: at most 2^32-1 pairs per call"); return -1; }
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  if (!d->allmask) {
+    std::vector<unsigned char> ones(NBB, 1);
+    if (hipMalloc(&d->allmask, NBB) != hipSuccess) { d->allmask = nullptr; set_err("device allocation failed"); return -2; }
+    if (hipMemcpy(d->allmask, ones.data(), NBB, hipMemcpyHostToDevice) != hipSuccess) { set_err("mask upload failed"); return -3; }
+  }
+  SurrUse su = { SurrNet(), d->allmask, w_dev };
+  kqed_tune keep = d->tune;
+  d->tune.sort_min = 0; /* the substitution runs inside the sorted pipeline only */
+  if (d->tune.sort_mode == KQED_SORT_NONE) d->tune.sort_mode = KQED_SORT_CELL;
+  const int rc = run_pairs(d, d->ws[stream], (hipStream_t)stream, KQED_L0, 0.0, n, x_dev, 4, y_dev, 4, out_dev, layout,
+                           nullptr, nullptr, &su);
   d->tune = keep;
   return rc;
 }

@@ -1384,6 +1384,31 @@ k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds
   algebra_main(r + KQED_REC_MAIN, r + KQED_REC_X, r + KQED_REC_Y);
 }
 
+/* Caller-supplied weight functions into the stage-1 records of the packed-path pairs the
+ * interpolation kernel flagged (KQED_SURR_BIT): w[p][34] for input pair p, the 34 in the order of
+ * KQED_SURR_SLOTS except that the last two are the second y derivatives of g2 and g3 (swap regime
+ * only; ddg1dydy follows from them as in the interpolation, chnr_dV.c:466). */
+__global__ void __launch_bounds__(256)
+k_inject_records(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds, const double *__restrict__ w)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !(kinds[i] & KQED_SURR_BIT)) return;
+  double *r = rec + i * KQED_REC_LEN;
+  double *m = r + KQED_REC_MAIN;
+  const double *src = w + (size_t)r[KQED_REC_P] * KQED_NSURR;
+  const int slots[KQED_NSURR] = KQED_SURR_SLOTS;
+  for (int k = 0; k < KQED_NSURR - 2; k++) m[slots[k]] = src[k];
+  Stage1 *st = (Stage1 *)m;
+  if (st->flag2 != 0.0) {
+    const double ddg2dydy = src[KQED_NSURR - 2], ddg3dydy = src[KQED_NSURR - 1];
+    st->V.ddg2dydy = ddg2dydy;
+    st->V.ddg1dydy = ddg3dydy - (st->y * ddg2dydy + 2.0 * st->V.dg2dy) * st->cb / st->x;
+  } else {
+    st->V.ddg2dydy = 0.0;
+    st->V.ddg1dydy = 0.0;
+  }
+}
+
 /* ------------------------------------------------------------------ Taylor-table primitives */
 /* Tabd_xeq0 lerps (reference Tabd.c:23-69): out = v[4], y, ysq, fx, fy, ell2, dell2adx, dell2ady,
  * ell3a, dg2dx, ddg2dxdy, dg1dx, dg1dy, phi1, phi2.  Returns 1 when |v| is past the table. */

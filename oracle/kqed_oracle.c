@@ -12,6 +12,22 @@
  *   - Clenshaw sums run in the FMA form of simd.c:79-118 with ratio look-up tables.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this file's
  * library.  Compile with -ffp-contract=off.
+ *
+ * What pins each part (tests/):
+ *   table readers/writers, CRC       shipped Taylor tables verify, byte-identical rewrites
+ *   Tabd_xeq0/yeq0 + EQZERO map      REGexample2/10 golden columns (6 digits)
+ *   Clenshaw sums                    exact U_k^(d) sums and the trigonometric form
+ *   assembly maps                    Dirac-trace derivation = the reference's unrolled text
+ *   chnr_dS/dT/dV algebra, swap      autodiff of the reference's stated definitions on injected
+ *                                    weight functions (oq_stv_inject)
+ *   interpolation (getff6/7, accessv6/7, extractff/extractff2, Hermite, lerp)
+ *                                    closed forms on a table whose interpolation is exact
+ *   Taylor-x blends (S, T, 13 V inputs)  continuity across x = XX[0]
+ *   L1/L2/L3/M-L2                    identities of the reference's examples 3, 9, 11, 15
+ *   parity unpinned: the x -> 0 end of the Taylor-x blends (the Taylor-x table rows as used in
+ *   chnr_dS.c:10-45, chnr_dT.c:16-143, chnr_dV.c:19-174) and the small-x forms of dg1dx, dg1dcb,
+ *   dg2dx, dg2dcb (chnr_dV.c:52-58): no reference output or closed form checks them without the
+ *   real form-factor table.
  */
 #include "kqed_oracle.h"
 
@@ -34,6 +50,16 @@ double oq_flops_get(void) { return g_flops; }
  * simd.c:343-613): work that depends only on the grid node */
 double oq_flops_build_get(void) { return g_flops_build; }
 void oq_flops_reset(void) { g_flops = 0.0; g_flops_build = 0.0; }
+
+/* ------------------------------------------------------------------ test hooks
+ * g_inj: when set, the standard branches of chnr_dS/dT/dV take their interpolated weight
+ * functions from this array (the 34 values in the order of oq_stv_inject) instead of the tables.
+ * g_plant: a deliberate mistake switched on by tests that show a pin catches it (0 = none). */
+static __thread const double *g_inj = NULL;
+static __thread double *g_vout = NULL; /* when set, chnr_dV copies its 27 g1/g2/g3 derivatives here */
+static int g_plant = 0;
+void oq_plant(int k) { g_plant = k; }
+#define PLANT(k) (g_plant == (k))
 
 /* ------------------------------------------------------------------ CRC-32C (crc32c.c:160-172) */
 static uint32_t crc_tab[256];
@@ -271,7 +297,7 @@ double oq_chebU_avx(int order, int nk, double x, const double *co, const oq_tabl
     FL(3.0 * nk);
     return yb;
   }
-  const double *PA = T->LUT[2 * order - 1], *PB = T->LUT[2 * order];
+  const double *PA = T->LUT[2 * order - 1] + (PLANT(13) && order == 1 ? 1 : 0), *PB = T->LUT[2 * order];
   for (int i = 0; i < nk - order; i++, j--) {
     ytmp = yb;
     ya = fma(PB[j], ya, -co[j]);
@@ -477,7 +503,7 @@ static int build_arrays(const oq_tables *T, const oq_pc *PC, int nm, int ix, con
     for (int j = 0; j < mm; j++) { facm[l] = facm[l] * y1[l]; facp[l] = facp[l] * y12[l]; }
   }
   mshm += mm; mshp += mm;
-  MSHM = mshm; MSHP = mshp;
+  MSHM = mshm; MSHP = PLANT(12) ? mshp + 1 : mshp;
   const double *pFm1 = PC->Fm[nm], *pFm2 = PC->Fm[dm], *pFp1 = PC->Fp[nm], *pFp2 = PC->Fp[dm];
   if (!calc) for (int l = 0; l < 4; l++) { facm[l] = facm[l] * y1[l]; facp[l] = facp[l] * y1[l]; }
   for (int j = 0; j < nf_max; j++) {
@@ -573,8 +599,8 @@ static void interpol5(double F[4], const oq_ip *P, const double f[12])
   const double Y3[4] = { f[11], 0.0, f[7], f[3] };
   const double a[4] = { P->A, 1.0 / P->D, P->A, P->A };
   const double b[4] = { P->B, P->lA / P->D, P->B, P->B };
-  const double c1[4] = { P->C1, -P->lA / P->D, P->C1, P->C1 };
-  const double c2[4] = { P->C2, 0.0, P->C2, P->C2 };
+  const double c1[4] = { P->C1, -P->lA / P->D, P->C1, PLANT(11) ? P->C2 : P->C1 };
+  const double c2[4] = { P->C2, 0.0, P->C2, PLANT(11) ? P->C1 : P->C2 };
   for (int k = 0; k < 4; k++) {
     double t = Y0[k] * a[k];
     t = fma(Y1[k], b[k], t);
@@ -645,7 +671,7 @@ static int dg_taylorx(const oq_tables *T, double *dg0dy, double *dg0dx, double *
   const double ax = (T->XX[0] - I->x) / (T->XX[0]);
   double f1 = I->cb * lerp(ay, T->TX[G0dx][iy_tay], T->TX[G0dx][iy2_tay]);
   double f2 = accessv(T, false, true, ix, iy, dxQG0, false, 0, I->cb, I->y);
-  *dg0dx = lerp(ax, f1, f2);
+  *dg0dx = PLANT(10) ? lerp(ax, f2, f1) : lerp(ax, f1, f2);
   f1 = lerp(ay, T->TX[G0dy][iy_tay], T->TX[G0dy][iy2_tay]);
   f2 = accessv(T, false, false, ix, iy, QG0, true, 0, I->cb, I->y);
   *dg0dy = lerp(ax, f1, f2);
@@ -663,6 +689,8 @@ static int chnr_dS(const oq_tables *T, const oq_pc *PC, const double xv[4], cons
     if (dg_taylorx(T, &dg0dy, &dg0dx, &dg0dcb, I)) return 1;
   } else if (I->x > T->XX[T->nstpx - 1]) {
     return 1;
+  } else if (g_inj) {
+    dg0dcb = g_inj[0]; dg0dx = g_inj[1]; dg0dy = g_inj[2];
   } else {
     double f[4];
     extractff2(T, PC, QG0, 0, I, f);
@@ -673,7 +701,7 @@ static int chnr_dS(const oq_tables *T, const oq_pc *PC, const double xv[4], cons
     const double dgs0dx = dg0dx, dgs0dcb = dg0dcb, dgs0dy = dg0dy;
     const double ca = I->cb;
     const double cd = (I->yorig - I->x * I->cborig) / I->xmy;
-    dg0dx = dgs0dx + (1.0 - ca * ca) / I->xmy * dgs0dcb + ca * dgs0dy;
+    dg0dx = dgs0dx + (PLANT(8) ? 1.0 + ca * ca : 1.0 - ca * ca) / I->xmy * dgs0dcb + ca * dgs0dy;
     dg0dcb = -(I->ysq * cd * dgs0dcb + I->x * I->yorig * I->xmy * dgs0dy) / I->xmysq;
     dg0dy = -(I->cborig + ca * cd) / I->xmy * dgs0dcb + cd * dgs0dy;
     FL(30);
@@ -681,7 +709,7 @@ static int chnr_dS(const oq_tables *T, const oq_pc *PC, const double xv[4], cons
   for (int bet = 0; bet < 4; bet++) {
     const double yhat = yv[bet] / I->yorig;
     const double xhat = xv[bet] / I->x;
-    const double c2 = (xhat - I->cborig * yhat) / I->yorig;
+    const double c2 = (PLANT(1) ? xhat + I->cborig * yhat : xhat - I->cborig * yhat) / I->yorig;
     const double c4 = (yhat - I->cborig * xhat) / I->x;
     dyv[bet] = yhat * dg0dy + c2 * dg0dcb;
     dxv[bet] = xhat * dg0dx + c4 * dg0dcb;
@@ -741,7 +769,7 @@ static int taylorx_T(const oq_tables *T_, ttmps *T, const oq_inv *I)
   const double dell3adcb = 0.0;
   const double dell3bdcb = (dell4bdcb / (2.0 * xb * ysq) - y * ell2b - y * cb * dell2bdcb) / xb;
   T->dell3dcb = lerp(ax, dell3adcb, dell3bdcb);
-  const double dell3bdx = (-ell4b / (xb * ysq) + dell4bdx / (2.0 * ysq) + y * cb * ell2b
+  const double dell3bdx = (-ell4b / (xb * ysq) + dell4bdx / (2.0 * ysq) + (PLANT(9) ? 0.0 : y * cb * ell2b)
                            - y * xb * cb * dell2bdx) / xbsq;
   T->dell3dx = dell3bdx;
   T->dell1dy = 4.0 / (3.0 * xbsq * xbsq) * (dv1bdy - 2.0 * xbsq * y * (cb * cb - 0.25) * ell2b
@@ -764,7 +792,7 @@ static void xmyswap_T(ttmps *T, const oq_inv *I)
   const double cd = (I->yorig - I->x * I->cborig) / I->xmy;
   const double cb = I->cb;
   T->ell2 = ell2s;
-  T->dell2dx = dells2dx + (1.0 - cb * cb) / I->xmy * dells2dcb + cb * dells2dy;
+  T->dell2dx = dells2dx + (1.0 - cb * cb) / I->xmy * dells2dcb + (PLANT(4) ? -cb : cb) * dells2dy;
   T->dell2dcb = -(I->ysq * cd * dells2dcb + I->x * I->yorig * I->xmy * dells2dy) / I->xmysq;
   T->dell2dy = -(I->cborig + cb * cd) / I->xmy * dells2dcb + cd * dells2dy;
   const double ell3s = -T->ell3, dells3dx = -T->dell3dx, dells3dcb = -T->dell3dcb, dells3dy = -T->dell3dy;
@@ -794,21 +822,28 @@ static int chnr_dT(const oq_tables *T_, const oq_pc *PC, const double xv[4], con
   } else {
     double f[4];
     const double x = I->x, y = I->y, cb = I->cb, xsq = I->xsq;
-    extractff2(T_, PC, QL2, 0, I, f);
-    if (sc) memcpy(sc, f, sizeof f);
-    T.dell2dx = f[1]; T.dell2dy = f[2]; T.ell2 = f[3]; T.dell2dcb = f[0];
-    extractff2(T_, PC, QL4, 0, I, f);
-    if (sc) memcpy(sc + 4, f, sizeof f);
-    T.dell4dx = f[1]; T.dell4dy = f[2]; T.ell4 = f[3]; T.dell4dcb = f[0];
-    extractff2(T_, PC, QG1, 0, I, f);
-    if (sc) memcpy(sc + 8, f, sizeof f);
-    T.dv1dx = f[1]; T.dv1dy = f[2]; T.v1 = f[3]; T.dv1dcb = f[0];
+    if (g_inj) {
+      T.dell2dx = g_inj[3]; T.dell2dy = g_inj[4]; T.ell2 = g_inj[5]; T.dell2dcb = g_inj[6];
+      T.dell4dx = g_inj[7]; T.dell4dy = g_inj[8]; T.ell4 = g_inj[9]; T.dell4dcb = g_inj[10];
+      T.dv1dx = g_inj[11]; T.dv1dy = g_inj[12]; T.v1 = g_inj[13]; T.dv1dcb = g_inj[14];
+    } else {
+      extractff2(T_, PC, QL2, 0, I, f);
+      if (sc) memcpy(sc, f, sizeof f);
+      T.dell2dx = f[1]; T.dell2dy = f[2]; T.ell2 = f[3]; T.dell2dcb = f[0];
+      extractff2(T_, PC, QL4, 0, I, f);
+      if (sc) memcpy(sc + 4, f, sizeof f);
+      T.dell4dx = f[1]; T.dell4dy = f[2]; T.ell4 = f[3]; T.dell4dcb = f[0];
+      extractff2(T_, PC, QG1, 0, I, f);
+      if (sc) memcpy(sc + 8, f, sizeof f);
+      T.dv1dx = f[1]; T.dv1dy = f[2]; T.v1 = f[3]; T.dv1dcb = f[0];
+    }
     T.ell3 = T.ell4 / (2.0 * xsq * ysq) - y * cb * T.ell2 / x;
     T.ell1 = 4.0 / (3.0 * xsq * xsq) * (T.v1 - xsq * ysq * (cb * cb - 0.25) * T.ell2
                                         - 1.5 * xsq * x * y * cb * T.ell3);
     T.dell3dy = (-T.ell4 / (ysq * y) + T.dell4dy / (2.0 * ysq) - cb * x * T.ell2 - y * x * cb * T.dell2dy) / (xsq);
     T.dell3dcb = (T.dell4dcb / (2.0 * x * ysq) - y * T.ell2 - y * cb * T.dell2dcb) / x;
-    T.dell3dx = (-T.ell4 / (x * ysq) + T.dell4dx / (2.0 * ysq) + y * cb * T.ell2 - y * x * cb * T.dell2dx) / xsq;
+    T.dell3dx = (-T.ell4 / (x * ysq) + T.dell4dx / (2.0 * ysq) + (PLANT(2) ? 0.0 : y * cb * T.ell2)
+                 - y * x * cb * T.dell2dx) / xsq;
     T.dell1dy = 4.0 / (3.0 * xsq * xsq) * (T.dv1dy - 2.0 * xsq * y * (cb * cb - 0.25) * T.ell2
                                            - xsq * ysq * (cb * cb - 0.25) * T.dell2dy
                                            - 1.5 * xsq * x * cb * T.ell3 - 1.5 * xsq * x * y * cb * T.dell3dy);
@@ -832,7 +867,7 @@ static int chnr_dT(const oq_tables *T_, const oq_pc *PC, const double xv[4], con
       for (int dta = 0; dta < 4; dta++) {
         const int d_bd = (bet == dta), d_ad = (alf == dta);
         const double t1 = d_ab * yv[dta] + d_bd * yv[alf] - 0.5 * d_ad * yv[bet];
-        const double t2 = d_ab * xv[dta] + d_bd * xv[alf] - 0.5 * d_ad * xv[bet];
+        const double t2 = d_ab * xv[dta] + d_bd * xv[PLANT(3) ? bet : alf] - 0.5 * d_ad * xv[bet];
         const double t3 = xv[alf] * xv[dta] - d_ad * I->xsq / 4;
         const double t4 = yv[alf] * yv[dta] - d_ad * I->ysq / 4;
         const double t5 = xv[alf] * yv[dta] + yv[alf] * xv[dta] - 0.5 * I->xdoty * d_ad;
@@ -898,7 +933,7 @@ static int taylorx_V(const oq_tables *T, vtmp *D, double x, double y, double xsq
   const double dg1bdy = dg3bdy - cb / xb * g2b - (y / xb) * cb * dg2bdy;
   const double ddg1bdxdx = ddg3bdxdx - 2.0 * y / (xbsq * xb) * cb * g2b + y / xbsq * cb * dg2bdx
     + y / xbsq * cb * dg2bdx - y / xb * cb * ddg2bdxdx;
-  const double ddg1bdxdy = ddg3bdxdy + cb / xbsq * g2b + y * cb / xbsq * dg2bdy - cb / xb * dg2bdx
+  const double ddg1bdxdy = ddg3bdxdy + (PLANT(14) ? 0.0 : cb / xbsq * g2b) + y * cb / xbsq * dg2bdy - cb / xb * dg2bdx
     - y / xb * cb * ddg2bdxdy;
   const double ddg1bdxdcb = ddg3bdxdcb + y / xbsq * g2b + y / xbsq * cb * dg2bdcb - y / xb * dg2bdx
     - y / xb * cb * ddg2bdxdcb;
@@ -961,7 +996,7 @@ static void xmyswap_V(vtmp *D, double x, double xmy, double y, double xsq, doubl
   D->ddg2dxdcb = ysq / (xmysq * xmy) * (cborig + 3.0 * ca * cd) * F.dg2dcb
     - ysq / xmysq * cd * (F.ddg2dxdcb + (1.0 - ca * ca) / xmy * F.ddg2dcbdcb + ca * F.ddg2dydcb)
     + y / xmysq * (x * ca - xmy) * F.dg2dy - x * y / xmy * (F.ddg2dxdy + (1.0 - ca * ca) / xmy * F.ddg2dydcb
-                                                          + ca * F.ddg2dydy);
+                                                          + (PLANT(7) ? -ca : ca) * F.ddg2dydy);
   D->ddg2dxdy = (-(cborig + ca * cd) / xmy * F.ddg2dxdcb + cd * F.ddg2dxdy)
     + (2.0 * ca * cborig + cd * (3.0 * ca * ca - 1.0)) / xmysq * F.dg2dcb
     + (1.0 - ca * ca) / xmy * (-(cborig + ca * cd) / xmy * F.ddg2dcbdcb + cd * F.ddg2dydcb)
@@ -1032,7 +1067,7 @@ static void get_dv(double x, const double xv[4], double y, const double yv[4], d
         const int d_bd = (bet == dta), d_ad = (alf == dta);
         double f = d_bd * astuff;
         f += d_ad * bstuff;
-        f += D1[dta] * xaxb + D2[dta] * xayb;
+        f += D1[dta] * (PLANT(6) ? xayb : xaxb) + D2[dta] * xayb;
         f += (D3[dta] * xhat[bet] + D4[dta] * yhat[bet]) * c4v[alf];
         f += (D5[dta] * xhat[alf] + D6[dta] * c4v[alf]) * c2c4;
         f += D7[dta] * (d_ab - xaxb);
@@ -1055,34 +1090,47 @@ static int chnr_dV(const oq_tables *T, const oq_pc *PC, const double xv[4], cons
   } else {
     const double x = I->x, y = I->y, cb = I->cb, xsq = I->xsq;
     double f[4];
-    D.dg3dy = extractff(T, PC, QG3, true, 0, I);
-    if (sc) sc[0] = D.dg3dy;
-    extractff2(T, PC, QG2, 0, I, f);
-    if (sc) memcpy(sc + 1, f, sizeof f);
-    D.g2 = f[3]; D.dg2dy = f[2];
-    extractff2(T, PC, dxQG2, 0, I, f);
-    if (sc) memcpy(sc + 5, f, sizeof f);
-    D.ddg2dxdx = f[1]; D.dg2dx = f[3]; D.ddg2dxdy = f[2]; D.ddg2dxdcb = f[0];
-    extractff2(T, PC, dxQG3, 0, I, f);
-    if (sc) memcpy(sc + 9, f, sizeof f);
-    D.ddg3dxdx = f[1]; D.dg3dx = f[3]; D.ddg3dxdy = f[2]; D.ddg3dxdcb = f[0];
-    extractff2(T, PC, QG2, 1, I, f);
-    if (sc) memcpy(sc + 13, f, sizeof f);
-    D.dg2dcb = f[3]; D.ddg2dydcb = f[2]; D.ddg2dcbdcb = f[0];
-    extractff2(T, PC, QG3, 1, I, f);
-    if (sc) memcpy(sc + 17, f, sizeof f);
-    D.dg3dcb = f[3]; D.ddg3dydcb = f[2]; D.ddg3dcbdcb = f[0];
+    if (g_inj) {
+      D.g2 = g_inj[15]; D.dg2dy = g_inj[16];
+      D.dg2dcb = g_inj[17]; D.ddg2dydcb = g_inj[18]; D.ddg2dcbdcb = g_inj[19];
+      D.ddg2dxdx = g_inj[20]; D.dg2dx = g_inj[21]; D.ddg2dxdy = g_inj[22]; D.ddg2dxdcb = g_inj[23];
+      D.ddg3dxdx = g_inj[24]; D.dg3dx = g_inj[25]; D.ddg3dxdy = g_inj[26]; D.ddg3dxdcb = g_inj[27];
+      D.dg3dcb = g_inj[28]; D.ddg3dydcb = g_inj[29]; D.ddg3dcbdcb = g_inj[30];
+      D.dg3dy = g_inj[31];
+    } else {
+      D.dg3dy = extractff(T, PC, QG3, true, 0, I);
+      if (sc) sc[0] = D.dg3dy;
+      extractff2(T, PC, QG2, 0, I, f);
+      if (sc) memcpy(sc + 1, f, sizeof f);
+      D.g2 = f[3]; D.dg2dy = f[2];
+      extractff2(T, PC, dxQG2, 0, I, f);
+      if (sc) memcpy(sc + 5, f, sizeof f);
+      D.ddg2dxdx = f[1]; D.dg2dx = f[3]; D.ddg2dxdy = f[2]; D.ddg2dxdcb = f[0];
+      extractff2(T, PC, dxQG3, 0, I, f);
+      if (sc) memcpy(sc + 9, f, sizeof f);
+      D.ddg3dxdx = f[1]; D.dg3dx = f[3]; D.ddg3dxdy = f[2]; D.ddg3dxdcb = f[0];
+      extractff2(T, PC, QG2, 1, I, f);
+      if (sc) memcpy(sc + 13, f, sizeof f);
+      D.dg2dcb = f[3]; D.ddg2dydcb = f[2]; D.ddg2dcbdcb = f[0];
+      extractff2(T, PC, QG3, 1, I, f);
+      if (sc) memcpy(sc + 17, f, sizeof f);
+      D.dg3dcb = f[3]; D.ddg3dydcb = f[2]; D.ddg3dcbdcb = f[0];
+    }
     D.dg1dx = D.dg3dx + y / xsq * cb * D.g2 - y / x * cb * D.dg2dx;
     D.dg1dy = D.dg3dy - cb / x * D.g2 - (y / x) * cb * D.dg2dy;
     D.dg1dcb = D.dg3dcb - y / x * D.g2 - y / x * cb * D.dg2dcb;
     D.ddg1dxdx = D.ddg3dxdx - 2.0 * y / (xsq * x) * cb * D.g2 + y / xsq * cb * D.dg2dx
       + y / xsq * cb * D.dg2dx - y / x * cb * D.ddg2dxdx;
-    D.ddg1dxdy = D.ddg3dxdy + cb / xsq * D.g2 + y * cb / xsq * D.dg2dy - cb / x * D.dg2dx - y / x * cb * D.ddg2dxdy;
+    D.ddg1dxdy = D.ddg3dxdy + (PLANT(5) ? 0.0 : cb / xsq * D.g2) + y * cb / xsq * D.dg2dy - cb / x * D.dg2dx
+      - y / x * cb * D.ddg2dxdy;
     D.ddg1dxdcb = D.ddg3dxdcb + y / xsq * D.g2 + y / xsq * cb * D.dg2dcb - y / x * D.dg2dx - y / x * cb * D.ddg2dxdcb;
     D.ddg1dydcb = D.ddg3dydcb - D.g2 / x - cb / x * D.dg2dcb - y / x * D.dg2dy - y / x * cb * D.ddg2dydcb;
     D.ddg1dcbdcb = D.ddg3dcbdcb - 2.0 * y / x * D.dg2dcb - y / x * cb * D.ddg2dcbdcb;
     FL(120);
-    if (I->flag2) {
+    if (I->flag2 && g_inj) {
+      D.ddg2dydy = g_inj[32];
+      D.ddg1dydy = g_inj[33] - (y * D.ddg2dydy + 2.0 * D.dg2dy) * cb / x;
+    } else if (I->flag2) {
       const int iy1 = oq_bsrch(T->YY, y, 0, T->nstpy);
       if (iy1 == T->nstpy - 1) return 1;
       oq_inv I1 = *I, I2 = *I;
@@ -1101,6 +1149,7 @@ static int chnr_dV(const oq_tables *T, const oq_pc *PC, const double xv[4], cons
     }
   }
   if (I->flag2) xmyswap_V(&D, I->x, I->xmy, I->yorig, I->xsq, I->xmysq, I->ysq, I->cb, I->cborig);
+  if (g_vout) memcpy(g_vout, &D, sizeof D);
   get_dv(I->x, xv, I->yorig, yv, I->cborig, &D, dv);
   return 0;
 }
@@ -1358,6 +1407,24 @@ int oq_stv(const oq_tables *T, const double x[4], const double y[4], double stv[
   return stv_full(T, x, y, stv, NULL);
 }
 
+int oq_stv_inject(const oq_tables *T, const double x[4], const double y[4], const double w[34], double stv[200])
+{
+  g_inj = w;
+  const int rc = stv_full(T, x, y, stv, NULL);
+  g_inj = NULL;
+  return rc;
+}
+
+int oq_vterms(const oq_tables *T, const double x[4], const double y[4], double v[27])
+{
+  double stv[200];
+  memset(v, 0, 27 * sizeof(double));
+  g_vout = v;
+  const int rc = stv_full(T, x, y, stv, NULL);
+  g_vout = NULL;
+  return rc;
+}
+
 int oq_interp_scalars(const oq_tables *T, const double x[4], const double y[4], double s[45])
 {
   double stv[200];
@@ -1413,6 +1480,14 @@ void oq_L0(const oq_tables *T, const double xv[4], const double yv[4], double k[
     return;
   }
   if (!stv_full(T, xv, yv, t, NULL)) apply_map(g_Mstd, 200, t, k);
+}
+
+void oq_L0_inject(const oq_tables *T, const double xv[4], const double yv[4], const double w[34], double k[384])
+{
+  maps_init();
+  double t[200];
+  memset(k, 0, 384 * sizeof(double));
+  if (!oq_stv_inject(T, xv, yv, w, t)) apply_map(g_Mstd, 200, t, k);
 }
 
 /* atomic_kernel_Saxpy (kernels.c:52-78), FMA build */

@@ -112,6 +112,31 @@ int oq_interp_scalars(const oq_tables *T, const double x[4], const double y[4], 
  * +8 when the x<->x-y swap is active. */
 int oq_regime(const oq_tables *T, const double x[4], const double y[4]);
 
+/* S/T/V tensors (layout of oq_stv) and L0 of a standard-regime pair (XX[0] <= |x| <= XX[last])
+ * with the interpolated weight functions taken from w instead of the tables: the chain-rule
+ * algebra of chnr_dS.c:50-95, chnr_dT.c:185-293, chnr_dV.c:363-479 (swap included) on given
+ * inputs.  w, 34 values in the order of the device library's kqed_batch_inject:
+ *   g0: dcb dx dy | l2: dx dy val dcb | l4: dx dy val dcb | v1: dx dy val dcb |
+ *   g2: val dy | dcb dydcb dcbdcb | dxdx dx dxdy dxdcb | g3: dxdx dx dxdy dxdcb |
+ *   dcb dydcb dcbdcb | dy | swap only: g2 dydy, g3 dydy
+ * (the last two replace the finite differences of chnr_dV.c:444-466).  Returns 1 as oq_stv. */
+int oq_stv_inject(const oq_tables *T, const double x[4], const double y[4], const double w[34], double stv[200]);
+void oq_L0_inject(const oq_tables *T, const double x[4], const double y[4], const double w[34], double k[384]);
+
+/* The g1/g2/g3 derivatives chnr_dV hands to its tensor step (struct vtmp of chnr_dV.c, in field
+ * order: g2, dg1dx, dg1dy, dg1dcb, dg2dx, dg2dy, dg2dcb, dg3dx, dg3dy, dg3dcb, ddg2dxdx, ddg2dxdy,
+ * ddg2dxdcb, ddg2dydcb, ddg2dcbdcb, ddg3dxdx, ddg3dxdy, ddg3dxdcb, ddg3dydcb, ddg3dcbdcb,
+ * ddg1dxdx, ddg1dxdy, ddg1dxdcb, ddg1dydcb, ddg1dcbdcb, ddg1dydy, ddg2dydy), after the swap. */
+int oq_vterms(const oq_tables *T, const double x[4], const double y[4], double v[27]);
+
+/* Deliberate mistakes for tests that show a pin catches them (0 switches them off): 1 chnr_dS c2
+ * sign, 2 chnr_dT dell3dx term dropped, 3 chnr_dT t2 index, 4 swap-T chain-rule sign, 5 chnr_dV
+ * ddg1dxdy term dropped, 6 get_dv operand transposed, 7 swap-V chain-rule sign, 8 swap-S factor,
+ * 9 Taylor-x dell3dx term dropped, 10 Taylor-x dg0dx blend reversed, 11 x-Hermite weights
+ * exchanged, 12 d/dy array exponent, 13 Clenshaw ratio table offset, 14 Taylor-x ddg1dxdy term
+ * dropped. */
+void oq_plant(int k);
+
 /* Algorithmic FP64 flop count of the last oq_L0 calls on this thread (reference algorithm). */
 double oq_flops_get(void);
 double oq_flops_build_get(void); /* part spent on per-pair coefficient arrays (node-only work) */

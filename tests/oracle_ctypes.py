@@ -55,6 +55,10 @@ def _lib():
     L.oq_assembly_maps.argtypes = [v4, v4]
     L.oq_interp_scalars.argtypes = [T, v4, v4, v4]
     L.oq_regime.argtypes = [T, v4, v4]
+    L.oq_stv_inject.argtypes = [T, v4, v4, v4, v4]
+    L.oq_L0_inject.argtypes = [T, v4, v4, v4, v4]
+    L.oq_plant.argtypes = [ctypes.c_int]
+    L.oq_vterms.argtypes = [T, v4, v4, v4]
     L.oq_flops_get.restype = ctypes.c_double
     L.oq_flops_build_get.restype = ctypes.c_double
     L.oq_crc32c.argtypes = [ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
@@ -187,6 +191,29 @@ class Oracle:
         s = np.zeros(45)
         reg = self.L.oq_interp_scalars(ctypes.byref(self.t), np.asarray(x, np.float64), np.asarray(y, np.float64), s)
         return reg, s
+
+    def stv_inject(self, x, y, w):
+        """S/T/V (oq_stv layout) of a standard-regime pair with the 34 weight functions w injected."""
+        s = np.zeros(200)
+        rc = self.L.oq_stv_inject(ctypes.byref(self.t), np.asarray(x, np.float64), np.asarray(y, np.float64),
+                                  np.ascontiguousarray(w, np.float64), s)
+        return rc, s
+
+    def L0_inject(self, x, y, w):
+        k = np.zeros(384)
+        self.L.oq_L0_inject(ctypes.byref(self.t), np.asarray(x, np.float64), np.asarray(y, np.float64),
+                            np.ascontiguousarray(w, np.float64), k)
+        return k
+
+    def vterms(self, x, y):
+        """The 27 g1/g2/g3 derivatives chnr_dV passes to its tensor step (oq_vterms)."""
+        v = np.zeros(27)
+        rc = self.L.oq_vterms(ctypes.byref(self.t), np.asarray(x, np.float64), np.asarray(y, np.float64), v)
+        return rc, v
+
+    def plant(self, k):
+        """Switch on deliberate mistake k of the oracle (0 = none); see oq_plant."""
+        self.L.oq_plant(int(k))
 
     def regime(self, x, y):
         return self.L.oq_regime(ctypes.byref(self.t), np.asarray(x, np.float64), np.asarray(y, np.float64))
