@@ -211,8 +211,11 @@ int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t 
 int kqed_contract(const struct QED_kernel_temps *t, int which, double M, size_t nx, const double *x_dev,
                   const float *T_dev, size_t ny, const double *y, double *s, void *stream);
 
-/* Bytes of device scratch kqed_batch needs for n pairs (allocated and cached by the library). */
+/* Bytes of device scratch kqed_batch needs for n pairs (allocated and cached by the library, per
+ * stream): kqed_batch_workspace assumes the default pass of 2^21 pairs and a table of at most
+ * 100 x 4096 nodes; kqed_batch_workspace_tables uses the table set's own grid and current tune. */
 size_t kqed_batch_workspace(size_t n);
+size_t kqed_batch_workspace_tables(const struct QED_kernel_temps *t, size_t n);
 
 /* Stage timing of kqed_batch on its own stream (HIP events around every kernel launch).
  * Reads and resets the accumulated milliseconds and launch counts of the stages

@@ -116,6 +116,18 @@ struct SurrUse {
   const double *inject = nullptr; /* weight functions from the caller instead of the network */
 };
 
+/* Device copies of the group offsets and the per-pair y vectors of a ragged batch. */
+struct RaggedBufs {
+  size_t n = 0, ng = 0;
+  unsigned long long *off = nullptr;
+  double *ypair = nullptr;
+  void release()
+  {
+    (void)hipFree(off); (void)hipFree(ypair);
+    off = nullptr; ypair = nullptr; n = ng = 0;
+  }
+};
+
 struct Workspace {
   size_t cap = 0, ncap = 0, nbuckets = 0;
   unsigned *key = nullptr, *rank = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
@@ -133,6 +145,8 @@ struct Workspace {
   /* lepton loop: pihat1 evaluations [3 pcap][1280], iPihat [pcap][384], scaled pairs [2][pcap][4] */
   size_t pcap = 0;
   double *pV = nullptr, *pP = nullptr, *pX = nullptr;
+  /* ragged batches enqueued on this stream: group offsets and per-pair y vectors */
+  RaggedBufs rb;
 };
 
 /* optional per-stage timing: events recorded on the launch stream around every kernel */
@@ -215,6 +229,7 @@ static void free_ws(Workspace &w)
   (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
   (void)hipFree(w.cargs); (void)hipFree(w.cL); (void)hipFree(w.cdots);
+  w.rb.release();
   (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
   w = Workspace();
 }
@@ -435,10 +450,27 @@ extern "C" int initialise(struct QED_kernel_temps *t)
   return kqed_initialise_dir(t, dir && *dir ? dir : KQED_DEFAULT_TABLE_DIR);
 }
 
+/* pass records and kinds of min(n, chunk) pairs, sort keys/ranks/permutation of n pairs and one
+ * counter per sort bucket (special pairs, nstpx * nstpy cells, nstpy Taylor-region y nodes) */
+static size_t batch_workspace(size_t n, size_t chunk, size_t nbuckets)
+{
+  const size_t c = n < chunk ? n : chunk;
+  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * nbuckets;
+}
+
 extern "C" size_t kqed_batch_workspace(size_t n)
 {
-  const size_t c = n < ((size_t)1 << 21) ? n : ((size_t)1 << 21);
-  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (1 + 100 * 4096 + 4096);
+  /* the default pass of 2^21 pairs and the largest table the reference's formats allow here
+   * (100 x nodes, 4096 y nodes) */
+  return batch_workspace(n, (size_t)1 << 21, 1 + 100 * 4096 + 4096);
+}
+
+extern "C" size_t kqed_batch_workspace_tables(const struct QED_kernel_temps *t, size_t n)
+{
+  if (!t || !t->dev) return 0;
+  kqed_device *d = t->dev;
+  std::lock_guard<std::mutex> lk(d->mu);
+  return batch_workspace(n, d->tune.chunk, 1 + (size_t)d->dt.nstpx * d->dt.nstpy + d->dt.nstpy);
 }
 
 /* records for one pass of c pairs; sort keys and permutation for the whole batch of n pairs */
@@ -623,18 +655,6 @@ extern "C" int kqed_set_tune(const struct QED_kernel_temps *t, const kqed_tune *
 }
 
 /* ---------------------------------------------------------------- ragged groups */
-/* Device copies of the group offsets and the per-pair y vectors of a ragged batch. */
-struct RaggedBufs {
-  size_t n = 0, ng = 0;
-  unsigned long long *off = nullptr;
-  double *ypair = nullptr;
-  void release()
-  {
-    (void)hipFree(off); (void)hipFree(ypair);
-    off = nullptr; ypair = nullptr; n = ng = 0;
-  }
-};
-
 static int ragged_prepare(RaggedBufs &rb, size_t ng, const size_t *offsets, hipStream_t s)
 {
   const size_t n = offsets[ng];
@@ -643,6 +663,8 @@ static int ragged_prepare(RaggedBufs &rb, size_t ng, const size_t *offsets, hipS
   if (offsets[0] != 0) { set_err("ragged offsets must start at 0"); return -1; }
   if (n > 0xFFFFFFFFull || ng > 0x7FFFFFFF) { set_err("ragged batch too large"); return -1; }
   if (rb.ng < ng + 1 || rb.n < n) {
+    /* the stream's previous ragged batch may still read the buffers being replaced */
+    if (rb.off && hipStreamSynchronize(s) != hipSuccess) { set_err("stream synchronisation failed"); return -3; }
     rb.release();
     if (hipMalloc(&rb.off, 8 * (ng + 1)) != hipSuccess || (n && hipMalloc(&rb.ypair, 32 * n) != hipSuccess)) {
       set_err("ragged workspace allocation failed");
@@ -669,8 +691,6 @@ static int ragged_launch(kqed_device *d, Workspace &w, RaggedBufs &rb, hipStream
   return run_pairs(d, w, s, which, M, n, x_dev, 4, rb.ypair, 4, out_dev, layout, nullptr, nullptr);
 }
 
-static thread_local RaggedBufs g_ragged;
-
 extern "C" int kqed_batch_ragged(const struct QED_kernel_temps *t, int which, double M, size_t ngroups,
                                  const size_t *offsets, const double *y_dev, const double *x_dev, void *out_dev,
                                  int layout, void *stream)
@@ -680,10 +700,13 @@ extern "C" int kqed_batch_ragged(const struct QED_kernel_temps *t, int which, do
   if (ngroups == 0 || offsets[ngroups] == 0) return 0;
   kqed_device *d = t->dev;
   hipStream_t s = (hipStream_t)stream;
-  int rc = ragged_prepare(g_ragged, ngroups, offsets, s);
-  if (rc) return rc;
+  /* the offsets and expanded y vectors live in the stream's workspace, like the pass buffers, so
+   * batches enqueued on different streams never share them */
   std::lock_guard<std::mutex> lk(d->mu);
-  return ragged_launch(d, d->ws[stream], g_ragged, s, which, M, ngroups, offsets[ngroups], y_dev, x_dev, out_dev, layout);
+  Workspace &w = d->ws[stream];
+  const int rc = ragged_prepare(w.rb, ngroups, offsets, s);
+  if (rc) return rc;
+  return ragged_launch(d, w, w.rb, s, which, M, ngroups, offsets[ngroups], y_dev, x_dev, out_dev, layout);
 }
 
 /* A ragged batch captured once into a HIP graph with its own workspace; every launch replays the
@@ -764,8 +787,9 @@ extern "C" int kqed_graph_launch(kqed_graph *g, void *stream)
 
 /* ---------------------------------------------------------------- autotuner */
 /* Times every candidate launch configuration on the caller's batch (device buffers, which/layout as
- * given) and keeps the fastest: passes of 2^17..2^21 pairs, assembly grids of 4096..65536
- * workgroups, sort by cell or by x node.  Each candidate runs once untimed, then `reps` times. */
+ * given) and keeps the fastest: passes of 2^18..2^22 pairs (no longer than the first pass that
+ * covers the batch), assembly grids of 512, 1024 or 4096 workgroups, sort by cell or by x node.
+ * Each candidate runs once untimed, then `reps` times. */
 extern "C" int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
                              const double *y_dev, void *out_dev, int layout, int reps, void *stream, kqed_tune *best,
                              double *best_ms)

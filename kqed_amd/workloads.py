@@ -1,5 +1,6 @@
 """Synthetic workloads of the benchmark and tests (inputs only; no evaluation here).
 
+* pairs_uniform: 4096 (x, y) pairs, |x|, |y| uniform in [0.01, 3.0], isotropic (seed 1).
 * pairs_log_uniform: 2^24 (x, y) pairs, |x|, |y| log-uniform in [1e-3, 4], isotropic (seed 2).
 * lattice: the sites of an L^4 periodic lattice, n_mu in [-L/2, L/2 - 1], x = a n (a m_mu = 0.0536),
   per-site FP32 tensors T(x)_c = N(0,1) exp(-|x|/2) (seed 3), and 16 fixed y sites (Y_SITES).
@@ -63,3 +64,12 @@ def ragged(seed=4, ny=256, median=16384, sigma=1.0, lo=1024, hi=262144):
     n = int(sizes.sum())
     x = iso(rng, n) * np.exp(rng.uniform(np.log(1e-3), np.log(4.0), (n, 1)))
     return y, sizes, x
+
+
+def pairs_uniform(seed, n, lo=0.01, hi=3.0):
+    """(x, y) pairs with |x|, |y| uniform in [lo, hi], isotropic: the CPU L0 benchmark law (4096
+    pairs, [0.01, 3.0], seed 1)."""
+    rng = np.random.default_rng(seed)
+    rx = rng.uniform(lo, hi, (n, 1))
+    ry = rng.uniform(lo, hi, (n, 1))
+    return iso(rng, n) * rx, iso(rng, n) * ry
