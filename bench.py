@@ -30,25 +30,50 @@ def workload(n, seed=2):
     return log_uniform(seed, n)
 
 
-def cpu_baseline(table, x, y, budget_s=12.0):
-    """The CPU oracle (restatement of the reference's AVX+FMA FP64 path), OpenMP on host cores."""
+def cpu_baseline(table, x2, y2, reps=9):
+    """The CPU oracle (restatement of the reference's AVX+FMA FP64 path, built -O3 -mavx2 -mfma
+    -ffp-contract=off) on the CPU L0 benchmark law: 4096 pairs, |x|, |y| uniform in [0.01, 3.0],
+    isotropic, seed 1 (the reference's timing loop is amu_forlattice.c:1402-1433).  Median of
+    `reps` runs after one warm-up, on one thread and on the host cores this process may use (at most
+    16, the box's CPU share per GPU).  Also counts the algorithmic flops of one L0 on 2000 pairs of
+    the GPU workload (x2, y2) for the roofline."""
     from oracle_ctypes import Oracle
+    from kqed_amd import workloads as W
     o = Oracle(table)
-    cores = min(16, os.cpu_count() or 1)
-    done, t0, chunk = 0, time.time(), 1024 * cores
-    sample_out = None
-    while time.time() - t0 < budget_s and done + chunk <= len(x):
-        r = o.batch(0, x[done:done + chunk], y[done:done + chunk], nthreads=cores)
-        if sample_out is None:
-            sample_out = r
-        done += chunk
-    dt = time.time() - t0
+    x, y = W.pairs_uniform(1, 4096)
+    cores = min(16, len(os.sched_getaffinity(0)))
+
+    def rate(nt):
+        o.batch(0, x, y, nthreads=nt)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            o.batch(0, x, y, nthreads=nt)
+            ts.append(time.perf_counter() - t0)
+        return len(x) / float(np.median(ts)), float(np.sum(ts))
+
+    single, t_single = rate(1)
+    multi, t_multi = rate(cores)
     o.flops_reset()
-    o.batch(0, x[:2000], y[:2000], nthreads=1)
+    o.batch(0, x2[:2000], y2[:2000], nthreads=1)
     flops = {"total": o.flops() / 2000, "node_only": o.flops_build() / 2000}
     o.close()
-    return {"value": done / dt, "unit": "evals/s", "cores": cores, "kind": "port",
-            "sample": f"first {done} pairs of the workload, L0, {dt:.1f} s"}, flops, sample_out
+    return {"value": multi, "unit": "evals/s", "cores": cores, "kind": "port",
+            "single_thread": single, "host_nproc": os.cpu_count(),
+            "sample": f"4096 pairs, |x|,|y| U[0.01,3.0], seed 1 (the CPU L0 law), L0, synthetic table "
+                      f"'{os.path.basename(table)}'; median of {reps} runs after a warm-up; "
+                      f"{t_single + t_multi * cores:.1f} CPU-seconds"}, flops
+
+
+def parity_sample(table, which, M, x, y, got, nthreads):
+    """max and median per-pair normwise error of GPU rows `got` against the oracle on the same pairs."""
+    from oracle_ctypes import Oracle
+    from pairs import normwise
+    o = Oracle(table)
+    ref = o.batch(which, x, y, M=M, nthreads=nthreads)
+    o.close()
+    err = normwise(got, ref)
+    return {"pairs": len(x), "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err))}
 
 
 def pass_traffic(pairs_per_pass):
@@ -281,8 +306,6 @@ def main():
 
     import torch
     import kqed_amd
-    from pairs import normwise
-
     table = kqed_amd.table_dir(a.table)
     t = kqed_amd.QEDKernelTemps.initialise(table)
     x, y = workload(a.n)
@@ -317,26 +340,31 @@ def main():
     stages = t.profile(False)
     evals = a.n * a.steps / dt
 
-    # parity of the measured output against the oracle on a sample of the workload
-    extra = {}
-    cpu = None
-    if not a.no_cpu:
-        cpu, flops_per_eval, ref = cpu_baseline(table, x, y)
-        m = len(ref)
-        got = out[:m].cpu().numpy()
-        err = normwise(got, ref)
-        extra["parity_sample"] = {"pairs": m, "max_rel_err": float(err.max()), "median_rel_err": float(np.median(err))}
-    else:
-        flops_per_eval = None
+    # uniform samples of the measured outputs (copied out before the buffer is reused), checked
+    # against the oracle after the GPU legs: 2^18 pairs of the timed kernel, 2^16 of the others
+    rng = np.random.default_rng(17)
+    samples = {}
 
+    def take_sample(which, M, m):
+        idx = np.sort(rng.choice(a.n, min(m, a.n), replace=False))
+        rows = out[torch.from_numpy(idx).to(dev)].cpu().numpy()
+        samples[KNAMES[which]] = (which, M, idx, rows)
+
+    extra = {}
+    take_sample(a.kernel, 0.5 if a.kernel == 4 else 0.0, 1 << 18)
     if not a.no_extra:
         rates = {}
         for name, which, M in (("L1", 1, 0.0), ("L2", 2, 0.0), ("L3", 3, 0.0), ("ML2_M0.5", 4, 0.5)):
+            if which == a.kernel:
+                continue
+            step(which, M)  # warm-up
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            step(which, M)
+            for _ in range(3):
+                step(which, M)
             torch.cuda.synchronize()
-            rates[name] = a.n / (time.perf_counter() - t1)
+            rates[name] = 3 * a.n / (time.perf_counter() - t1)
+            take_sample(which, M, 1 << 16)
         extra["evals_per_s_other_kernels"] = rates
 
     if not a.no_extra:
@@ -348,6 +376,15 @@ def main():
         if a.surrogate:
             del out
             extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
+
+    cpu = None
+    flops_per_eval = None
+    if not a.no_cpu:
+        cpu, flops_per_eval = cpu_baseline(table, x, y)
+    nthr = min(16, len(os.sched_getaffinity(0)))
+    extra["parity_sample"] = {
+        name: dict(parity_sample(table, which, M, x[idx], y[idx], rows, nthr), law="config-2 workload, uniform sample")
+        for name, (which, M, idx, rows) in samples.items()}
 
     ms_i, cnt_i = stages["interp"]
     ms_a, cnt_a = stages["assemble"]
@@ -363,9 +400,14 @@ def main():
         pairs_per_pass = a.n * a.steps / max(cnt_i, 1)
         avg = (ms_i + ms_a) / max(cnt_i, 1) / 1e3
         achieved = per_eval * pairs_per_pass / avg
-        roof = {"bound": "mfma", "resource": "FP64 vector ALU (peak equal to the FP64 matrix peak)",
+        roof = {"bound": "valu", "resource": "FP64 vector ALU: no matrix instructions on this path; the MI355X "
+                "FP64 vector peak (78.6 TFLOP/s) equals its FP64 matrix peak",
                 "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64, "traffic": pass_traffic(pairs_per_pass),
+                "traffic_source": "profiles/traffic_l0_pass.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of the "
+                                  "same pass; not measured in this run)",
+                "flop_method": "the oracle's count of the reference algorithm's FP64 operations (add, mul, div, "
+                               "sqrt 1 each, FMA 2), tallied per operation group on 2000 pairs of this workload",
                 "kernel": "one pipeline pass: k_interp<0,1,2> (interpolation and chain-rule algebra) + k_assemble2 / k_assemble",
                 "pairs_per_pass": pairs_per_pass, "pass_ms": avg * 1e3,
                 "flops_per_eval": per_eval, "flops_per_eval_with_node_arrays": flops_per_eval["total"]}
@@ -375,7 +417,9 @@ def main():
         "ms_per_step": dt / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"QED_kernel_{KNAMES[a.kernel]}, 2^24 pairs, |x|,|y| log-uniform [1e-3,4], isotropic, FP64",
-                   "pairs": a.n, "table": a.table, "kernel": KNAMES[a.kernel], "path": "plain FP64"},
+                   "pairs": a.n, "table": a.table, "kernel": KNAMES[a.kernel], "path": "plain FP64",
+                   "taylor_fraction": float(np.mean(np.linalg.norm(x, axis=1) < t.XX[0])),
+                   "out_of_grid_fraction": float(np.mean(np.linalg.norm(x, axis=1) > t.XX[-1]))},
         "stages_ms_per_step": {"sort": ms_s / a.steps, "interp": ms_i / a.steps, "assemble": ms_a / a.steps},
         "dominant_kernel": dom,
         "roofline": roof,

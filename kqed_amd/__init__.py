@@ -20,7 +20,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libkqed_hip.so")
-GOLDEN_PRECOMP = os.path.join(ROOT, "tests", "golden", "PRECOMP")
+# the reference's shipped Taylor-x and Taylor-y tables (reference PRECOMP/taylor{x,y}_cksum.bin),
+# package data: every table directory this package builds pairs them with a form-factor table
+TAYLOR_TABLES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tables")
 
 KQED_L0, KQED_L1, KQED_L2, KQED_L3, KQED_ML2 = range(5)
 KQED_AOS, KQED_SOA, KQED_AOS_F32, KQED_SOA_F32 = 0, 1, 2, 3
@@ -174,7 +176,7 @@ def table_dir(name="small", root=None):
     for f in ("taylorx_cksum.bin", "taylory_cksum.bin"):
         dst = os.path.join(d, f)
         if not os.path.exists(dst):
-            shutil.copyfile(os.path.join(GOLDEN_PRECOMP, f), dst)
+            shutil.copyfile(os.path.join(TAYLOR_TABLES, f), dst)
     return d
 
 

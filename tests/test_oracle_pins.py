@@ -188,6 +188,70 @@ def _sym_projector(pairs):
     return proj
 
 
+def _linear_form(expr, env):
+    """Value of one right-hand side of the reference's unrolled assembly under `env`, by a strict
+    parser of the only grammar those lines use (no eval of the reference's text):
+        expr := term (('+' | '-') term)*      term := unary ('*' unary)*
+        unary := ('+' | '-') unary | atom     atom := NUMBER | NAME | NAME '[' INT ']' | '(' expr ')'
+    NAME must be a key of env; anything else raises ValueError."""
+    import re
+    toks = re.findall(r"[A-Za-z_]\w*|\d+(?:\.\d*)?|[-+*()\[\]]|\S", expr)
+    pos = [0]
+
+    def peek():
+        return toks[pos[0]] if pos[0] < len(toks) else None
+
+    def take(want=None):
+        t = peek()
+        if t is None or (want is not None and t != want):
+            raise ValueError(f"unexpected {t!r} in {expr!r}")
+        pos[0] += 1
+        return t
+
+    def atom():
+        t = take()
+        if t == "(":
+            v = parse_expr()
+            take(")")
+            return v
+        if re.fullmatch(r"\d+(?:\.\d*)?", t):
+            return float(t)
+        if t not in env:
+            raise ValueError(f"unknown name {t!r}")
+        if peek() == "[":
+            take("[")
+            k = take()
+            if not k.isdigit():
+                raise ValueError(f"bad index {k!r}")
+            take("]")
+            return env[t][int(k)]
+        return env[t]
+
+    def unary():
+        if peek() in ("+", "-"):
+            sign = -1.0 if take() == "-" else 1.0
+            return sign * unary()
+        return atom()
+
+    def term():
+        v = unary()
+        while peek() == "*":
+            take("*")
+            v = v * unary()
+        return v
+
+    def parse_expr():
+        v = term()
+        while peek() in ("+", "-"):
+            v = v + term() if take() == "+" else v - term()
+        return v
+
+    v = parse_expr()
+    if peek() is not None:
+        raise ValueError(f"trailing {peek()!r} in {expr!r}")
+    return v
+
+
 def _parse_unrolled(path, fn):
     import re
     src = open(path).read()
@@ -203,7 +267,7 @@ def _parse_unrolled(path, fn):
         env[f"sxyv{k}"] = (env["sxv"][k] + env["syv"][k]) / 4.0
     M = np.zeros((384, N))
     for idx, expr in re.findall(r"kp\[(\d+)\]\s*=\s*(.*?);", body, re.S):
-        M[int(idx)] = eval(expr.replace("\n", " "), {}, env)
+        M[int(idx)] = _linear_form(expr.replace("\n", " "), env)
     return M
 
 
