@@ -211,6 +211,20 @@ int kqed_autotune(const struct QED_kernel_temps *t, int which, double M, size_t 
 int kqed_contract(const struct QED_kernel_temps *t, int which, double M, size_t nx, const double *x_dev,
                   const float *T_dev, size_t ny, const double *y, double *s, void *stream);
 
+/* kqed_batch that allocates no device memory: no counterpart in the reference; for callers that
+ * cannot allocate (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Its workspace is a fixed scratch array in device memory, one per GPU, held until the process
+ * ends (about 43 MB): at most KQED_NOALLOC_MAX_PAIRS pairs per call (larger batches are rejected
+ * with an error), evaluated in passes of KQED_NOALLOC_PASS pairs, tables with at most
+ * KQED_NOALLOC_MAX_BUCKETS sort buckets (1 + nstpx * nstpy + nstpy).  Outputs equal kqed_batch's.
+ * Every call on a GPU shares that GPU's array: calls that overlap in time on one GPU (any stream,
+ * any table set) race.  Enqueued on `stream`; 0 or a negative error code. */
+#define KQED_NOALLOC_MAX_PAIRS 262144
+#define KQED_NOALLOC_PASS 32768
+#define KQED_NOALLOC_MAX_BUCKETS (1 + 100 * 4096 + 4096)
+int kqed_batch_noalloc(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
+                       const double *y_dev, void *out_dev, int layout, void *stream);
+
 /* Bytes of device scratch kqed_batch needs for n pairs (allocated and cached by the library, per
  * stream): kqed_batch_workspace assumes the default pass of 2^21 pairs and a table of at most
  * 100 x 4096 nodes; kqed_batch_workspace_tables uses the table set's own grid and current tune. */

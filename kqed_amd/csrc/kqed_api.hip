@@ -170,6 +170,9 @@ struct kqed_device {
   signed char *pert_tab = nullptr;
   /* every surrogate bin set: routes all packed-path pairs through the substitution step */
   unsigned char *allmask = nullptr;
+  /* device address of this GPU's copy of the allocation-free entry's scratch array (null when
+   * hipGetSymbolAddress failed at set-up) */
+  unsigned char *noalloc = nullptr;
   std::mutex mu;
   std::unordered_map<void *, Workspace> ws;
   bool profiling = false;
@@ -318,6 +321,22 @@ static int build_wave_maps(kqed_device *d)
   return 0;
 }
 
+/* ---------------------------------------------------------------- allocation-free scratch */
+/* Scratch of kqed_batch_noalloc, one copy per GPU for the life of the process: the records and
+ * kinds of one pass of KQED_NOALLOC_PASS pairs, the sort keys, ranks and permutation of
+ * KQED_NOALLOC_MAX_PAIRS pairs and KQED_NOALLOC_MAX_BUCKETS sort-bucket counters (every part
+ * 256-byte aligned).  Host code never uses the name `scratch` as an address: initialise() takes
+ * this GPU's device address with hipGetSymbolAddress and keeps it in kqed_device::noalloc. */
+static constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+static constexpr size_t NA_REC = 0;
+static constexpr size_t NA_KINDS = NA_REC + al256((size_t)KQED_NOALLOC_PASS * KQED_REC_LEN * 8);
+static constexpr size_t NA_KEY = NA_KINDS + al256((size_t)KQED_NOALLOC_PASS * 4);
+static constexpr size_t NA_RANK = NA_KEY + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
+static constexpr size_t NA_PERM = NA_RANK + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
+static constexpr size_t NA_COUNT = NA_PERM + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
+static constexpr size_t NA_BYTES = NA_COUNT + al256((size_t)KQED_NOALLOC_MAX_BUCKETS * 4);
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
 extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
 {
   memset(t, 0, sizeof *t);
@@ -338,6 +357,11 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
     goto fail_host;
   }
   HIPCHK(hipGetDevice(&d->device));
+  /* this device's copy of the allocation-free entry's scratch (the device is current here) */
+  if (hipGetSymbolAddress((void **)&d->noalloc, HIP_SYMBOL(scratch)) != hipSuccess) {
+    d->noalloc = nullptr;
+    (void)hipGetLastError();
+  }
   {
     kqed_dev_tables &T = d->dt;
     T.nstpx = h->nstpx;
@@ -631,6 +655,48 @@ extern "C" int kqed_batch(const struct QED_kernel_temps *t, int which, double M,
   std::lock_guard<std::mutex> lk(d->mu);
   return run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
                    nullptr);
+}
+
+/* kqed_batch without any device allocation: the pass workspace is this GPU's scratch array
+ * (kqed_device::noalloc), passes of at most KQED_NOALLOC_PASS pairs.  Outputs equal kqed_batch's
+ * (results do not depend on the pass length).  Calls that overlap on one GPU share the array and
+ * race; the pipeline clears its counters itself (fill kernel) on every call. */
+extern "C" int kqed_batch_noalloc(const struct QED_kernel_temps *t, int which, double M, size_t n, const double *x_dev,
+                                  const double *y_dev, void *out_dev, int layout, void *stream)
+{
+  if (!t || !t->dev) { set_err("kqed_batch_noalloc: tables not initialised"); return -1; }
+  if (which < 0 || which > 4 || !layout_ok(layout)) { set_err("kqed_batch_noalloc: bad argument"); return -1; }
+  if (n == 0) return 0;
+  if (n > (size_t)KQED_NOALLOC_MAX_PAIRS) {
+    set_err("kqed_batch_noalloc: %zu pairs exceed the maximum of %d per call", n, KQED_NOALLOC_MAX_PAIRS);
+    return -1;
+  }
+  kqed_device *d = t->dev;
+  if (!d->noalloc) { set_err("kqed_batch_noalloc: scratch address unavailable on this device"); return -3; }
+  const size_t nb = 1 + (size_t)d->dt.nstpx * d->dt.nstpy + d->dt.nstpy;
+  if (nb > (size_t)KQED_NOALLOC_MAX_BUCKETS) {
+    set_err("kqed_batch_noalloc: the table's %zu sort buckets exceed the maximum of %d", nb, KQED_NOALLOC_MAX_BUCKETS);
+    return -1;
+  }
+  Workspace w; /* a view of the scratch array, sized so that the pipeline allocates nothing */
+  w.cap = KQED_NOALLOC_PASS;
+  w.ncap = KQED_NOALLOC_MAX_PAIRS;
+  w.nbuckets = nb;
+  w.rec = (double *)(d->noalloc + NA_REC);
+  w.kinds = (unsigned *)(d->noalloc + NA_KINDS);
+  w.key = (unsigned *)(d->noalloc + NA_KEY);
+  w.rank = (unsigned *)(d->noalloc + NA_RANK);
+  w.perm = (unsigned *)(d->noalloc + NA_PERM);
+  w.count = (unsigned *)(d->noalloc + NA_COUNT);
+  std::lock_guard<std::mutex> lk(d->mu);
+  const kqed_tune keep = d->tune;
+  const bool prof = d->profiling;
+  if (d->tune.chunk > (size_t)KQED_NOALLOC_PASS) d->tune.chunk = KQED_NOALLOC_PASS;
+  d->profiling = false; /* no timing events */
+  const int rc = run_pairs(d, w, (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr, nullptr);
+  d->tune = keep;
+  d->profiling = prof;
+  return rc;
 }
 
 extern "C" int kqed_get_tune(const struct QED_kernel_temps *t, kqed_tune *tune)
