@@ -17,6 +17,7 @@
 
 #include "../../include/kqed.h"
 #include "kqed_assembly.h"
+#include "kqed_bank_layout.h"
 #include "kqed_combine.h"
 #include "kqed_device.h"
 #include "kqed_pert.h"
@@ -222,6 +223,7 @@ extern "C" int kqed_profile(const struct QED_kernel_temps *t, int enable, double
 static void *dev_alloc(kqed_device *d, size_t bytes)
 {
   void *p = nullptr;
+  if (d->nalloc >= (int)(sizeof d->allocs / sizeof d->allocs[0])) return nullptr;
   if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
   d->allocs[d->nalloc++] = p;
   return p;
@@ -264,6 +266,21 @@ static int build_wave_maps(kqed_device *d)
   kqed_dev_tables &T = d->dt;
   std::vector<unsigned> E;
   std::vector<unsigned short> R(3 * 384);
+  /* the standard map's LDS layout (kqed_bank_layout.h): the slot of each tensor entry and the lane
+   * of each row within its row slot, chosen to spread the gathers over the LDS banks */
+  static const unsigned char std_slot[200] = KQED_STD_SLOT;
+  static const unsigned short std_rows[384] = KQED_STD_ROWS;
+  {
+    bool seen_s[200] = {}, seen_r[384] = {};
+    for (int c = 0; c < 200; c++) {
+      if (std_slot[c] >= 200 || seen_s[std_slot[c]]) { set_err("bank layout: bad slot table"); return 1; }
+      seen_s[std_slot[c]] = true;
+    }
+    for (int i = 0; i < 384; i++) {
+      if (std_rows[i] >= 384 || seen_r[std_rows[i]]) { set_err("bank layout: bad row table"); return 1; }
+      seen_r[std_rows[i]] = true;
+    }
+  }
   for (int q = 0; q < 3; q++) {
     const kqed_map &m = d->maps[q];
     int order[384];
@@ -271,6 +288,16 @@ static int build_wave_maps(kqed_device *d)
     std::stable_sort(order, order + 384, [&](int a, int b) {
       return m.rowptr[a + 1] - m.rowptr[a] > m.rowptr[b + 1] - m.rowptr[b];
     });
+    if (q == 0) {
+      /* same rows per row slot (a permutation within each slot), in the layout's lane order */
+      for (int sl = 0; sl < 6; sl++) {
+        std::vector<int> a(order + 64 * sl, order + 64 * sl + 64), b(std_rows + 64 * sl, std_rows + 64 * sl + 64);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        if (a != b) { set_err("bank layout does not match the standard map"); return 1; }
+      }
+      for (int i = 0; i < 384; i++) order[i] = std_rows[i];
+    }
     static const int KB[6] = KQED_WM_K;
     for (int sl = 0; sl < 6; sl++) {
       T.wmOff[q][sl] = (int)E.size();
@@ -298,9 +325,10 @@ static int build_wave_maps(kqed_device *d)
             const double a = fabs(v);
             if ((a != 0.5 && a != 1.0 && a != 2.0) || col >= 200) { set_err("assembly coefficient %g not encodable", v); return 1; }
           }
+          const int slot = q == 0 && col < 200 ? std_slot[col] : col;
           unsigned long long bits;
           memcpy(&bits, &v, 8);
-          E.push_back((unsigned)(bits >> 48) << 16 | (unsigned)(col * 8));
+          E.push_back((unsigned)(bits >> 48) << 16 | (unsigned)(slot * 8));
         }
     }
     T.wmOff[q][6] = (int)E.size();
@@ -311,13 +339,16 @@ static int build_wave_maps(kqed_device *d)
   if (T.wmTotal > KQED_WM_MAX) { set_err("assembly maps too large for LDS"); return 1; }
   unsigned *e = (unsigned *)dev_alloc(d, 4 * E.size());
   unsigned short *r = (unsigned short *)dev_alloc(d, 2 * R.size());
-  if (!e || !r || hipMemcpy(e, E.data(), 4 * E.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(r, R.data(), 2 * R.size(), hipMemcpyHostToDevice) != hipSuccess) {
+  unsigned char *sl = (unsigned char *)dev_alloc(d, 256);
+  if (!e || !r || !sl || hipMemcpy(e, E.data(), 4 * E.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(r, R.data(), 2 * R.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(sl, std_slot, 200, hipMemcpyHostToDevice) != hipSuccess) {
     set_err("wave map upload failed");
     return 1;
   }
   T.wmE = e;
   T.wmRow = r;
+  T.wmSlot = sl;
   return 0;
 }
 

@@ -87,6 +87,9 @@ struct kqed_dev_tables {
    * coefficient +-0.5, +-1 or +-2; padded entries read the zero slot (col 200) */
   const unsigned *wmE;         /* all three maps, concatenated */
   const unsigned short *wmRow; /* [3][6][64] output row of (map, slot, lane) */
+  /* LDS slot of each of the standard map's 200 tensor entries (its column c is read from slot
+   * wmSlot[c]; the zero column stays at 200); the Tabd maps use their columns as slots */
+  const unsigned char *wmSlot;
   int wmOff[3][7];             /* start of slot m of map q in wmE; wmOff[q][6] = end */
   int wmK[3][6];               /* entries per lane in slot m of map q */
   int wmTotal;

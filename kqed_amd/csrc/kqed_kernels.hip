@@ -1872,6 +1872,17 @@ DEV void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+/* LDS slots of a lane's standard tensor entries c = lane + 64 k (k < 4, c < 200), one byte each */
+DEV unsigned std_slots(const DT &T, int lane)
+{
+  unsigned w = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (lane + 64 * k < 200) w |= (unsigned)T.wmSlot[lane + 64 * k] << (8 * k);
+  return w;
+}
+DEV int std_slot(unsigned w, int k) { return (w >> (8 * k)) & 0xFF; }
+
 /* One row of a lane: KB entries (byte offset col*8 of the column | upper 16 bits of the FP64
  * coefficient << 16), unrolled so that every LDS read of the row is in flight at once.  The FP64
  * coefficient is the entry with its low half cleared, as the high word of a double. */
@@ -1999,6 +2010,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
     return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  const unsigned slw = std_slots(T, lane); /* LDS slots of this lane's standard entries */
   size_t p = (size_t)blockIdx.x * KQED_ASM_WAVES + w;
   /* one evaluation slot reads the header and main payload only (80 of the record's doubles) */
   const int used = nslots == 1 ? KQED_REC_AUX : WREC;
@@ -2091,7 +2103,8 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       }
       wave_sync();
       if (kd == KIND_STD) {
-        for (int c = lane; c < 200; c += 64) Tv[c] = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+        for (int k = 0, c = lane; c < 200; k++, c += 64)
+          Tv[std_slot(slw, k)] = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
       } else if (kd == KIND_XEQ0 || kd == KIND_XEQ0_M) {
         for (int c = lane; c < 192; c += 64) Tv[c] = xeq0_entry(pl, H, c);
       } else {
@@ -2102,7 +2115,7 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
         /* contraction shortcut: only slot 0 exists (nslots == 1) */
         const double *wq = Wc + q * 200;
         double part = 0.0;
-        for (int c = lane; c < 200; c += 64) part = fma((double)Tv[c], wq[c], part);
+        for (int k = 0, c = lane; c < 200; k++, c += 64) part = fma((double)Tv[std_slot(slw, k)], wq[c], part);
         dot = wave_sum(part);
         have_dot = true;
         continue;
@@ -2295,6 +2308,7 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  const unsigned slw = std_slots(T, lane); /* LDS slots of this lane's standard entries */
   /* the first row slot's 13 map entries stay in registers (a longer set spills) */
   unsigned er[13];
 #pragma unroll
@@ -2359,9 +2373,9 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     H[lane] = std_helper(R, lane, hd0);
     if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
     wave_sync();
-    for (int c = lane; c < 200; c += 64) {
+    for (int k = 0, c = lane; c < 200; k++, c += 64) {
       const D2 t = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
-      Tv[c] = TV{ (MT)t.x, (MT)t.y };
+      Tv[std_slot(slw, k)] = TV{ (MT)t.x, (MT)t.y };
     }
     wave_sync();
     if constexpr (WDOT) {
@@ -2369,8 +2383,8 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
        * site's pulled-back weights, as the one-pair kernel forms it; an empty slot gives 0 */
       const double *wa = Wc + qa * 200, *wb = Wc + qb * 200;
       double sa = 0.0, sb = 0.0;
-      for (int c = lane; c < 200; c += 64) {
-        const TV t = Tv[c];
+      for (int k = 0, c = lane; c < 200; k++, c += 64) {
+        const TV t = Tv[std_slot(slw, k)];
         sa = fma((double)t.x, wa[c], sa);
         sb = fma((double)t.y, wb[c], sb);
       }
@@ -2456,6 +2470,7 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
     return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  const unsigned slw = std_slots(T, lane); /* LDS slots of this lane's standard entries */
   unsigned er[13]; /* map 0's first row slot, kept in registers */
 #pragma unroll
   for (int k = 0; k < 13; k++) er[k] = sE[64 * k + lane];
@@ -2541,9 +2556,9 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
          * LDS store lies between their reads of the record, so the per-pair values are read and
          * formed once) */
         if (kd == KIND_STD) {
-          for (int c = lane; c < 200; c += 64) {
+          for (int k = 0, c = lane; c < 200; k++, c += 64) {
             const D2 t = std_entry(pl, R + KQED_REC_X, R + KQED_REC_Y, H, c);
-            Tv[c] = TV{ (MT)t.x, (MT)t.y };
+            Tv[std_slot(slw, k)] = TV{ (MT)t.x, (MT)t.y };
           }
         } else if (kd == KIND_XEQ0 || kd == KIND_XEQ0_M) {
           const D2 t0 = xeq0_entry(pl, H, lane), t1 = xeq0_entry(pl, H, lane + 64), t2 = xeq0_entry(pl, H, lane + 128);
