@@ -2348,6 +2348,23 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
       }
       return;
     }
+    if constexpr (sizeof(MT) == 8) {
+      if (!(layout & 1)) {
+        /* AoS FP64: two consecutive outputs of a pair per lane, one 16-byte store each */
+        typedef double dbl2 __attribute__((ext_vector_type(2)));
+        double *out = (double *)outv;
+        for (int i = 2 * lane; i < 384; i += 128) {
+          const D2 o0 = O[i], o1 = O[i + 1];
+          const dbl2 va = { o0.x, o1.x };
+          __builtin_nontemporal_store(va, (dbl2 *)(out + pa * 384 + i));
+          if (pbv) {
+            const dbl2 vb = { o0.y, o1.y };
+            __builtin_nontemporal_store(vb, (dbl2 *)(out + pb * 384 + i));
+          }
+        }
+        return;
+      }
+    }
     for (int i = lane; i < 384; i += 64) {
       const D2 o = O[i];
       MT *out = (MT *)outv;
