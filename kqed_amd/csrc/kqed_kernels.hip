@@ -2451,6 +2451,20 @@ template __global__ void k_assemble2<KQED_ASM2_WAVES, float, false>(DT, size_t, 
 template __global__ void k_assemble2<KQED_ASM2_WAVES, double, true>(DT, size_t, const unsigned *, size_t, const double *,
                                                                     const unsigned *, void *, int, size_t, const double *);
 
+/* The Tabd slots 1..nslots-1 of two records (kinds packed 3 bits per slot) can be applied map by
+ * map on combined inputs: every slot is empty or a plain Tabd kind (no M-L2 term), and where both
+ * records have a slot set they agree on its kind. */
+DEV bool tabd_mergeable(unsigned ka, unsigned kb, int nslots)
+{
+  for (int s = 1; s < nslots; s++) {
+    const unsigned a = (ka >> (3 * s)) & 7u, b = (kb >> (3 * s)) & 7u;
+    const bool pa = a == KIND_NONE || a == KIND_XEQ0 || a == KIND_YEQ0 || a == KIND_YEQ0_MULAM;
+    const bool pb = b == KIND_NONE || b == KIND_XEQ0 || b == KIND_YEQ0 || b == KIND_YEQ0_MULAM;
+    if (!pa || !pb || (a != KIND_NONE && b != KIND_NONE && a != b)) return false;
+  }
+  return true;
+}
+
 /* Several evaluation slots (L1, L2, L3, M-L2) in sorted order past the special pairs, two records per
  * wavefront as in k_assemble2: a slot is evaluated once for both records when their kinds agree
  * (past the special pairs they do, unless a slot of one record is KIND_NONE, whose component is then
@@ -2549,6 +2563,74 @@ k_assemble2m(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
     wave_sync();
     const size_t qa = (size_t)R[KQED_REC_P].x, qb = hasb ? (size_t)R[KQED_REC_P].y : qa;
     for (int slot = 0; slot < nslots; slot++) {
+      if (slot == 1 && tabd_mergeable(kda, kdb, nslots)) {
+        /* L1, L2, L3: the Tabd maps are linear, so the slots that share a map are combined on their
+         * inputs (coefficient times tensor entry, FP64, in slot order; an empty slot adds nothing)
+         * and each map is applied once: L1 and L2 need one Tabd map application instead of two, L3
+         * two instead of three (reference kernels.c:119-177) */
+        for (int mq = 1; mq <= 2; mq++) {
+          int sl[3];
+          unsigned kk[3];
+          int ns = 0;
+          for (int s2 = 1; s2 < nslots; s2++) {
+            const unsigned a = (kda >> (3 * s2)) & 7u, b = (kdb >> (3 * s2)) & 7u;
+            const unsigned k = a != KIND_NONE ? a : b;
+            if (k == KIND_NONE || (k == KIND_YEQ0_MULAM ? 2 : 1) != mq) continue;
+            sl[ns] = s2;
+            kk[ns] = k;
+            ns++;
+          }
+          if (ns == 0) continue;
+          /* unit vectors of the slots' arguments, H[4 j + i] */
+          if (lane < 4 * ns) {
+            const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (sl[lane >> 2] - 1);
+            H[lane] = pl[lane & 3] / pl[4];
+          }
+          wave_sync();
+          D2 acc0 = D2{ 0.0, 0.0 }, acc1 = D2{ 0.0, 0.0 }, acc2 = D2{ 0.0, 0.0 };
+          for (int j = 0; j < ns; j++) {
+            const int s2 = sl[j];
+            const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (s2 - 1);
+            const bool va = ((kda >> (3 * s2)) & 7u) != KIND_NONE, vb = ((kdb >> (3 * s2)) & 7u) != KIND_NONE;
+            D2 t0, t1, t2;
+            if (kk[j] == KIND_XEQ0) {
+              t0 = xeq0_entry(pl, H + 4 * j, lane); t1 = xeq0_entry(pl, H + 4 * j, lane + 64);
+              t2 = xeq0_entry(pl, H + 4 * j, lane + 128);
+            } else {
+              t0 = yeq0_entry(pl, H + 4 * j, lane); t1 = yeq0_entry(pl, H + 4 * j, lane + 64);
+              t2 = yeq0_entry(pl, H + 4 * j, lane + 128);
+            }
+            const D2 cf = R[KQED_REC_COEF + s2];
+            /* an empty slot's payload is not read: its component is zero */
+            const D2 c = D2{ va ? cf.x : 0.0, vb ? cf.y : 0.0 };
+            if (!va) t0.x = t1.x = t2.x = 0.0;
+            if (!vb) t0.y = t1.y = t2.y = 0.0;
+            acc0 = fma(c, t0, acc0);
+            acc1 = fma(c, t1, acc1);
+            acc2 = fma(c, t2, acc2);
+          }
+          Tv[lane] = TV{ (MT)acc0.x, (MT)acc0.y };
+          Tv[lane + 64] = TV{ (MT)acc1.x, (MT)acc1.y };
+          Tv[lane + 128] = TV{ (MT)acc2.x, (MT)acc2.y };
+          if (lane == 0) Tv[200] = TV{ 0, 0 };
+          wave_sync();
+          const unsigned *e = sE + mq * 64 * KQED_WM_PER_LANE;
+          D2 v[6];
+          v[0] = wm_dot2<13>(e, lane, Tv);
+          v[1] = wm_dot2<9>(e + 64 * 13, lane, Tv);
+          v[2] = wm_dot2<6>(e + 64 * 22, lane, Tv);
+          v[3] = wm_dot2<6>(e + 64 * 28, lane, Tv);
+          v[4] = wm_dot2<4>(e + 64 * 34, lane, Tv);
+          v[5] = wm_dot2<3>(e + 64 * 38, lane, Tv);
+#pragma unroll
+          for (int m = 0; m < 6; m++) {
+            const int r = sRow[mq * 384 + 64 * m + lane];
+            O[r] = O[r] + v[m];
+          }
+          wave_sync();
+        }
+        break;
+      }
       const unsigned sa = (kda >> (3 * slot)) & 7u, sb = (kdb >> (3 * slot)) & 7u;
       for (int pass = 0; pass < 2; pass++) {
         /* pass 0: the kind of record a (of b when a's slot is empty); pass 1: b's differing kind */

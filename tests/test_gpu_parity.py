@@ -130,12 +130,14 @@ def test_two_pair_assembly_bit_identical(gpu_full):
     assert np.array_equal(got32_s, ref32.T)
 
 
-def test_two_pair_multislot_assembly_bit_identical(gpu_full):
+def test_two_pair_multislot_assembly(gpu_full, oracle_full):
     """Sorted L1, L2, L3 and M-L2 batches assemble the pairs past the special ones two per wavefront
-    with every slot (L0(x,x), L0(0,y), L0(x,0), the M-L2 terms): each output equals the one-pair
-    kernel's on the unsorted batch bit for bit, FP64 and FP32, AoS and SoA, an odd number of pairs
-    with special pairs mixed in, several passes of odd length, and the FP64 per-pair dot of the
-    lepton loop."""
+    with every slot (L0(x,x), L0(0,y), L0(x,0), the M-L2 terms).  M-L2 equals the one-pair kernel's
+    output on the unsorted batch bit for bit.  L1, L2 and L3 apply each Tabd map once to the
+    combined inputs of its slots, which rounds differently: they are held to the FP64 bar against
+    the one-pair kernel and the oracle.  FP64 and FP32, AoS and SoA, an odd number of pairs with
+    special pairs mixed in, several passes of odd length, and the FP64 per-pair dot of the lepton
+    loop."""
     import torch
     x, y = log_uniform(23, 4001)
     x[:3] = 0.0
@@ -161,17 +163,35 @@ def test_two_pair_multislot_assembly_bit_identical(gpu_full):
         refd = {w: loop(w, M) for w, M in cases}
         gpu_full.set_tune(old)
         gpu_full.set_tune(sort_min=0)
+        idx = np.random.default_rng(5).choice(len(x), 400, replace=False)
         for w, M in cases:
             print(f"multi-slot two-pair which={w}", flush=True)
-            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M), ref[w]), w
-            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA), ref[w].T), w
+            got = gpu_full.batch_host(w, x, y, M=M)
             got32 = gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_AOS_F32)
-            assert got32.dtype == np.float32 and np.array_equal(got32, ref32[w]), w
-            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA_F32), ref32[w].T), w
-            assert np.array_equal(loop(w, M).view(np.int64), refd[w].view(np.int64)), w  # NaN at x = 0 too
+            assert got32.dtype == np.float32
+            if w == 4:
+                assert np.array_equal(got, ref[w]), w
+                assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA), ref[w].T), w
+                assert np.array_equal(got32, ref32[w]), w
+                assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA_F32), ref32[w].T), w
+                assert np.array_equal(loop(w, M).view(np.int64), refd[w].view(np.int64)), w  # NaN at x = 0 too
+                continue
+            assert normwise(got, ref[w]).max() <= TOL, w
+            assert normwise(got[idx], oracle_full.batch(w, x[idx], y[idx], M=M)).max() <= TOL, w
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA), got.T), w
+            assert normwise(got32.astype(np.float64), got).max() <= 1e-5, w
+            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M, layout=kqed_amd.KQED_SOA_F32), got32.T), w
+            d, r = loop(w, M), refd[w]
+            fin = np.isfinite(r)
+            assert np.array_equal(np.isfinite(d), fin), w
+            assert np.all(np.abs(d[fin] - r[fin]) <= 1e-11 * np.maximum(np.abs(r[fin]), np.abs(r[fin]).max() * 1e-6)), w
         gpu_full.set_tune(chunk=999)
         for w, M in cases:
-            assert np.array_equal(gpu_full.batch_host(w, x, y, M=M), ref[w]), w
+            got = gpu_full.batch_host(w, x, y, M=M)
+            if w == 4:
+                assert np.array_equal(got, ref[w]), w
+            else:
+                assert normwise(got, ref[w]).max() <= TOL, w
     finally:
         gpu_full.set_tune(old)
 
