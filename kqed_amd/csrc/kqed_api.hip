@@ -28,7 +28,8 @@ __global__ void k_build_arrays(kqed_dev_tables T, const float *Fm, const float *
 __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
 __global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, int mode,
                            unsigned *key, unsigned *rank, unsigned *count);
-__global__ void k_scan(unsigned *count, unsigned nb);
+__global__ void k_scan_tiles(unsigned *count, unsigned nb, unsigned *tsum);
+__global__ void k_scan_add(unsigned *count, unsigned nb, const unsigned *tsum);
 __global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm);
 __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
 __global__ void k_inject_records(size_t n, double *rec, const unsigned *kinds, const double *w);
@@ -365,7 +366,7 @@ static constexpr size_t NA_KEY = NA_KINDS + al256((size_t)KQED_NOALLOC_PASS * 4)
 static constexpr size_t NA_RANK = NA_KEY + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
 static constexpr size_t NA_PERM = NA_RANK + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
 static constexpr size_t NA_COUNT = NA_PERM + al256((size_t)KQED_NOALLOC_MAX_PAIRS * 4);
-static constexpr size_t NA_BYTES = NA_COUNT + al256((size_t)KQED_NOALLOC_MAX_BUCKETS * 4);
+static constexpr size_t NA_BYTES = NA_COUNT + al256(((size_t)KQED_NOALLOC_MAX_BUCKETS + (KQED_NOALLOC_MAX_BUCKETS + 1023) / 1024) * 4);
 static __device__ __align__(256) unsigned char scratch[NA_BYTES];
 
 extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
@@ -449,6 +450,7 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
       /* standard map by column: entries of column c in row order */
       const kqed_map &m0 = d->maps[0];
       std::vector<int> cp(201, 0);
+      if (m0.nnz > 4096) { set_err("standard map has %d entries: the pull-back kernel holds at most 4096", m0.nnz); goto fail; }
       std::vector<short> rows(m0.nnz);
       std::vector<double> vals(m0.nnz);
       for (int e = 0; e < m0.nnz; e++) cp[m0.col[e] + 1]++;
@@ -510,7 +512,7 @@ extern "C" int initialise(struct QED_kernel_temps *t)
 static size_t batch_workspace(size_t n, size_t chunk, size_t nbuckets)
 {
   const size_t c = n < chunk ? n : chunk;
-  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * nbuckets;
+  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (nbuckets + (nbuckets + 1023) / 1024);
 }
 
 extern "C" size_t kqed_batch_workspace(size_t n)
@@ -535,7 +537,7 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
   if (w.cap < c || w.nbuckets != nb) {
     (void)hipFree(w.kinds); (void)hipFree(w.rec); (void)hipFree(w.count);
     w.kinds = w.count = nullptr; w.rec = nullptr; w.cap = 0;
-    if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * nb) != hipSuccess ||
+    if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * (nb + (nb + 1023) / 1024)) != hipSuccess ||
         hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
       set_err("workspace allocation of %zu pairs failed", c);
       return -2;
@@ -580,7 +582,9 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((w.nbuckets + 255) / 256)), dim3(256), 0, s, w.count, w.nbuckets, 0u);
     hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, tu.sort_mode, w.key, w.rank,
                        w.count);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets);
+    const unsigned ntile = (unsigned)((w.nbuckets + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets, w.count + w.nbuckets);
+    hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets, w.count + w.nbuckets);
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm);
     perm = w.perm;
     prof_end(d, s, ev);
@@ -1003,7 +1007,7 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
   if (e != hipSuccess) { set_err("kqed_contract: copy failed: %s", hipGetErrorString(e)); return -3; }
   int rc = 0;
   if (shortcut)
-    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)((nx * 200 + 255) / 256)), dim3(256), 0, s, nx, T_dev, d->csc_cp,
+    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)std::min((nx + 3) / 4, (size_t)4096)), dim3(256), 0, s, nx, T_dev, d->csc_cp,
                        d->csc_rows, d->csc_vals, w.W);
   for (size_t j = 0; j < ny && !rc; j++) {
     rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, LAYOUT_DOT, T_dev,

@@ -3,7 +3,7 @@
  *
  * Pipeline for a batch of (x,y) pairs (driven by kqed_api.hip):
  *   k_classify   regime + interpolation-cell key per pair, and its slot inside the cell
- *   k_count / k_scan / k_scatter   counting sort of the pairs by cell (table locality)
+ *   k_classify / k_scan_tiles / k_scan_add / k_scatter   counting sort of the pairs by cell (table locality)
  *   k_interp     one thread per pair, in cell order: invariants, weight-function interpolation
  *                (Clenshaw sums over the precomputed per-node arrays), the chain-rule algebra of
  *                chnr_dS/dT/dV and the Taylor-table lerps; writes a per-pair record
@@ -1558,23 +1558,44 @@ __global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
 }
 
 /* exclusive scan of the bucket counts, one block */
-__global__ void k_scan(unsigned *__restrict__ count, unsigned nb)
+/* Exclusive prefix of the nb bucket counts, in two launches over tiles of 1024 counters: k_scan_tiles
+ * turns each tile into its own exclusive prefix and writes the tile's total to tsum[tile];
+ * k_scan_add adds the sum of the preceding tiles' totals.  Coalesced loads and stores. */
+__global__ void __launch_bounds__(1024) k_scan_tiles(unsigned *__restrict__ count, unsigned nb, unsigned *__restrict__ tsum)
 {
-  __shared__ unsigned part[1024];
-  const unsigned t = threadIdx.x, per = (nb + 1023) / 1024;
-  const unsigned lo = t * per, hi = min(nb, lo + per);
-  unsigned s = 0;
-  for (unsigned b = lo; b < hi; b++) s += count[b];
-  part[t] = s;
-  __syncthreads();
-  for (unsigned off = 1; off < 1024; off <<= 1) {
-    const unsigned v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  __shared__ unsigned wsum[16];
+  const unsigned t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const unsigned i = blockIdx.x * 1024 + t;
+  const unsigned v = i < nb ? count[i] : 0u;
+  unsigned inc = v; /* inclusive prefix within the wavefront */
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned u = __shfl_up(inc, off, 64);
+    if (lane >= (unsigned)off) inc += u;
   }
-  unsigned run = t ? part[t - 1] : 0u;
-  for (unsigned b = lo; b < hi; b++) { const unsigned c = count[b]; count[b] = run; run += c; }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  unsigned base = 0;
+  for (unsigned k = 0; k < w; k++) base += wsum[k];
+  if (i < nb) count[i] = base + inc - v;
+  if (t == 1023) {
+    unsigned tot = 0;
+    for (unsigned k = 0; k < 16; k++) tot += wsum[k];
+    tsum[blockIdx.x] = tot;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_add(unsigned *__restrict__ count, unsigned nb, const unsigned *__restrict__ tsum)
+{
+  __shared__ unsigned off;
+  if (threadIdx.x == 0) {
+    unsigned s = 0;
+    for (unsigned k = 0; k < blockIdx.x; k++) s += tsum[k];
+    off = s;
+  }
+  __syncthreads();
+  const unsigned i = blockIdx.x * 1024 + threadIdx.x;
+  if (i < nb && blockIdx.x) count[i] += off;
 }
 
 /* start[b] is the exclusive prefix of the bucket counts: pair i goes to start[key] + its rank */
@@ -2200,17 +2221,37 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
 /* W[i][c] = sum over the entries (r, v) of column c of the standard assembly map of v * T[i][r]:
  * the site tensor pulled back through the linear assembly (columns in a fixed order, so the sums
  * are deterministic).  One thread per (site, column). */
-__global__ void k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restrict__ cp,
-                                   const short *__restrict__ rows, const double *__restrict__ vals, double *__restrict__ W)
+/* Pulled-back site tensors of the contraction shortcut: W[i][c] = sum over column c of the standard
+ * map of coefficient * T_i[row] (entries in the column's order).  One wavefront per site
+ * (grid-stride), the site's 384 FP32 values staged in LDS; the map's columns (nnz <= CW_NNZ) are
+ * held in LDS by the block.  Writes are coalesced rows of 200 doubles. */
+#define CW_NNZ 4096
+__global__ void __launch_bounds__(256)
+k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restrict__ cp,
+                   const short *__restrict__ rows, const double *__restrict__ vals, double *__restrict__ W)
 {
-  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= nx * 200) return;
-  const size_t i = id / 200;
-  const int c = (int)(id % 200);
-  const float *t = Tc + i * 384;
-  double s = 0.0;
-  for (int e = cp[c]; e < cp[c + 1]; e++) s = fma(vals[e], (double)t[rows[e]], s);
-  W[id] = s;
+  __shared__ int scp[201];
+  __shared__ short srows[CW_NNZ];
+  __shared__ double svals[CW_NNZ];
+  __shared__ float st[4][384];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nnz = cp[200];
+  for (int k = threadIdx.x; k < 201; k += blockDim.x) scp[k] = cp[k];
+  for (int k = threadIdx.x; k < nnz; k += blockDim.x) { srows[k] = rows[k]; svals[k] = vals[k]; }
+  __syncthreads();
+  for (size_t i = (size_t)blockIdx.x * 4 + w; i < nx; i += (size_t)gridDim.x * 4) {
+    const float *t = Tc + i * 384;
+#pragma unroll
+    for (int k = 0; k < 6; k++) st[w][lane + 64 * k] = t[lane + 64 * k];
+    wave_sync();
+    double *o = W + i * 200;
+    for (int c = lane; c < 200; c += 64) {
+      double s = 0.0;
+      for (int e = scp[c]; e < scp[c + 1]; e++) s = fma(svals[e], (double)st[w][srows[e]], s);
+      o[c] = s;
+    }
+    wave_sync();
+  }
 }
 
 /* Deterministic sum of in[0, n): block b reduces the contiguous range [b*per, (b+1)*per) in a fixed
