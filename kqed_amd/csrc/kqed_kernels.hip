@@ -1222,11 +1222,12 @@ DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, cons
 /* Interpolation of one pair into st.  CLS: 0 any pair, 1 packed-path pairs only, 2 Taylor-x pairs
  * only (the sort sends each class to its own kernel, which then carries only its own code path).
  * Returns 1 when the pair is outside the grid (the reference then leaves an all-zero kernel). */
+DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, const double *hy);
+
 template <int CLS>
-DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st)
+DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st, bool fuse, bool &alg_done)
 {
   const Inv I = set_inv(T, xv, yv);
-  write_header(st, I);
   if (CLS != 1 && (CLS == 2 || I.x < T.XX[0])) {
     /* Taylor-x region: every chnr_* takes its Taylor branch (scalar path, x node 0); the Taylor
      * branches already apply the algebra (chnr_dT.c:16-143, chnr_dV.c:19-174) */
@@ -1249,6 +1250,19 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
       if (T_taylor<false>(T, Q, I, iy, ity)) return 1;
       if (V_taylor<false>(T, D, I.x, I.y, I.xsq, I.flag2, I.cb, iy, ity)) return 1;
     }
+    if (fuse) {
+      /* the chain-rule algebra straight from registers: the stage-1 record is never written */
+      Stage1 s1;
+      write_header(&s1, I);
+      s1.need_alg = 0.0;
+      s1.dg0dx = dg0dx; s1.dg0dy = dg0dy; s1.dg0dcb = dg0dcb;
+      s1.T = Q;
+      s1.V = D;
+      algebra_finish((double *)st, s1, xv, yv);
+      alg_done = true;
+      return 0;
+    }
+    write_header(st, I);
     st->dg0dx = dg0dx; st->dg0dy = dg0dy; st->dg0dcb = dg0dcb;
     st->T = Q;
     st->V = D;
@@ -1257,6 +1271,7 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
   }
   if (CLS == 2) return 1;
   if (CLS != 1 && I.x > T.XX[T.nstpx - 1]) return 1;
+  write_header(st, I);
   st->need_alg = 1.0;
   Tt &Q = st->T;
   Vt &D = st->V;
@@ -1323,9 +1338,8 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
 
 /* Chain-rule algebra of chnr_dT/chnr_dV (standard branch), the x <-> x-y swap, and the 36
  * primitives the assembly kernel reads (S: 3, T: 12, invariants: 6, V: 15), written in place. */
-DEV void algebra_main(double *__restrict__ m, const double *hx, const double *hy)
+DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, const double *hy)
 {
-  Stage1 s = *(const Stage1 *)m;
   Inv I;
   I.x = s.x; I.cb = s.cb; I.y = s.y; I.xsq = s.xsq; I.ysq = s.ysq; I.xdoty = s.xdoty;
   I.cborig = s.cborig; I.yorig = s.yorig; I.xmysq = s.xmysq; I.xmy = s.xmy; I.flag2 = s.flag2 != 0.0;
@@ -1372,6 +1386,12 @@ DEV void algebra_main(double *__restrict__ m, const double *hx, const double *hy
   h[24] = 1.0;
   h[25] = 0.0;
   for (int i = 0; i < 26; i++) m[KQED_HELP_BASE + i] = h[i];
+}
+
+DEV void algebra_main(double *__restrict__ m, const double *hx, const double *hy)
+{
+  Stage1 s = *(const Stage1 *)m;
+  algebra_finish(m, s, hx, hy);
 }
 
 /* The algebra as its own pass (records whose stage-1 values come from elsewhere: the surrogate) */
@@ -1622,7 +1642,7 @@ DEV void plan_tabd(const DT &T, int kind, const double *v, double *payload, unsi
 /* L0(a,b) into slot `slot`: regime dispatch of reference kernels.c:82-115 */
 template <int CLS>
 DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot,
-                 const unsigned char *smask)
+                 const unsigned char *smask, bool fuse = false, bool *alg_done = nullptr)
 {
   const bool az = is_zero(a), bz = is_zero(b);
   double *pl = slot == 0 ? rec + KQED_REC_MAIN : rec + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
@@ -1643,7 +1663,9 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
       return;
     }
   }
-  if (interp_eval<CLS>(T, a, b, (Stage1 *)pl) == 0) kinds |= (unsigned)KIND_STD;
+  bool done = false;
+  if (interp_eval<CLS>(T, a, b, (Stage1 *)pl, fuse, done) == 0) kinds |= (unsigned)KIND_STD;
+  if (alg_done) *alg_done = done;
 }
 
 /* occupancy: the packed-path kernel runs at two waves per SIMD and the Taylor-region kernel at
@@ -1677,7 +1699,8 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   unsigned kd = 0;
   const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
   double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  plan_L0<CLS>(T, xv, yv, r, kd, 0, smask);
+  bool alg_done = false;
+  plan_L0<CLS>(T, xv, yv, r, kd, 0, smask, fuse != 0, &alg_done);
   switch (which) {
   case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
     plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, yv, yv, r, kd, 2, nullptr); s1 = -0.5; s2 = -0.5; break;
@@ -1704,7 +1727,7 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   r[KQED_REC_P] = (double)p;
   kinds[ri] = kd;
   /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
-  if (fuse && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
+  if (fuse && !alg_done && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
 }
 
 template __global__ void k_interp<0>(DT, size_t, const double *, int, const double *, int, const unsigned *,
