@@ -314,6 +314,9 @@ def main():
     ap.add_argument("--kernel", type=int, default=0, choices=range(5),
                     help="kernel of the timed workload: 0 L0 (the headline), 1 L1, 2 L2, 3 L3, 4 M-L2 (M = 0.5); "
                          "kernels other than L0 time the GPU only (no CPU baseline or roofline)")
+    ap.add_argument("--legs", default="all",
+                    help="comma list of extra legs to run (other_kernels, contraction, ragged, combinations, "
+                         "lepton_loop, lattice_symxy, surrogate) or 'all'")
     ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
     ap.add_argument("--surrogate", type=int, default=1, help="run the surrogate + FP32 leg (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=1 << 24, help="training pairs of the surrogate leg")
@@ -372,8 +375,12 @@ def main():
         samples[KNAMES[which]] = (which, M, idx, rows)
 
     extra = {}
+    legs = set(a.legs.split(",")) if a.legs != "all" else {"other_kernels", "contraction", "ragged", "combinations",
+                                                            "lepton_loop", "lattice_symxy", "surrogate"}
+    if a.no_extra:
+        legs = set()
     take_sample(a.kernel, 0.5 if a.kernel == 4 else 0.0, 1 << 18)
-    if not a.no_extra:
+    if "other_kernels" in legs:
         rates = {}
         for name, which, M in (("L1", 1, 0.0), ("L2", 2, 0.0), ("L3", 3, 0.0), ("ML2_M0.5", 4, 0.5)):
             if which == a.kernel:
@@ -388,15 +395,19 @@ def main():
             take_sample(which, M, 1 << 16)
         extra["evals_per_s_other_kernels"] = rates
 
-    if not a.no_extra:
+    if "contraction" in legs:
         extra["contraction"] = contraction_leg(t, dev, stream, a.lattice)
+    if "ragged" in legs:
         extra["ragged"] = ragged_leg(t, dev, stream)
+    if "combinations" in legs:
         extra["combinations"] = combination_leg(t, dev, stream, xd, yd, min(a.n, 1 << 19))
+    if "lepton_loop" in legs:
         extra["lepton_loop"] = lepton_loop_leg(t, dev, stream, 1 << 18)
+    if "lattice_symxy" in legs:
         extra["lattice_symxy"] = lattice_comb_leg(t, dev, stream, a.lattice)
-        if a.surrogate:
-            del out
-            extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
+    if "surrogate" in legs and a.surrogate:
+        del out
+        extra["surrogate"] = surrogate_leg(t, dev, stream, a.surrogate_train, a.surrogate_epochs, 1 << 26, evals)
 
     cpu = None
     flops_per_eval = None
