@@ -72,10 +72,9 @@ template <int AW, typename MT>
 __global__ void k_assemble2m(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
                              const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
-__global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const short *rows, const double *vals,
-                                   double *W);
+__global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const unsigned *ent, const int *sched,
+                                   double *W, double *colpart);
 __global__ void k_block_sum(size_t n, size_t per, const double *in, double *out);
-__global__ void k_colsum(size_t nx, size_t per, const float *Tc, double *part);
 __global__ void k_colsum_final(int nb, const double *part, double *sum);
 __global__ void k_combine(int ny, const double *sigma, const double *La, double ca, const double *sumT,
                           const double *consts, double cc0, double cc1, double *s);
@@ -166,8 +165,10 @@ struct kqed_device {
   kqed_map *maps = nullptr;
   /* the standard assembly map by column (for the contraction pull-back) */
   int *csc_cp = nullptr;
-  short *csc_rows = nullptr;
-  double *csc_vals = nullptr;
+  /* its entries as u32: upper 16 bits of the FP64 coefficient << 16 | row; and the pull-back's
+   * schedule: lane l takes columns csc_sched[65 + j] for j in [csc_sched[l], csc_sched[l + 1]) */
+  unsigned *csc_ent = nullptr;
+  int *csc_sched = nullptr;
   /* trace tables of the perturbative lepton loop (built on first use) */
   signed char *pert_tab = nullptr;
   /* every surrogate bin set: routes all packed-path pairs through the substitution step */
@@ -450,25 +451,46 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
       /* standard map by column: entries of column c in row order */
       const kqed_map &m0 = d->maps[0];
       std::vector<int> cp(201, 0);
-      if (m0.nnz > 4096) { set_err("standard map has %d entries: the pull-back kernel holds at most 4096", m0.nnz); goto fail; }
-      std::vector<short> rows(m0.nnz);
-      std::vector<double> vals(m0.nnz);
+      if (m0.nnz > 2560) { set_err("standard map has %d entries: the pull-back kernel holds at most 2560", m0.nnz); goto fail; }
+      std::vector<unsigned> ent(m0.nnz);
       for (int e = 0; e < m0.nnz; e++) cp[m0.col[e] + 1]++;
       for (int c = 0; c < 200; c++) cp[c + 1] += cp[c];
       std::vector<int> fill(cp.begin(), cp.end() - 1);
       for (int r = 0; r < 384; r++)
         for (int e = m0.rowptr[r]; e < m0.rowptr[r + 1]; e++) {
           const int k = fill[m0.col[e]]++;
-          rows[k] = (short)r;
-          vals[k] = m0.val[e];
+          unsigned long long bits;
+          memcpy(&bits, &m0.val[e], 8);
+          if (bits & 0xFFFFFFFFFFFFull) { set_err("standard map coefficient %g not encodable", m0.val[e]); goto fail; }
+          ent[k] = (unsigned)(bits >> 48) << 16 | (unsigned)r;
         }
+      /* columns dealt to the 64 lanes of a wavefront longest first, each to the least loaded lane
+       * (load: its entries plus one write), so the lanes of a site finish together */
+      std::vector<int> sched(65 + 200);
+      {
+        std::vector<int> order(200), load(64, 0);
+        std::vector<std::vector<int>> cols(64);
+        for (int c = 0; c < 200; c++) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cp[a + 1] - cp[a] > cp[b + 1] - cp[b]; });
+        for (int c : order) {
+          const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+          cols[l].push_back(c);
+          load[l] += cp[c + 1] - cp[c] + 1;
+        }
+        int k = 65;
+        for (int l = 0; l < 64; l++) {
+          sched[l] = k;
+          for (int c : cols[l]) sched[k++] = c;
+        }
+        sched[64] = k;
+      }
       d->csc_cp = (int *)dev_alloc(d, 4 * 201);
-      d->csc_rows = (short *)dev_alloc(d, 2 * (size_t)m0.nnz);
-      d->csc_vals = (double *)dev_alloc(d, 8 * (size_t)m0.nnz);
-      if (!d->csc_cp || !d->csc_rows || !d->csc_vals) { set_err("device allocation failed"); goto fail; }
+      d->csc_ent = (unsigned *)dev_alloc(d, 4 * (size_t)m0.nnz);
+      d->csc_sched = (int *)dev_alloc(d, 4 * sched.size());
+      if (!d->csc_cp || !d->csc_ent || !d->csc_sched) { set_err("device allocation failed"); goto fail; }
       HIPCHK(hipMemcpy(d->csc_cp, cp.data(), 4 * 201, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d->csc_rows, rows.data(), 2 * (size_t)m0.nnz, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d->csc_vals, vals.data(), 8 * (size_t)m0.nnz, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d->csc_ent, ent.data(), 4 * (size_t)m0.nnz, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(d->csc_sched, sched.data(), 4 * sched.size(), hipMemcpyHostToDevice));
     }
   }
   /* host-visible mirror of the grid (reference Grid_coeffs fields) */
@@ -635,9 +657,15 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
                      ((f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc || (Wc && layout == LAYOUT_DOT));
     /* several slots (L1..L3, M-L2): the multi-slot two-pair kernel, which also forms the
      * contraction dot */
-    const bool twom = perm && nslots[which] > 1 && !Wc &&
-                      (f32 || layout == KQED_AOS || layout == KQED_SOA || layout == LAYOUT_DOT || layout == LAYOUT_DOT64);
-    const unsigned *split = two || twom ? w.count + 1 : nullptr;
+    /* a one-slot dot pass without the linear shortcut (the contraction's x-only sums, whose pairs are
+     * all special points): every record, special or not, two per wavefront in the multi-slot kernel */
+    const bool dot_all = perm && nslots[which] == 1 && !Wc && layout == LAYOUT_DOT;
+    const bool twom = perm && !Wc &&
+                      ((nslots[which] > 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA || layout == LAYOUT_DOT ||
+                                              layout == LAYOUT_DOT64)) || dot_all);
+    /* split[0]: the first record the two-pair kernels take (count[1]: past the special pairs; count[0]
+     * = 0: every record); the one-pair kernel takes the records before it */
+    const unsigned *split = dot_all ? w.count : two || twom ? w.count + 1 : nullptr;
     if (f32)
       hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
                          nslots[which], n, Tc, Wc, split, c0);
@@ -1006,9 +1034,11 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
   hipError_t e = hipMemcpyAsync(yd, y, 8 * 4 * ny, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) { set_err("kqed_contract: copy failed: %s", hipGetErrorString(e)); return -3; }
   int rc = 0;
+  /* pull-back of the site tensors; for L1..L3 the same pass sums them per column (sum_x T) */
+  const size_t cw_blocks = std::min((nx + 3) / 4, RED_BLOCKS);
   if (shortcut)
-    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)std::min((nx + 3) / 4, (size_t)4096)), dim3(256), 0, s, nx, T_dev, d->csc_cp,
-                       d->csc_rows, d->csc_vals, w.W);
+    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)cw_blocks), dim3(256), 0, s, nx, T_dev, d->csc_cp, d->csc_ent,
+                       d->csc_sched, w.W, which != KQED_L0 ? colpart : nullptr);
   for (size_t j = 0; j < ny && !rc; j++) {
     rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, LAYOUT_DOT, T_dev,
                    shortcut ? w.W : nullptr);
@@ -1017,9 +1047,7 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
   double ca = 0.0, cc0 = 0.0, cc1 = 0.0;
   bool with_y = false;
   if (!rc && shortcut && which != KQED_L0) {
-    const size_t nb = std::min(RED_BLOCKS, nx), per = (nx + nb - 1) / nb;
-    hipLaunchKernelGGL(k_colsum, dim3((unsigned)nb), dim3(384), 0, s, nx, per, T_dev, colpart);
-    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(384), 0, s, (int)nb, colpart, sumT);
+    hipLaunchKernelGGL(k_colsum_final, dim3(384), dim3(256), 0, s, (int)cw_blocks, colpart, sumT);
     with_y = true;
     /* x-only sums over the sites: consts[k] = sum_x L0(a_k(x), b_k(x)) . T(x) */
     const double *ax[2] = { nullptr, nullptr }, *bx[2] = { nullptr, nullptr };

@@ -2245,36 +2245,66 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
  * the site tensor pulled back through the linear assembly (columns in a fixed order, so the sums
  * are deterministic).  One thread per (site, column). */
 /* Pulled-back site tensors of the contraction shortcut: W[i][c] = sum over column c of the standard
- * map of coefficient * T_i[row] (entries in the column's order).  One wavefront per site
- * (grid-stride), the site's 384 FP32 values staged in LDS; the map's columns (nnz <= CW_NNZ) are
- * held in LDS by the block.  Writes are coalesced rows of 200 doubles. */
-#define CW_NNZ 4096
+ * map of coefficient * T_i[row] (entries in the column's order).  One wavefront per site, the sites
+ * of a wavefront strided by the grid; the site's 384 FP32 values are staged in LDS (the next site's
+ * are already loaded), the map's columns (nnz <= CW_NNZ, u32 entries: coefficient's upper 16 bits,
+ * row) are held in LDS by the block and dealt to the lanes by the host's balanced schedule.  With colpart, the block also sums its sites' tensors per column in
+ * a fixed order (each wavefront in site order, then the four wavefronts in order) into
+ * colpart[block][384]. */
+#define CW_NNZ 2560
 __global__ void __launch_bounds__(256)
 k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restrict__ cp,
-                   const short *__restrict__ rows, const double *__restrict__ vals, double *__restrict__ W)
+                   const unsigned *__restrict__ ent, const int *__restrict__ sched, double *__restrict__ W,
+                   double *__restrict__ colpart)
 {
   __shared__ int scp[201];
-  __shared__ short srows[CW_NNZ];
-  __shared__ double svals[CW_NNZ];
+  __shared__ unsigned sent[CW_NNZ];
+  __shared__ int ssched[65 + 200];
   __shared__ float st[4][384];
+  __shared__ double psum[4][384];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nnz = cp[200];
   for (int k = threadIdx.x; k < 201; k += blockDim.x) scp[k] = cp[k];
-  for (int k = threadIdx.x; k < nnz; k += blockDim.x) { srows[k] = rows[k]; svals[k] = vals[k]; }
+  for (int k = threadIdx.x; k < 265; k += blockDim.x) ssched[k] = sched[k];
+  for (int k = threadIdx.x; k < nnz; k += blockDim.x) sent[k] = ent[k];
   __syncthreads();
-  for (size_t i = (size_t)blockIdx.x * 4 + w; i < nx; i += (size_t)gridDim.x * 4) {
-    const float *t = Tc + i * 384;
+  /* this lane's columns (dealt to balance the lanes' entry counts) */
+  const int j0 = ssched[lane], j1 = ssched[lane + 1];
+  double cs[6] = { 0, 0, 0, 0, 0, 0 };
+  const size_t stride = (size_t)gridDim.x * 4;
+  size_t i = (size_t)blockIdx.x * 4 + w;
+  float nt[6];
+  if (i < nx)
 #pragma unroll
-    for (int k = 0; k < 6; k++) st[w][lane + 64 * k] = t[lane + 64 * k];
+    for (int k = 0; k < 6; k++) nt[k] = Tc[i * 384 + lane + 64 * k];
+  for (; i < nx; i += stride) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      st[w][lane + 64 * k] = nt[k];
+      cs[k] += (double)nt[k];
+    }
+    if (i + stride < nx)
+#pragma unroll
+      for (int k = 0; k < 6; k++) nt[k] = Tc[(i + stride) * 384 + lane + 64 * k];
     wave_sync();
     double *o = W + i * 200;
-    for (int c = lane; c < 200; c += 64) {
+    for (int j = j0; j < j1; j++) {
+      const int c = ssched[j];
       double s = 0.0;
-      for (int e = scp[c]; e < scp[c + 1]; e++) s = fma(svals[e], (double)st[w][srows[e]], s);
+      for (int e = scp[c]; e < scp[c + 1]; e++) {
+        const unsigned u = sent[e];
+        s = fma(__hiloint2double((int)(u & 0xFFFF0000u), 0), (double)st[w][u & 0xFFFFu], s);
+      }
       o[c] = s;
     }
     wave_sync();
   }
+  if (!colpart) return;
+#pragma unroll
+  for (int k = 0; k < 6; k++) psum[w][lane + 64 * k] = cs[k];
+  __syncthreads();
+  for (int c = threadIdx.x; c < 384; c += blockDim.x)
+    colpart[(size_t)blockIdx.x * 384 + c] = ((psum[0][c] + psum[1][c]) + psum[2][c]) + psum[3][c];
 }
 
 /* Deterministic sum of in[0, n): block b reduces the contiguous range [b*per, (b+1)*per) in a fixed
@@ -2294,23 +2324,21 @@ __global__ void k_block_sum(size_t n, size_t per, const double *__restrict__ in,
   if (threadIdx.x == 0) out[blockIdx.x] = part[0];
 }
 
-/* Column sums of the site tensors, FP64: block b sums rows [b*per, (b+1)*per) of column
- * threadIdx.x into part[b][c] (384 threads). */
-__global__ void k_colsum(size_t nx, size_t per, const float *__restrict__ Tc, double *__restrict__ part)
+/* sum[c] = sum over the nb partials part[b][c] in a fixed order: block c, thread t adds the partials
+ * b = t, t + 256, ..., then a fixed tree over the 256 threads */
+__global__ void __launch_bounds__(256) k_colsum_final(int nb, const double *__restrict__ part, double *__restrict__ sum)
 {
-  const int c = threadIdx.x;
-  const size_t lo = (size_t)blockIdx.x * per, hi = lo + per < nx ? lo + per : nx;
+  __shared__ double r[256];
+  const int c = blockIdx.x, t = threadIdx.x;
   double s = 0.0;
-  for (size_t i = lo; i < hi; i++) s += (double)Tc[i * 384 + c];
-  part[(size_t)blockIdx.x * 384 + c] = s;
-}
-
-__global__ void k_colsum_final(int nb, const double *__restrict__ part, double *__restrict__ sum)
-{
-  const int c = threadIdx.x;
-  double s = 0.0;
-  for (int b = 0; b < nb; b++) s += part[(size_t)b * 384 + c];
-  sum[c] = s;
+  for (int b = t; b < nb; b += 256) s += part[(size_t)b * 384 + c];
+  r[t] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (t < h) r[t] += r[t + h];
+    __syncthreads();
+  }
+  if (t == 0) sum[c] = r[0];
 }
 
 /* s[j] = sigma[j] + ca * (La[j] . sumT) + cc0 * consts[0] + cc1 * consts[1]  (one wavefront per j) */
