@@ -320,7 +320,7 @@ def main():
     ap.add_argument("--lattice", type=int, default=32, help="lattice extent of the contraction leg")
     ap.add_argument("--surrogate", type=int, default=1, help="run the surrogate + FP32 leg (0 = skip)")
     ap.add_argument("--surrogate-train", type=int, default=1 << 24, help="training pairs of the surrogate leg")
-    ap.add_argument("--surrogate-epochs", type=int, default=1)
+    ap.add_argument("--surrogate-epochs", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (reference kernel for HBM byte counters)")
     a = ap.parse_args()

@@ -1743,7 +1743,7 @@ static void surr_fwd_bwd(kqed_surrogate *g, int B)
     const float *Ain = l ? g->A[l - 1] : g->X;
     const long ldA = l ? 128 : 16;
     gemm(s, SK[l], SN[l], B, Ain, 1, ldA, g->dZ, SN[l], 1, g->G + surr_off_W(l), SN[l], nullptr, g->part);
-    hipLaunchKernelGGL(k_colsum_f, dim3((SN[l] + 127) / 128), dim3(128), 0, s, B, SN[l], g->dZ, g->G + surr_off_b(l));
+    hipLaunchKernelGGL(k_colsum_f, dim3((SN[l] + 63) / 64), dim3(1024), 0, s, B, SN[l], g->dZ, g->G + surr_off_b(l));
     if (l == 0) break;
     /* dA_{l-1} = dZ_l W_l^T, then dZ_{l-1} = dA_{l-1} silu'(Z_{l-1}) */
     gemm(s, B, SK[l], SN[l], g->dZ, SN[l], 1, g->P + surr_off_W(l), 1, SN[l], g->dA, SK[l], nullptr, g->part);

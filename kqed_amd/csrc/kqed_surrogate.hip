@@ -95,14 +95,24 @@ __global__ void k_silu_bwd(long n, const float *__restrict__ Z, const float *__r
   dZ[i] = dA[i] * s * (1.f + Z[i] * (1.f - s));
 }
 
-/* db[n] = sum_b dZ[b][n] (one thread per column, rows in order) */
-__global__ void k_colsum_f(int B, int N, const float *__restrict__ dZ, float *__restrict__ db)
+/* db[n] = sum_b dZ[b][n] in a fixed order: a block of 1024 threads takes 64 columns; thread
+ * (phase p = t / 64, column t % 64) sums rows b = p, p + 16, ... (a wavefront reads 64 consecutive
+ * columns of one row), then the 16 phases are added in order */
+__global__ void __launch_bounds__(1024) k_colsum_f(int B, int N, const float *__restrict__ dZ, float *__restrict__ db)
 {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+  __shared__ float ps[16][64];
+  const int c = threadIdx.x & 63, p = threadIdx.x >> 6;
+  const int n = blockIdx.x * 64 + c;
   float s = 0.f;
-  for (int b = 0; b < B; b++) s += dZ[(long)b * N + n];
-  db[n] = s;
+  if (n < N)
+    for (int b = p; b < B; b += 16) s += dZ[(long)b * N + n];
+  ps[p][c] = s;
+  __syncthreads();
+  if (p == 0 && n < N) {
+    float t = 0.f;
+    for (int k = 0; k < 16; k++) t += ps[k][c];
+    db[n] = t;
+  }
 }
 
 /* Mean squared error over the valid (sample, output) entries: pred[B][NP], tgt/wgt[B][NOUT].
