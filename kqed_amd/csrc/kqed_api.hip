@@ -71,6 +71,9 @@ __global__ void k_assemble2(kqed_dev_tables T, size_t n, const unsigned *split, 
 template <int AW, typename MT>
 __global__ void k_assemble2m(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
                              const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
+template <int AW, typename MT>
+__global__ void k_assemble2t(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
+                             const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
 __global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const unsigned *ent, const int *sched,
                                    double *W, double *colpart);
@@ -654,7 +657,7 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
      * special ones to the general one */
     const bool f32 = layout == KQED_AOS_F32 || layout == KQED_SOA_F32;
     const bool two = perm && nslots[which] == 1 &&
-                     ((f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc || (Wc && layout == LAYOUT_DOT));
+                     (((f32 || layout == KQED_AOS || layout == KQED_SOA) && !Wc) || (Wc && layout == LAYOUT_DOT));
     /* several slots (L1..L3, M-L2): the multi-slot two-pair kernel, which also forms the
      * contraction dot */
     /* a one-slot dot pass without the linear shortcut (the contraction's x-only sums, whose pairs are
@@ -685,7 +688,17 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
         hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double, false>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
                            split, c0, w.rec, w.kinds, out_dev, layout, n, (const double *)nullptr);
     }
-    if (twom) {
+    if (twom && which >= 1 && which <= 3) {
+      /* L1, L2, L3: one Tabd map application on the combined inputs */
+      const size_t nb2 = ((c + 1) / 2 + KQED_ASM2T_WAVES - 1) / KQED_ASM2T_WAVES;
+      const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
+      if (f32)
+        hipLaunchKernelGGL((k_assemble2t<KQED_ASM2T_WAVES, float>), dim3(grid2), dim3(64 * KQED_ASM2T_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc);
+      else
+        hipLaunchKernelGGL((k_assemble2t<KQED_ASM2T_WAVES, double>), dim3(grid2), dim3(64 * KQED_ASM2T_WAVES), 0, s, T, c,
+                           split, c0, w.rec, w.kinds, out_dev, layout, nslots[which], n, Tc);
+    } else if (twom) {
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2M_WAVES - 1) / KQED_ASM2M_WAVES;
       const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
       if (f32)

@@ -59,6 +59,7 @@
 #define KQED_INTERP_THREADS 128 /* interpolation block: two wavefronts (256 measured 1.5-2% slower) */
 #define KQED_ASM2_WAVES 12  /* wavefronts per two-pair assembly block (LDS 100 KB; 16 measured 18 % slower: VGPR-bound) */
 #define KQED_ASM2M_WAVES 8 /* multi-slot two-pair assembly: LDS 139 KB (all three maps, record, O) */
+#define KQED_ASM2T_WAVES 12 /* L1-L3 two-pair assembly with one Tabd map application: LDS 112 KB */
 #define KQED_ASM_WAVES 16   /* wavefronts per assembly block: one block (LDS 136 KB) fills a CU */
 
 enum {

@@ -2868,6 +2868,247 @@ template __global__ void k_assemble2m<KQED_ASM2M_WAVES, double>(DT, size_t, cons
 template __global__ void k_assemble2m<KQED_ASM2M_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
                                                                const unsigned *, void *, int, int, size_t, const void *);
 
+/* Tabd tensor entry c of a slot's payload pl (unit vector h) for kind k: Tabd_xeq0, Tabd_yeq0, or,
+ * for L(v, v) = -L_{mu<->lambda}(v, 0), the Tabd_yeq0 entry that the Tabd map needs at c for the
+ * same outputs: the trace reversal Tr{Gam g_l g_a g_n g_b g_m} = -Tr{Gam g_m g_b g_n g_a g_l} turns
+ * the mu<->lambda exchange of the rows into alpha<->beta on the inputs with T^II and T^III exchanged,
+ * and the sign with it: (T^I, T^II, T^III)_abd <- (T^I, T^III, T^II)_bad. */
+DEV D2 tabd_value(unsigned k, const D2 *pl, const D2 *h, int c)
+{
+  if (k == KIND_XEQ0) return xeq0_entry(pl, h, c);
+  if (k == KIND_YEQ0) return yeq0_entry(pl, h, c);
+  const int t = c >> 6, swp = 16 * ((c >> 2) & 3) + 4 * ((c >> 4) & 3) + (c & 3);
+  return yeq0_entry(pl, h, (t == 0 ? 0 : t == 1 ? 128 : 64) + swp);
+}
+
+/* L1, L2, L3 in sorted order past the special pairs, two records per wavefront, with ONE application
+ * of the Tabd map.  Both maps of kqed_assembly.c are the Dirac-trace contraction
+ * L = sum_A G^A T^(A); the standard map is it on the S/T/V entries folded into
+ *   T^I = Vv, T^II_abd = Txv[b][a][d] + 1/4 d_bd Sxv[a], T^III_abd = Txv + Tyv + 1/4 d_ad (Sxv + Syv)[b],
+ * and L(v, v) is it on permuted Tabd_yeq0 entries (tabd_value).  The kernel combinations are linear,
+ * so (reference kernels.c:119-177)
+ *   L1 = M (F(u) - 1/2 s(t(x,0)) - 1/2 s(t(y,0))),  L2 = M (F(u) - t(0,y) - t(x,0)),
+ *   L3 = M (F(u) - s(t(x,0)) + t(0,x) - t(0,y)),
+ * one gather of 2016 map entries per pair where the slot-by-slot form runs 2280 + 2016 (L1, L2) or
+ * 2280 + 2 x 2016 (L3).  The inputs are combined in FP64 (standard part, then the slots in order
+ * with their coefficients; an empty slot adds nothing), so the result rounds differently from the
+ * one-pair kernel, within the FP64 bar.  Only the rep columns of the folded T^II (b <= d) and
+ * T^III (a <= d) are formed: the map reads no other.
+ * Layouts as k_assemble2m (0/1 AoS/SoA of MT; 16/64 the contraction dot).  The work area holds the
+ * helpers and the standard entries (FP64, natural order), then the combined inputs (MT), then the
+ * outputs, each phase overlaying the previous one after a wavefront barrier. */
+template <int AW, typename MT>
+__global__ void __launch_bounds__(64 * AW)
+k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const double *__restrict__ rec,
+             const unsigned *__restrict__ kinds, void *__restrict__ outv, int layout, int nslots, size_t ntot,
+             const void *__restrict__ Tc)
+{
+  __shared__ unsigned sE[64 * KQED_WM_PER_LANE]; /* the Tabd map (map 1) */
+  __shared__ unsigned short sRow[384];
+  __shared__ D2 sW[AW][WREC + WT];
+  static_assert(WH + 201 <= WT, "helpers and standard entries fit the work area");
+  size_t lo = split[0] > c0 ? split[0] - c0 : 0;
+  lo = lo < n ? lo : n;
+  const size_t np = (n - lo + 1) / 2; /* record pairs */
+  if ((size_t)blockIdx.x * AW >= np) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = threadIdx.x; i < 64 * KQED_WM_PER_LANE; i += blockDim.x) sE[i] = T.wmE[T.wmOff[1][0] + i];
+  for (int i = threadIdx.x; i < 384; i += blockDim.x) sRow[i] = T.wmRow[384 + i];
+  __syncthreads();
+  typedef typename std::conditional<sizeof(MT) == 8, D2, F2>::type TV;
+  D2 *R = sW[w], *H = R + WREC, *Ts = H + WH, *O = H;
+  TV *U = (TV *)H;
+  const size_t stride = (size_t)gridDim.x * AW;
+  const bool dotl = layout == 16 || layout == 64;
+  auto hdesc = [](int g) {
+    const unsigned char *e = HDESC[g < 19 ? g : 0];
+    return make_uint2(e[0] | e[1] << 8 | e[2] << 16 | (unsigned)e[3] << 24, e[4] | e[5] << 8 | e[6] << 16);
+  };
+  const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
+  /* the rep columns this lane forms: T^I always, T^II when b <= d, T^III when a <= d */
+  const int la = lane >> 4, lb = (lane >> 2) & 3, ld = lane & 3;
+  const bool rep2 = lb <= ld, rep3 = la <= ld;
+  static_assert(WREC > 128 && WREC <= 192, "record staging covers three doubles per lane");
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0;
+  unsigned ka = KIND_NONE, kb = KIND_NONE;
+  const int used = KQED_REC_AUX + KQED_AUX_LEN * (nslots - 1);
+  auto load = [&](size_t qq) {
+    const size_t ia = lo + 2 * qq, ib = ia + 1;
+    const double *ra = rec + ia * WREC;
+    a0 = __builtin_nontemporal_load(ra + lane);
+    a1 = __builtin_nontemporal_load(ra + lane + 64);
+    a2 = lane + 128 < used ? __builtin_nontemporal_load(ra + lane + 128) : 0.0;
+    ka = kinds[ia];
+    if (ib < n) {
+      const double *rb = ra + WREC;
+      b0 = __builtin_nontemporal_load(rb + lane);
+      b1 = __builtin_nontemporal_load(rb + lane + 64);
+      b2 = lane + 128 < used ? __builtin_nontemporal_load(rb + lane + 128) : 0.0;
+      kb = kinds[ib];
+    } else {
+      b0 = b1 = b2 = 0.0;
+      kb = KIND_NONE;
+    }
+  };
+  size_t q = (size_t)blockIdx.x * AW + w;
+  if (q < np) load(q);
+  size_t pa = 0, pb = 0;
+  double pda = 0.0, pdb = 0.0;
+  bool pending = false, pbv = false;
+  auto flush = [&]() {
+    if (dotl) {
+      if (lane == 0) {
+        ((double *)outv)[pa] = pda;
+        if (pbv) ((double *)outv)[pb] = pdb;
+      }
+      return;
+    }
+    if constexpr (sizeof(MT) == 8) {
+      if (!(layout & 1)) {
+        /* AoS FP64: two consecutive outputs of a pair per lane, one 16-byte store each */
+        typedef double dbl2 __attribute__((ext_vector_type(2)));
+        double *out = (double *)outv;
+        for (int i = 2 * lane; i < 384; i += 128) {
+          const D2 o0 = O[i], o1 = O[i + 1];
+          const dbl2 va = { o0.x, o1.x };
+          __builtin_nontemporal_store(va, (dbl2 *)(out + pa * 384 + i));
+          if (pbv) {
+            const dbl2 vb = { o0.y, o1.y };
+            __builtin_nontemporal_store(vb, (dbl2 *)(out + pb * 384 + i));
+          }
+        }
+        return;
+      }
+    }
+    MT *out = (MT *)outv;
+    for (int i = lane; i < 384; i += 64) {
+      const D2 o = O[i];
+      if (layout & 1) {
+        __builtin_nontemporal_store((MT)o.x, out + (size_t)i * ntot + pa);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + (size_t)i * ntot + pb);
+      } else {
+        __builtin_nontemporal_store((MT)o.x, out + pa * 384 + i);
+        if (pbv) __builtin_nontemporal_store((MT)o.y, out + pb * 384 + i);
+      }
+    }
+  };
+  for (; q < np; q += stride) {
+    R[lane] = D2{ a0, b0 };
+    R[lane + 64] = D2{ a1, b1 };
+    if (lane + 128 < used) R[lane + 128] = D2{ a2, b2 };
+    const unsigned kda = __builtin_amdgcn_readfirstlane(ka), kdb = __builtin_amdgcn_readfirstlane(kb);
+    const bool hasb = lo + 2 * q + 1 < n;
+    if (pending) flush();
+    if (q + stride < np) load(q + stride);
+    wave_sync();
+    const size_t qa = (size_t)R[KQED_REC_P].x, qb = hasb ? (size_t)R[KQED_REC_P].y : qa;
+    const bool sa = (kda & 7u) == KIND_STD, sb = (kdb & 7u) == KIND_STD;
+    /* the standard part: helpers, then the 200 S/T/V entries (FP64, column order) */
+    D2 u0 = D2{ 0.0, 0.0 }, u1 = u0, u2 = u0;
+    if (sa || sb) {
+      H[lane] = std_helper(R, lane, hd0);
+      if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
+      wave_sync();
+      for (int c = lane; c < 200; c += 64) Ts[c] = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
+      wave_sync();
+      /* fold onto T^I, T^II, T^III (reps only) */
+      u0 = Ts[lane];
+      if (rep2) {
+        u1 = Ts[64 + 16 * lb + 4 * la + ld];
+        if (lb == ld) u1 = u1 + 0.25 * Ts[192 + la];
+      }
+      if (rep3) {
+        u2 = Ts[64 + lane] + Ts[128 + lane];
+        if (la == ld) u2 = u2 + 0.25 * (Ts[192 + lb] + Ts[196 + lb]);
+      }
+      if (!sa) { u0.x = 0.0; u1.x = 0.0; u2.x = 0.0; }
+      if (!sb) { u0.y = 0.0; u1.y = 0.0; u2.y = 0.0; }
+      wave_sync(); /* the helper area takes the slots' unit vectors next */
+    }
+    /* unit vectors of the Tabd slots' arguments, H[4 (s - 1) + i] */
+    if (lane < 4 * (nslots - 1)) {
+      const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (lane >> 2);
+      H[lane] = pl[lane & 3] / pl[4];
+    }
+    wave_sync();
+    for (int s2 = 1; s2 < nslots; s2++) {
+      const unsigned ks_a = (kda >> (3 * s2)) & 7u, ks_b = (kdb >> (3 * s2)) & 7u;
+      if (ks_a == KIND_NONE && ks_b == KIND_NONE) continue;
+      const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (s2 - 1);
+      const D2 *h = H + 4 * (s2 - 1);
+      const D2 cf = R[KQED_REC_COEF + s2];
+      /* an empty slot's payload is not read: its component is zero */
+      const D2 cc = D2{ ks_a != KIND_NONE ? cf.x : 0.0, ks_b != KIND_NONE ? cf.y : 0.0 };
+      const unsigned k1 = ks_a != KIND_NONE ? ks_a : ks_b;
+      D2 t0 = tabd_value(k1, pl, h, lane);
+      D2 t1 = rep2 ? tabd_value(k1, pl, h, lane + 64) : D2{ 0.0, 0.0 };
+      D2 t2 = rep3 ? tabd_value(k1, pl, h, lane + 128) : D2{ 0.0, 0.0 };
+      if (ks_b != KIND_NONE && ks_b != k1) {
+        /* the records' kinds differ in this slot: b's component from its own kind */
+        t0.y = tabd_value(ks_b, pl, h, lane).y;
+        if (rep2) t1.y = tabd_value(ks_b, pl, h, lane + 64).y;
+        if (rep3) t2.y = tabd_value(ks_b, pl, h, lane + 128).y;
+      }
+      if (ks_a == KIND_NONE) t0.x = t1.x = t2.x = 0.0;
+      if (ks_b == KIND_NONE) t0.y = t1.y = t2.y = 0.0;
+      u0 = fma(cc, t0, u0);
+      u1 = fma(cc, t1, u1);
+      u2 = fma(cc, t2, u2);
+    }
+    wave_sync(); /* every lane is done with the helpers and entries, which U overlays */
+    U[lane] = TV{ (MT)u0.x, (MT)u0.y };
+    U[lane + 64] = TV{ (MT)u1.x, (MT)u1.y };
+    U[lane + 128] = TV{ (MT)u2.x, (MT)u2.y };
+    if (lane == 0) U[200] = TV{ 0, 0 };
+    wave_sync();
+    D2 v[6];
+    v[0] = wm_dot2<13>(sE, lane, U);
+    v[1] = wm_dot2<9>(sE + 64 * 13, lane, U);
+    v[2] = wm_dot2<6>(sE + 64 * 22, lane, U);
+    v[3] = wm_dot2<6>(sE + 64 * 28, lane, U);
+    v[4] = wm_dot2<4>(sE + 64 * 34, lane, U);
+    v[5] = wm_dot2<3>(sE + 64 * 38, lane, U);
+    wave_sync(); /* every lane is done with U, which O overlays */
+#pragma unroll
+    for (int m = 0; m < 6; m++) O[sRow[64 * m + lane]] = v[m];
+    double da = 0.0, db = 0.0;
+    if (dotl) {
+      wave_sync();
+      double xa = 0.0, xb = 0.0;
+      if (layout == 16) {
+        const float *ta = (const float *)Tc + qa * 384, *tb = (const float *)Tc + qb * 384;
+        for (int i = lane; i < 384; i += 64) {
+          const D2 o = O[i];
+          xa = fma(o.x, (double)ta[i], xa);
+          xb = fma(o.y, (double)tb[i], xb);
+        }
+      } else {
+        const double *ta = (const double *)Tc + qa * 384, *tb = (const double *)Tc + qb * 384;
+        for (int i = lane; i < 384; i += 64) {
+          const D2 o = O[i];
+          xa = fma(o.x, ta[i], xa);
+          xb = fma(o.y, tb[i], xb);
+        }
+      }
+      da = wave_sum(xa);
+      db = wave_sum(xb);
+    }
+    pa = qa;
+    pb = qb;
+    pda = da;
+    pdb = db;
+    pbv = hasb;
+    pending = true;
+    wave_sync();
+  }
+  if (pending) flush();
+}
+
+template __global__ void k_assemble2t<KQED_ASM2T_WAVES, double>(DT, size_t, const unsigned *, size_t, const double *,
+                                                                const unsigned *, void *, int, int, size_t, const void *);
+template __global__ void k_assemble2t<KQED_ASM2T_WAVES, float>(DT, size_t, const unsigned *, size_t, const double *,
+                                                               const unsigned *, void *, int, int, size_t, const void *);
+
 /* ragged groups: ypair[i] = y[g] for off[g] <= i < off[g+1] (binary search over the ng+1 offsets) */
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *__restrict__ off,
                                 const double *__restrict__ y, double *__restrict__ ypair)
