@@ -287,3 +287,50 @@ def test_assembly_matches_reference_text():
     B, Me = P(B, 64, (1, 2)), P(Me, 64, (1, 2))
     B, Me = P(B, 128, (0, 2)), P(Me, 128, (0, 2))
     assert np.abs(B - Me).max() == 0.0
+
+
+def _sym_fill(t, off, sw):
+    """Give the 64 inputs at off the symmetry under swapping tensor slots sw (copy rep onto twin)."""
+    for c in range(64):
+        idx = [c >> 4, (c >> 2) & 3, c & 3]
+        j = idx.copy()
+        j[sw[0]], j[sw[1]] = idx[sw[1]], idx[sw[0]]
+        cj = 16 * j[0] + 4 * j[1] + j[2]
+        if cj < c:
+            t[..., off + c] = t[..., off + cj]
+    return t
+
+
+def test_single_map_factorisations():
+    """The identities the L1-L3 assembly kernel runs on (one Tabd map application per pair), checked
+    on the oracle's maps, which are built from explicit gamma matrices:
+    (i) standard map = Tabd map after folding S/T/V into T^I = Vv, T^II_abd = Txv[b][a][d] +
+        1/4 d_bd Sxv[a], T^III_abd = Txv + Tyv + 1/4 d_ad (Sxv + Syv)[b];
+    (ii) L(v,v) = -L_{mu<->lambda}(v,0) (reference kernels.c:30-48) = the Tabd map on
+        (T^I, T^II, T^III)_abd <- (T^I, T^III, T^II)_bad.
+    A dropped 1/4 term, a missed transposition or T^II/T^III left unexchanged fails."""
+    Ms, Me = assembly_maps()
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=(16, 200))
+    u = _sym_fill(_sym_fill(u, 64, (0, 2)), 128, (0, 2))
+    t = np.zeros((16, 192))
+    for c in range(64):
+        a, b, d = c >> 4, (c >> 2) & 3, c & 3
+        t[:, c] = u[:, c]
+        t[:, 64 + c] = u[:, 64 + 16 * b + 4 * a + d] + (0.25 * u[:, 192 + a] if b == d else 0.0)
+        t[:, 128 + c] = u[:, 64 + c] + u[:, 128 + c] + (0.25 * (u[:, 192 + b] + u[:, 196 + b]) if a == d else 0.0)
+    assert np.abs(u @ Ms.T - t @ Me.T).max() < 1e-12
+    tt = rng.normal(size=(16, 192))
+    tt = _sym_fill(_sym_fill(tt, 64, (1, 2)), 128, (0, 2))
+    s = np.zeros_like(tt)
+    for c in range(64):
+        a, b, d = c >> 4, (c >> 2) & 3, c & 3
+        p = 16 * b + 4 * a + d
+        s[:, c], s[:, 64 + c], s[:, 128 + c] = tt[:, p], tt[:, 128 + p], tt[:, 64 + p]
+    rows = np.array([64 * (i // 64) + 16 * (i & 3) + 4 * ((i >> 2) & 3) + ((i >> 4) & 3) for i in range(384)])
+    mulam = -(tt @ Me.T)[:, rows]
+    assert np.abs(mulam - s @ Me.T).max() < 1e-12
+    # the unexchanged form is not the same map
+    s_bad = s.copy()
+    s_bad[:, 64:128], s_bad[:, 128:] = s[:, 128:], s[:, 64:128]
+    assert np.abs(mulam - s_bad @ Me.T).max() > 1e-3
