@@ -2906,7 +2906,7 @@ k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
   __shared__ unsigned sE[64 * KQED_WM_PER_LANE]; /* the Tabd map (map 1) */
   __shared__ unsigned short sRow[384];
   __shared__ D2 sW[AW][WREC + WT];
-  static_assert(WH + 201 <= WT, "helpers and standard entries fit the work area");
+  static_assert(WH + 201 <= WT && WHG + 12 <= WH, "helpers, unit vectors and standard entries fit the work area");
   size_t lo = split[0] > c0 ? split[0] - c0 : 0;
   lo = lo < n ? lo : n;
   const size_t np = (n - lo + 1) / 2; /* record pairs */
@@ -3008,7 +3008,14 @@ k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
     if (sa || sb) {
       H[lane] = std_helper(R, lane, hd0);
       if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
-      wave_sync();
+    }
+    /* the unit vectors of the Tabd slots' arguments, in the free helper slots H[WHG + 4 (s - 1) + i] */
+    if (lane < 4 * (nslots - 1)) {
+      const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (lane >> 2);
+      H[WHG + lane] = pl[lane & 3] / pl[4];
+    }
+    wave_sync();
+    if (sa || sb) {
       for (int c = lane; c < 200; c += 64) Ts[c] = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
       wave_sync();
       /* fold onto T^I, T^II, T^III (reps only) */
@@ -3023,19 +3030,12 @@ k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
       }
       if (!sa) { u0.x = 0.0; u1.x = 0.0; u2.x = 0.0; }
       if (!sb) { u0.y = 0.0; u1.y = 0.0; u2.y = 0.0; }
-      wave_sync(); /* the helper area takes the slots' unit vectors next */
     }
-    /* unit vectors of the Tabd slots' arguments, H[4 (s - 1) + i] */
-    if (lane < 4 * (nslots - 1)) {
-      const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (lane >> 2);
-      H[lane] = pl[lane & 3] / pl[4];
-    }
-    wave_sync();
     for (int s2 = 1; s2 < nslots; s2++) {
       const unsigned ks_a = (kda >> (3 * s2)) & 7u, ks_b = (kdb >> (3 * s2)) & 7u;
       if (ks_a == KIND_NONE && ks_b == KIND_NONE) continue;
       const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (s2 - 1);
-      const D2 *h = H + 4 * (s2 - 1);
+      const D2 *h = H + WHG + 4 * (s2 - 1);
       const D2 cf = R[KQED_REC_COEF + s2];
       /* an empty slot's payload is not read: its component is zero */
       const D2 cc = D2{ ks_a != KIND_NONE ? cf.x : 0.0, ks_b != KIND_NONE ? cf.y : 0.0 };
