@@ -1099,7 +1099,8 @@ static_assert(offsetof(Stage1, V) / 8 == 35 && offsetof(Stage1, V) / 8 + offseto
               "stage-1 layout");
 
 /* header of the stage-1 record (invariants) */
-DEV void write_header(Stage1 *st, const Inv &I)
+template <typename S1>
+DEV void write_header(S1 *st, const Inv &I)
 {
   st->x = I.x; st->cb = I.cb; st->y = I.y; st->xsq = I.xsq; st->ysq = I.ysq; st->xdoty = I.xdoty;
   st->cborig = I.cborig; st->yorig = I.yorig; st->xmysq = I.xmysq; st->xmy = I.xmy;
@@ -1123,11 +1124,11 @@ DEV int surr_bin(const DT &T, double x, double y)
 /* The packed path (simd.c getff6/getff7 + accessv6/7 + extractff/extractff2, FMA build) for one
  * pair: every weight function of chnr_dS/dT/dV into st.  UNI: I's cell (INVx.idx, INVy.idx) and
  * nfw are the same in every active lane, so table addresses are scalar. */
-template <int CLS, int UNI>
-DEV void packed_section(const DT &T, const Inv &I, Stage1 *__restrict__ st, const int nfw[2])
+template <int CLS, int UNI, typename S1>
+DEV void packed_section(const DT &T, const Inv &I, S1 *__restrict__ st, const int nfw[2])
 {
-  Tt &Q = st->T;
-  Vt &D = st->V;
+  auto &Q = st->T;
+  auto &D = st->V;
   double F[4];
   const double twox = 2. * I.cb;
   /* chnr_dS: QG0 */
@@ -1270,6 +1271,43 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     return 0;
   }
   if (CLS == 2) return 1;
+  if constexpr (CLS == 1) {
+    /* the stage-1 values go to private memory, whose lane-interleaved layout makes every store and
+     * load of a field one coalesced access (the record's fields lie 1152 bytes apart from lane to
+     * lane); the algebra then reads them back from there */
+    typedef __attribute__((address_space(5))) Stage1 PS1;
+    typedef __attribute__((address_space(5))) double PD;
+    Stage1 loc;
+    unsigned pa = (unsigned)(size_t)(PS1 *)&loc;
+    asm volatile("" : "+v"(pa)); /* an opaque address: the fields stay in memory, not registers */
+    PS1 *ps = (PS1 *)(size_t)pa;
+    write_header(ps, I);
+    ps->need_alg = 1.0;
+    const int nfw[2] = { wave_max_active(T.nfx[I.INVx.idx]), wave_max_active(T.nfx[I.INVx.idx + 1]) };
+    const int cell = I.INVx.idx * T.nstpy + I.INVy.idx;
+    const int cu = __builtin_amdgcn_readfirstlane(cell);
+    const unsigned long long other = __ballot(cell != cu);
+    if (other == 0) {
+      Inv Iu = I;
+      Iu.INVx.idx = __builtin_amdgcn_readfirstlane(I.INVx.idx);
+      Iu.INVy.idx = __builtin_amdgcn_readfirstlane(I.INVy.idx);
+      packed_section<CLS, 1>(T, Iu, ps, nfw);
+    } else {
+      const int c2 = __builtin_amdgcn_readlane(cell, (int)__builtin_ctzll(other));
+      if (__ballot(cell != cu && cell != c2) == 0) packed_section<CLS, 2>(T, I, ps, nfw);
+      else packed_section<CLS, 0>(T, I, ps, nfw);
+    }
+    const PD *pd = (const PD *)ps;
+    if (fuse) {
+      Stage1 s1;
+      for (int i = 0; i < (int)(sizeof(Stage1) / 8); i++) ((double *)&s1)[i] = pd[i];
+      algebra_finish((double *)st, s1, xv, yv);
+      alg_done = true;
+    } else {
+      for (int i = 0; i < (int)(sizeof(Stage1) / 8); i++) ((double *)st)[i] = pd[i];
+    }
+    return 0;
+  }
   if (CLS != 1 && I.x > T.XX[T.nstpx - 1]) return 1;
   write_header(st, I);
   st->need_alg = 1.0;
