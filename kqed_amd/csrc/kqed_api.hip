@@ -27,10 +27,11 @@
 __global__ void k_build_arrays(kqed_dev_tables T, const float *Fm, const float *Fp, double2 *P, double *PNC, double2 *S);
 __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
 __global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, int mode,
-                           unsigned *key, unsigned *rank, unsigned *count);
+                           unsigned *key, unsigned *rank, unsigned *count, int G);
 __global__ void k_scan_tiles(unsigned *count, unsigned nb, unsigned *tsum);
 __global__ void k_scan_add(unsigned *count, unsigned nb, const unsigned *tsum);
-__global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm);
+__global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm,
+                          int G);
 __global__ void k_algebra(size_t n, double *rec, const unsigned *kinds);
 __global__ void k_inject_records(size_t n, double *rec, const unsigned *kinds, const double *w);
 template <int CLS>
@@ -132,8 +133,19 @@ struct RaggedBufs {
   }
 };
 
+/* sub-counters per sort bucket: a large batch counts each bucket in KQED_SORT_GROUPS counters
+ * (the classification blocks take them in turn), which spreads the atomics on the buckets that most
+ * blocks touch; the scan over (bucket, group) then gives every group its own start */
+#define KQED_SORT_GROUPS 16
+static int sort_groups(size_t n, int gmax)
+{
+  const int g = n >= ((size_t)1 << 22) ? 16 : n >= ((size_t)1 << 20) ? 4 : 1;
+  return g < gmax ? g : gmax;
+}
+
 struct Workspace {
   size_t cap = 0, ncap = 0, nbuckets = 0;
+  int groups = KQED_SORT_GROUPS; /* sub-counters per bucket that count holds */
   unsigned *key = nullptr, *rank = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
   double *rec = nullptr;
   /* contraction: per-pair dots [xcap], per-site pulled-back tensors W [xcap][200], and the small
@@ -537,7 +549,8 @@ extern "C" int initialise(struct QED_kernel_temps *t)
 static size_t batch_workspace(size_t n, size_t chunk, size_t nbuckets)
 {
   const size_t c = n < chunk ? n : chunk;
-  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (nbuckets + (nbuckets + 1023) / 1024);
+  const size_t nc = nbuckets * KQED_SORT_GROUPS;
+  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (nc + (nc + 1023) / 1024);
 }
 
 extern "C" size_t kqed_batch_workspace(size_t n)
@@ -562,7 +575,8 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
   if (w.cap < c || w.nbuckets != nb) {
     (void)hipFree(w.kinds); (void)hipFree(w.rec); (void)hipFree(w.count);
     w.kinds = w.count = nullptr; w.rec = nullptr; w.cap = 0;
-    if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * (nb + (nb + 1023) / 1024)) != hipSuccess ||
+    const size_t nc = nb * KQED_SORT_GROUPS;
+    if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * (nc + (nc + 1023) / 1024)) != hipSuccess ||
         hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
       set_err("workspace allocation of %zu pairs failed", c);
       return -2;
@@ -598,19 +612,22 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
   if (ensure_ws(d, w, n < CH ? n : CH, sort ? n : 0)) return -2;
   const kqed_dev_tables &T = d->dt;
   const unsigned *perm = nullptr;
+  int G = 1; /* sub-counters per bucket: the start of bucket b is count[b * G] */
   StageEvent ev;
   if (sort) {
     /* one counting sort of the whole batch by interpolation cell; passes then walk the sorted
      * order, so every pass sees the densest possible cells */
     const unsigned blocks = (unsigned)((n + 255) / 256);
     prof_begin(d, s, ev, ST_SORT);
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((w.nbuckets + 255) / 256)), dim3(256), 0, s, w.count, w.nbuckets, 0u);
+    G = sort_groups(n, w.groups);
+    const size_t nc = w.nbuckets * G;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, w.count, nc, 0u);
     hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, tu.sort_mode, w.key, w.rank,
-                       w.count);
-    const unsigned ntile = (unsigned)((w.nbuckets + 1023) / 1024);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets, w.count + w.nbuckets);
-    hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)w.nbuckets, w.count + w.nbuckets);
-    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm);
+                       w.count, G);
+    const unsigned ntile = (unsigned)((nc + 1023) / 1024);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)nc, w.count + nc);
+    hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)nc, w.count + nc);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm, G);
     perm = w.perm;
     prof_end(d, s, ev);
   }
@@ -624,13 +641,13 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     const unsigned iblocks = (unsigned)((c + KQED_INTERP_THREADS - 1) / KQED_INTERP_THREADS);
     prof_begin(d, s, ev, ST_INTERP);
     if (perm) {
-      /* count[b] is the start of bucket b: special pairs [0, count[1]), packed path
-       * [count[1], count[1 + nstpx*nstpy]), Taylor-x region [count[1 + nstpx*nstpy], n) */
-      const int ip = 1 + T.nstpx * T.nstpy;
+      /* count[b G] is the start of bucket b: special pairs [0, count[G]), packed path
+       * [count[G], count[(1 + nstpx*nstpy) G]), Taylor-x region [count[(1 + nstpx*nstpy) G], n) */
+      const int ip = (1 + T.nstpx * T.nstpy) * G;
       const unsigned char *smask = sur ? sur->mask : nullptr;
-      hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, 1, c0, c1,
+      hipLaunchKernelGGL(k_interp<0>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, -1, G, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
-      hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(KQED_INTERP_THREADS), KQED_INTERP_THREADS / 64 * KQED_LDS_ARRAYS * T.ks * sizeof(double2), s, T, n, x_dev, xs, y_dev, ys, perm, w.count, 1, ip, c0, c1,
+      hipLaunchKernelGGL(k_interp<1>, dim3(iblocks), dim3(KQED_INTERP_THREADS), KQED_INTERP_THREADS / 64 * KQED_LDS_ARRAYS * T.ks * sizeof(double2), s, T, n, x_dev, xs, y_dev, ys, perm, w.count, G, ip, c0, c1,
                          which, M, w.rec, w.kinds, smask, fuse);
       hipLaunchKernelGGL(k_interp<2>, dim3(iblocks), dim3(KQED_INTERP_THREADS), 0, s, T, n, x_dev, xs, y_dev, ys, perm, w.count, ip, -1, c0, c1,
                          which, M, w.rec, w.kinds, (const unsigned char *)nullptr, fuse);
@@ -666,9 +683,9 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     const bool twom = perm && !Wc &&
                       ((nslots[which] > 1 && (f32 || layout == KQED_AOS || layout == KQED_SOA || layout == LAYOUT_DOT ||
                                               layout == LAYOUT_DOT64)) || dot_all);
-    /* split[0]: the first record the two-pair kernels take (count[1]: past the special pairs; count[0]
+    /* split[0]: the first record the two-pair kernels take (count[G]: past the special pairs; count[0]
      * = 0: every record); the one-pair kernel takes the records before it */
-    const unsigned *split = dot_all ? w.count : two || twom ? w.count + 1 : nullptr;
+    const unsigned *split = dot_all ? w.count : two || twom ? w.count + G : nullptr;
     if (f32)
       hipLaunchKernelGGL(k_assemble<float>, dim3(grid), dim3(64 * KQED_ASM_WAVES), 0, s, T, c, w.rec, w.kinds, out_dev, layout,
                          nslots[which], n, Tc, Wc, split, c0);
@@ -758,6 +775,7 @@ extern "C" int kqed_batch_noalloc(const struct QED_kernel_temps *t, int which, d
   w.cap = KQED_NOALLOC_PASS;
   w.ncap = KQED_NOALLOC_MAX_PAIRS;
   w.nbuckets = nb;
+  w.groups = 1;
   w.rec = (double *)(d->noalloc + NA_REC);
   w.kinds = (unsigned *)(d->noalloc + NA_KINDS);
   w.key = (unsigned *)(d->noalloc + NA_KEY);

@@ -1543,7 +1543,7 @@ DEV int yeq0_prims(const DT &T, const double *xv, double *o)
  * instead of one per pair.  Launch with 256 threads. */
 __global__ void __launch_bounds__(256)
 k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys, int mode,
-           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count)
+           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count, int G)
 {
   __shared__ unsigned sk[256], sp[256], sbase[256];
   __shared__ unsigned long long swb[4];
@@ -1595,7 +1595,9 @@ k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *_
     for (int w2 = wv + 1; w2 < 4; w2++)
       if (swb[w2]) { e = w2 * 64 + __builtin_ctzll(swb[w2]); break; }
   }
-  if (kt != 0xFFFFFFFFu && bound) sbase[t] = atomicAdd(&count[kt], (unsigned)(e - t));
+  /* bucket kt counts in G sub-counters count[kt * G + g], g = block mod G: a bucket that most
+   * blocks touch takes 1/G of the atomics on each address */
+  if (kt != 0xFFFFFFFFu && bound) sbase[t] = atomicAdd(&count[(size_t)kt * G + blockIdx.x % G], (unsigned)(e - t));
   __syncthreads();
   if (kt != 0xFFFFFFFFu) {
     int s0 = 0;
@@ -1645,24 +1647,29 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(unsigned *__restrict__ coun
 
 __global__ void __launch_bounds__(1024) k_scan_add(unsigned *__restrict__ count, unsigned nb, const unsigned *__restrict__ tsum)
 {
-  __shared__ unsigned off;
-  if (threadIdx.x == 0) {
-    unsigned s = 0;
-    for (unsigned k = 0; k < blockIdx.x; k++) s += tsum[k];
-    off = s;
-  }
+  /* the preceding tiles' totals, summed by the whole block (integer sums: any order is exact) */
+  __shared__ unsigned wsum[16];
+  const unsigned t = threadIdx.x;
+  unsigned part = 0;
+  for (unsigned k = t; k < blockIdx.x; k += 1024) part += tsum[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((t & 63) == 0) wsum[t >> 6] = part;
   __syncthreads();
-  const unsigned i = blockIdx.x * 1024 + threadIdx.x;
+  unsigned off = 0;
+  for (int k = 0; k < 16; k++) off += wsum[k];
+  const unsigned i = blockIdx.x * 1024 + t;
   if (i < nb && blockIdx.x) count[i] += off;
 }
 
 /* start[b] is the exclusive prefix of the bucket counts: pair i goes to start[key] + its rank */
 __global__ void k_scatter(size_t n, const unsigned *__restrict__ key, const unsigned *__restrict__ rank,
-                          const unsigned *__restrict__ start, unsigned *__restrict__ perm)
+                          const unsigned *__restrict__ start, unsigned *__restrict__ perm, int G)
 {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  perm[start[key[i]] + rank[i]] = (unsigned)i;
+  /* the sub-counter of pair i's classification block (256 pairs per block) */
+  perm[start[(size_t)key[i] * G + (i >> 8) % G] + rank[i]] = (unsigned)i;
 }
 
 /* ------------------------------------------------------------------ interpolation kernel */
