@@ -1223,10 +1223,15 @@ DEV void packed_section(const DT &T, const Inv &I, S1 *__restrict__ st, const in
 /* Interpolation of one pair into st.  CLS: 0 any pair, 1 packed-path pairs only, 2 Taylor-x pairs
  * only (the sort sends each class to its own kernel, which then carries only its own code path).
  * Returns 1 when the pair is outside the grid (the reference then leaves an all-zero kernel). */
-DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, const double *hy);
+/* private (lane-interleaved) memory: a field's stores and loads across a wavefront are coalesced */
+typedef __attribute__((address_space(5))) double PD;
+template <typename MP>
+DEV void algebra_finish(MP m, Stage1 &s, const double *hx, const double *hy);
+DEV void algebra_wave(double (*t)[17], double *m, Stage1 &s, const double *hx, const double *hy);
 
 template <int CLS>
-DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st, bool fuse, bool &alg_done)
+DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__restrict__ st, bool fuse, bool &alg_done,
+                    double (*tw)[17])
 {
   const Inv I = set_inv(T, xv, yv);
   if (CLS != 1 && (CLS == 2 || I.x < T.XX[0])) {
@@ -1259,7 +1264,9 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
       s1.dg0dx = dg0dx; s1.dg0dy = dg0dy; s1.dg0dcb = dg0dcb;
       s1.T = Q;
       s1.V = D;
-      algebra_finish((double *)st, s1, xv, yv);
+      /* the packed-path and Taylor-region kernels store through their LDS tile */
+      if constexpr (CLS == 0) algebra_finish((double *)st, s1, xv, yv);
+      else algebra_wave(tw, (double *)st, s1, xv, yv);
       alg_done = true;
       return 0;
     }
@@ -1276,7 +1283,6 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
      * load of a field one coalesced access (the record's fields lie 1152 bytes apart from lane to
      * lane); the algebra then reads them back from there */
     typedef __attribute__((address_space(5))) Stage1 PS1;
-    typedef __attribute__((address_space(5))) double PD;
     Stage1 loc;
     unsigned pa = (unsigned)(size_t)(PS1 *)&loc;
     asm volatile("" : "+v"(pa)); /* an opaque address: the fields stay in memory, not registers */
@@ -1301,7 +1307,7 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
     if (fuse) {
       Stage1 s1;
       for (int i = 0; i < (int)(sizeof(Stage1) / 8); i++) ((double *)&s1)[i] = pd[i];
-      algebra_finish((double *)st, s1, xv, yv);
+      algebra_wave(tw, (double *)st, s1, xv, yv);
       alg_done = true;
     } else {
       for (int i = 0; i < (int)(sizeof(Stage1) / 8); i++) ((double *)st)[i] = pd[i];
@@ -1376,7 +1382,7 @@ DEV int interp_eval(const DT &T, const double *xv, const double *yv, Stage1 *__r
 
 /* Chain-rule algebra of chnr_dT/chnr_dV (standard branch), the x <-> x-y swap, and the 36
  * primitives the assembly kernel reads (S: 3, T: 12, invariants: 6, V: 15), written in place. */
-DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, const double *hy)
+DEV void algebra_values(Stage1 &s, const double *hx, const double *hy, double o[64])
 {
   Inv I;
   I.x = s.x; I.cb = s.cb; I.y = s.y; I.xsq = s.xsq; I.ysq = s.ysq; I.xdoty = s.xdoty;
@@ -1392,7 +1398,6 @@ DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, con
   }
   const Tt &Q = s.T;
   const Vt &D = s.V;
-  double o[36];
   o[0] = s.dg0dx; o[1] = s.dg0dy; o[2] = s.dg0dcb;
   o[3] = Q.ell1; o[4] = Q.ell2; o[5] = Q.ell3;
   o[6] = Q.dell1dx; o[7] = Q.dell1dy; o[8] = Q.dell1dcb;
@@ -1403,12 +1408,11 @@ DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, con
   o[26] = D.ddg1dxdx; o[27] = D.ddg2dxdx; o[28] = D.ddg1dxdy; o[29] = D.ddg2dxdy;
   o[30] = D.ddg1dxdcb; o[31] = D.ddg2dxdcb; o[32] = D.ddg1dydcb; o[33] = D.ddg2dydcb;
   o[34] = D.ddg1dcbdcb; o[35] = D.ddg2dcbdcb;
-  for (int i = 0; i < 36; i++) m[i] = o[i];
   /* Per-pair vectors and sums the assembly's helper vectors are built from (std_helper), written
    * after the primitives: xh = x/|x|, yh = y/|y|, c2 = (xh - cb yh)/|y|, c4 = (yh - cb xh)/|x|,
    * e12 = xh/|y| + yh/|x| - cb (xh/|x| + yh/|y|), P21 + P22, P23 + P24, 1/|x|, 1/|y|, 1, 0. */
   const double rx = 1.0 / I.x, ry = 1.0 / I.yorig, cb = I.cborig;
-  double h[26];
+  double *h = o + KQED_HELP_BASE;
   for (int i = 0; i < 4; i++) {
     const double xh = hx[i] * rx, yh = hy[i] * ry;
     h[i] = xh;
@@ -1423,7 +1427,49 @@ DEV void algebra_finish(double *__restrict__ m, Stage1 &s, const double *hx, con
   h[23] = ry;
   h[24] = 1.0;
   h[25] = 0.0;
-  for (int i = 0; i < 26; i++) m[KQED_HELP_BASE + i] = h[i];
+  o[62] = o[63] = 0.0;
+}
+
+template <typename MP>
+DEV void algebra_finish(MP m, Stage1 &s, const double *hx, const double *hy)
+{
+  double o[64];
+  algebra_values(s, hx, hy, o);
+  for (int i = 0; i < 36; i++) m[i] = o[i];
+  for (int i = 0; i < 26; i++) m[KQED_HELP_BASE + i] = o[KQED_HELP_BASE + i];
+}
+
+/* Wavefront-wide store of 16 doubles per lane: lane l's v[0..15] become doubles 0..15 of the
+ * record at base + l * REC_LEN (base: lane 0's record, or the same offset in it), for the lanes in
+ * mask m.  The values pass through the wavefront's LDS tile t[64][17], so each store instruction
+ * writes 128 contiguous bytes of each of four records, where the lanes' own stores would each touch
+ * a different line (the records lie 1152 bytes apart).  The whole wavefront calls it together (lane l
+ * stores field l & 15 of records l >> 4, 4 + (l >> 4), ...). */
+DEV void wave_store16(double (*t)[17], double *base, const double *v, unsigned long long m)
+{
+  const int lane = __lane_id();
+  for (int k = 0; k < 16; k++) t[lane][k] = v[k];
+  wave_sync();
+  for (int j = 0; j < 16; j++) {
+    const int rr = 4 * j + (lane >> 4);
+    if ((m >> rr) & 1) base[(size_t)rr * KQED_REC_LEN + (lane & 15)] = t[rr][lane & 15];
+  }
+  wave_sync();
+}
+
+/* algebra_finish with the outputs of a whole wavefront stored together through its LDS tile t
+ * (wave_store16); a wavefront with lanes missing (the last one of a segment, pairs past the Taylor
+ * table) stores each lane's own record instead */
+DEV void algebra_wave(double (*t)[17], double *m, Stage1 &s, const double *hx, const double *hy)
+{
+  double o[64];
+  algebra_values(s, hx, hy, o);
+  if (__ballot(1) == ~0ull) {
+    double *base = m - __lane_id() * KQED_REC_LEN;
+    for (int c = 0; c < 4; c++) wave_store16(t, base + 16 * c, o + 16 * c, ~0ull);
+  } else {
+    for (int i = 0; i < 62; i++) m[i] = o[i];
+  }
 }
 
 DEV void algebra_main(double *__restrict__ m, const double *hx, const double *hy)
@@ -1687,7 +1733,7 @@ DEV void plan_tabd(const DT &T, int kind, const double *v, double *payload, unsi
 /* L0(a,b) into slot `slot`: regime dispatch of reference kernels.c:82-115 */
 template <int CLS>
 DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, unsigned &kinds, int slot,
-                 const unsigned char *smask, bool fuse = false, bool *alg_done = nullptr)
+                 const unsigned char *smask, bool fuse = false, bool *alg_done = nullptr, double (*tw)[17] = nullptr)
 {
   const bool az = is_zero(a), bz = is_zero(b);
   double *pl = slot == 0 ? rec + KQED_REC_MAIN : rec + KQED_REC_AUX + KQED_AUX_LEN * (slot - 1);
@@ -1709,9 +1755,11 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
     }
   }
   bool done = false;
-  if (interp_eval<CLS>(T, a, b, (Stage1 *)pl, fuse, done) == 0) kinds |= (unsigned)KIND_STD;
+  if (interp_eval<CLS>(T, a, b, (Stage1 *)pl, fuse, done, tw) == 0) kinds |= (unsigned)KIND_STD;
   if (alg_done) *alg_done = done;
 }
+
+DEV void wave_sync();
 
 /* occupancy: the packed-path kernel runs at two waves per SIMD and the Taylor-region kernel at
  * three (both then spill a little: measured faster than fewer waves with more registers), the
@@ -1734,45 +1782,61 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
   lo = lo > c0 ? lo : c0;
   hi = hi < c1 ? hi : c1;
   const size_t i = lo + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const size_t p = perm ? perm[i] : i;
-  const size_t ri = i - c0;
-  double xv[4], yv[4];
-  for (int k = 0; k < 4; k++) { xv[k] = x[xs * p + k]; yv[k] = y[ys * p + k]; }
-  double *r = rec + ri * KQED_REC_LEN;
-  for (int k = 0; k < 4; k++) { r[KQED_REC_X + k] = xv[k]; r[KQED_REC_Y + k] = yv[k]; }
-  unsigned kd = 0;
-  const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
-  double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  bool alg_done = false;
-  plan_L0<CLS>(T, xv, yv, r, kd, 0, smask, fuse != 0, &alg_done);
-  switch (which) {
-  case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
-    plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, yv, yv, r, kd, 2, nullptr); s1 = -0.5; s2 = -0.5; break;
-  case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
-    plan_L0<CLS>(T, z, yv, r, kd, 1, nullptr); plan_L0<CLS>(T, xv, z, r, kd, 2, nullptr); s1 = -1.0; s2 = -1.0; break;
-  case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
-    plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, z, xv, r, kd, 2, nullptr);
-    plan_L0<CLS>(T, z, yv, r, kd, 3, nullptr);
-    s1 = -1.0; s2 = 1.0; s3 = -1.0; break;
-  case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
-    if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
-    if (!is_zero(xv)) plan_tabd(T, KIND_YEQ0_M, xv, r + KQED_REC_AUX + KQED_AUX_LEN, kd, 2);
-    break;
-  default: break;
+  if constexpr (CLS == 0) {
+    if (i >= hi) return;
   }
-  r[KQED_REC_COEF + 0] = 1.0;
-  r[KQED_REC_COEF + 1] = s1;
-  r[KQED_REC_COEF + 2] = s2;
-  r[KQED_REC_COEF + 3] = s3;
-  r[KQED_REC_M] = M;
-  /* the M-L2 damping factors (exp(-0) = 1 exactly, so M = 0 skips the exponentials) */
-  r[KQED_REC_GX] = M != 0.0 ? exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.) : 1.0;
-  r[KQED_REC_GY] = M != 0.0 ? exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.) : 1.0;
-  r[KQED_REC_P] = (double)p;
-  kinds[ri] = kd;
-  /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
-  if (fuse && !alg_done && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
+  /* the packed-path and Taylor-region kernels end with a wavefront-wide copy of the records */
+  const bool act = i < hi;
+  if (__ballot(act) == 0) return;
+  /* CLS 1, 2: the wavefront's LDS tile through which the records' header and the algebra's outputs
+   * reach HBM in whole lines (wave_store16) */
+  __shared__ double tb[KQED_INTERP_THREADS / 64][64][17];
+  double(*tw)[17] = CLS == 0 ? nullptr : tb[threadIdx.x >> 6];
+  const size_t ri = i - c0;
+  double *r = rec + ri * KQED_REC_LEN;
+  double h[16];
+  if (act) {
+    const size_t p = perm ? perm[i] : i;
+    double xv[4], yv[4];
+    for (int k = 0; k < 4; k++) { xv[k] = x[xs * p + k]; yv[k] = y[ys * p + k]; }
+    unsigned kd = 0;
+    const double z[4] = { 0.0, 0.0, 0.0, 0.0 };
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    switch (which) {
+    case 1: /* L1 = L0(x,y) - L0(x,x)/2 - L0(y,y)/2 */
+      plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, yv, yv, r, kd, 2, nullptr); s1 = -0.5; s2 = -0.5; break;
+    case 2: /* L2 = L0(x,y) - L0(0,y) - L0(x,0) */
+      plan_L0<CLS>(T, z, yv, r, kd, 1, nullptr); plan_L0<CLS>(T, xv, z, r, kd, 2, nullptr); s1 = -1.0; s2 = -1.0; break;
+    case 3: /* L3 = L0(x,y) - L0(x,x) + L0(0,x) - L0(0,y) */
+      plan_L0<CLS>(T, xv, xv, r, kd, 1, nullptr); plan_L0<CLS>(T, z, xv, r, kd, 2, nullptr);
+      plan_L0<CLS>(T, z, yv, r, kd, 3, nullptr);
+      s1 = -1.0; s2 = 1.0; s3 = -1.0; break;
+    case 4: /* M-L2 = L0(x,y) - e^{-Mx^2/2}(...L0(0,y)) - e^{-My^2/2}(...L0(x,0)) */
+      if (!is_zero(yv)) plan_tabd(T, KIND_XEQ0_M, yv, r + KQED_REC_AUX, kd, 1);
+      if (!is_zero(xv)) plan_tabd(T, KIND_YEQ0_M, xv, r + KQED_REC_AUX + KQED_AUX_LEN, kd, 2);
+      break;
+    default: break;
+    }
+    bool alg_done = false;
+    plan_L0<CLS>(T, xv, yv, r, kd, 0, smask, fuse != 0, &alg_done, tw);
+    for (int k = 0; k < 4; k++) { h[KQED_REC_X + k] = xv[k]; h[KQED_REC_Y + k] = yv[k]; }
+    h[KQED_REC_COEF + 0] = 1.0;
+    h[KQED_REC_COEF + 1] = s1;
+    h[KQED_REC_COEF + 2] = s2;
+    h[KQED_REC_COEF + 3] = s3;
+    h[KQED_REC_M] = M;
+    /* the M-L2 damping factors (exp(-0) = 1 exactly, so M = 0 skips the exponentials) */
+    h[KQED_REC_GX] = M != 0.0 ? exp(-M * (xv[0] * xv[0] + xv[1] * xv[1] + xv[2] * xv[2] + xv[3] * xv[3]) / 2.) : 1.0;
+    h[KQED_REC_GY] = M != 0.0 ? exp(-M * (yv[0] * yv[0] + yv[1] * yv[1] + yv[2] * yv[2] + yv[3] * yv[3]) / 2.) : 1.0;
+    h[KQED_REC_P] = (double)p;
+    if constexpr (CLS == 0)
+      for (int k = 0; k < 16; k++) r[k] = h[k];
+    kinds[ri] = kd;
+    /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
+    if (fuse && !alg_done && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
+  }
+  /* the whole wavefront stores every active lane's header */
+  if constexpr (CLS != 0) wave_store16(tw, r - (threadIdx.x & 63) * KQED_REC_LEN, h, __ballot(act));
 }
 
 template __global__ void k_interp<0>(DT, size_t, const double *, int, const double *, int, const unsigned *,
