@@ -2588,6 +2588,18 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     wave_sync();
     if (lane == 0) Tv[200] = TV{ 0, 0 };
     const size_t qa = (size_t)R[KQED_REC_P].x, qb = (size_t)R[KQED_REC_P].y;
+    /* the contraction shortcut's site weights: loaded now, so that the helper and tensor phases hide
+     * their latency */
+    double wpa[4], wpb[4];
+    if constexpr (WDOT) {
+      const double *wa = Wc + qa * 200, *wb = Wc + qb * 200;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int c = lane + 64 * k;
+        wpa[k] = c < 200 ? wa[c] : 0.0;
+        wpb[k] = c < 200 ? wb[c] : 0.0;
+      }
+    }
     H[lane] = std_helper(R, lane, hd0);
     if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
     wave_sync();
@@ -2599,12 +2611,11 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     if constexpr (WDOT) {
       /* contraction shortcut (layout 16, FP64 map type): dot of the 200 tensor entries with the
        * site's pulled-back weights, as the one-pair kernel forms it; an empty slot gives 0 */
-      const double *wa = Wc + qa * 200, *wb = Wc + qb * 200;
       double sa = 0.0, sb = 0.0;
       for (int k = 0, c = lane; c < 200; k++, c += 64) {
         const TV t = Tv[std_slot(slw, k)];
-        sa = fma((double)t.x, wa[c], sa);
-        sb = fma((double)t.y, wb[c], sb);
+        sa = fma((double)t.x, wpa[k], sa);
+        sb = fma((double)t.y, wpb[k], sb);
       }
       sa = wave_sum(sa);
       sb = wave_sum(sb);
