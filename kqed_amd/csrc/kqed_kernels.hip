@@ -1761,12 +1761,12 @@ DEV void plan_L0(const DT &T, const double *a, const double *b, double *rec, uns
 
 DEV void wave_sync();
 
-/* occupancy: the packed-path kernel runs at two waves per SIMD and the Taylor-region kernel at
- * three (both then spill a little: measured faster than fewer waves with more registers), the
- * generic kernel at one */
+/* occupancy: the packed-path and Taylor-region kernels run at two waves per SIMD (the packed one
+ * spills a little: measured faster than one wave with more registers; the Taylor one measured 6 %
+ * faster than at three waves, where it spilled ~450 registers), the generic kernel at one */
 template <int CLS> struct InterpWaves { static constexpr int v = 1; };
 template <> struct InterpWaves<1> { static constexpr int v = 2; };
-template <> struct InterpWaves<2> { static constexpr int v = 3; };
+template <> struct InterpWaves<2> { static constexpr int v = 2; };
 
 /* Sorted position i (input position when perm is null) in the window [c0, c1) of this pass and in
  * the class segment [lo, hi), lo = bounds[ilo] (0 when ilo < 0), hi = bounds[ihi] (n when ihi < 0).
