@@ -331,7 +331,8 @@ static int build_wave_maps(kqed_device *d)
       for (int k = 0; k < K; k++)
         for (int l = 0; l < 64; l++) {
           const int r = order[64 * sl + l], b = m.rowptr[r], len = m.rowptr[r + 1] - b;
-          /* entry: byte offset of the column in an FP64 tensor row (col * 8) in the low half, the
+          /* entry: byte offset of the column's slot in a row of two-pair FP64 slots (slot * 16, the
+           * address the two-pair kernels use as it is) in the low half, the
            * upper 16 bits of the coefficient's FP64 bit pattern (+-0.5, +-1, +-2: low 48 bits are
            * zero) in the high half; padding takes the zero slot (col 200) */
           double v = 1.0;
@@ -345,7 +346,7 @@ static int build_wave_maps(kqed_device *d)
           const int slot = q == 0 && col < 200 ? std_slot[col] : col;
           unsigned long long bits;
           memcpy(&bits, &v, 8);
-          E.push_back((unsigned)(bits >> 48) << 16 | (unsigned)(slot * 8));
+          E.push_back((unsigned)(bits >> 48) << 16 | (unsigned)(slot * 16));
         }
     }
     T.wmOff[q][6] = (int)E.size();

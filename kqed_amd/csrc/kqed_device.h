@@ -84,7 +84,7 @@ struct kqed_dev_tables {
   const double2 *S;
   /* assembly maps (0 standard, 1 Tabd, 2 Tabd with mu<->lambda and sign) dealt to the 64 lanes of
    * a wavefront: rows sorted by length, row 64m + l of the sorted order goes to lane l, slot m;
-   * entries [m][k][lane] as u32 = col * 8 | (upper 16 bits of the FP64 coefficient) << 16 with
+   * entries [m][k][lane] as u32 = col * 16 | (upper 16 bits of the FP64 coefficient) << 16 with
    * coefficient +-0.5, +-1 or +-2; padded entries read the zero slot (col 200) */
   const unsigned *wmE;         /* all three maps, concatenated */
   const unsigned short *wmRow; /* [3][6][64] output row of (map, slot, lane) */

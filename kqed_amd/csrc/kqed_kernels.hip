@@ -2036,8 +2036,8 @@ DEV unsigned std_slots(const DT &T, int lane)
 }
 DEV int std_slot(unsigned w, int k) { return (w >> (8 * k)) & 0xFF; }
 
-/* One row of a lane: KB entries (byte offset col*8 of the column | upper 16 bits of the FP64
- * coefficient << 16), unrolled so that every LDS read of the row is in flight at once.  The FP64
+/* One row of a lane: KB entries (byte offset col*16 of the column's two-pair slot | upper 16 bits
+ * of the FP64 coefficient << 16; one pair's FP64 or FP32 rows use half or a quarter of it), unrolled so that every LDS read of the row is in flight at once.  The FP64
  * coefficient is the entry with its low half cleared, as the high word of a double. */
 template <int KB, typename R>
 DEV R wm_dot(const unsigned *__restrict__ e, int lane, const R *Tv)
@@ -2048,7 +2048,7 @@ DEV R wm_dot(const unsigned *__restrict__ e, int lane, const R *Tv)
   R t[KB];
 #pragma unroll
   for (int k = 0; k < KB; k++)
-    t[k] = *(const R *)((const char *)Tv + (sizeof(R) == 8 ? (w[k] & 0xFFFFu) : (w[k] & 0xFFFFu) >> 1));
+    t[k] = *(const R *)((const char *)Tv + ((w[k] & 0xFFFFu) >> (sizeof(R) == 8 ? 1 : 2)));
   R s = 0;
 #pragma unroll
   for (int k = 0; k < KB; k++) {
@@ -2068,7 +2068,7 @@ DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const D2 *Tv)
   for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
   D2 t[KB];
 #pragma unroll
-  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + 2 * (w[k] & 0xFFFFu));
+  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + (w[k] & 0xFFFFu));
   D2 s = D2{ 0.0, 0.0 };
 #pragma unroll
   for (int k = 0; k < KB; k++) {
@@ -2084,7 +2084,7 @@ DEV D2 wm_dot2r(const unsigned *w, const D2 *Tv)
 {
   D2 t[KB];
 #pragma unroll
-  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + 2 * (w[k] & 0xFFFFu));
+  for (int k = 0; k < KB; k++) t[k] = *(const D2 *)((const char *)Tv + (w[k] & 0xFFFFu));
   D2 s = D2{ 0.0, 0.0 };
 #pragma unroll
   for (int k = 0; k < KB; k++) {
@@ -2094,7 +2094,8 @@ DEV D2 wm_dot2r(const unsigned *w, const D2 *Tv)
   return s;
 }
 
-/* the same in FP32 (the map of the FP32 layouts): 8-byte elements, so the byte offset is col * 8 */
+/* the same in FP32 (the map of the FP32 layouts): 8-byte elements, so the byte offset is col * 8,
+ * half the entry's */
 template <int KB>
 DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const F2 *Tv)
 {
@@ -2103,7 +2104,7 @@ DEV D2 wm_dot2(const unsigned *__restrict__ e, int lane, const F2 *Tv)
   for (int k = 0; k < KB; k++) w[k] = e[64 * k + lane];
   F2 t[KB];
 #pragma unroll
-  for (int k = 0; k < KB; k++) t[k] = *(const F2 *)((const char *)Tv + (w[k] & 0xFFFFu));
+  for (int k = 0; k < KB; k++) t[k] = *(const F2 *)((const char *)Tv + ((w[k] & 0xFFFFu) >> 1));
   float sx = 0.0f, sy = 0.0f;
 #pragma unroll
   for (int k = 0; k < KB; k++) {
