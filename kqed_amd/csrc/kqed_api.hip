@@ -633,9 +633,10 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     prof_end(d, s, ev);
   }
   static const int nslots[5] = { 1, 3, 3, 4, 3 };
-  /* the interpolation kernels run the chain-rule algebra themselves unless stage-1 values are
-   * still to be replaced (surrogate) or are the output (extraction) */
-  const int fuse = sur == nullptr && layout != LAYOUT_SCALARS;
+  /* the interpolation kernels run the chain-rule algebra themselves unless the stage-1 values are
+   * the output (extraction); records whose values a surrogate or the caller supplies (flagged
+   * KQED_SURR_BIT) get theirs in k_algebra after the substitution */
+  const int fuse = layout != LAYOUT_SCALARS;
   for (size_t c0 = 0; c0 < n; c0 += CH) {
     const size_t c = n - c0 < CH ? n - c0 : CH, c1 = c0 + c;
     const unsigned blocks = (unsigned)((c + 255) / 256);
@@ -665,7 +666,7 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
       prof_end(d, s, ev);
       continue;
     }
-    if (!fuse) hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
+    if (sur) hipLaunchKernelGGL(k_algebra, dim3(blocks), dim3(256), 0, s, c, w.rec, w.kinds);
     prof_end(d, s, ev);
     const size_t nblk = (c + KQED_ASM_WAVES - 1) / KQED_ASM_WAVES;
     const unsigned grid = (unsigned)(nblk < (size_t)tu.asm_grid ? nblk : (size_t)tu.asm_grid);
@@ -2022,7 +2023,14 @@ extern "C" int kqed_batch_surrogate(const struct QED_kernel_temps *t, const kqed
   if (n > 0xFFFFFFFFull) { set_err("kqed_batch_surrogate: at most 2^32-1 pairs per call"); return -1; }
   kqed_device *d = t->dev;
   SurrUse su = { surr_net(g), g->mask };
+  /* no bin passed its error check: every pair takes the tables, so the plain pipeline gives the
+   * same result */
+  bool any = false;
+  for (unsigned char m : g->h_mask) any = any || m;
   std::lock_guard<std::mutex> lk(d->mu);
+  if (!any)
+    return run_pairs(d, d->ws[stream], (hipStream_t)stream, which, M, n, x_dev, 4, y_dev, 4, out_dev, layout, nullptr,
+                     nullptr);
   kqed_tune keep = d->tune;
   d->tune.sort_min = 0;
   if (d->tune.sort_mode == KQED_SORT_NONE) d->tune.sort_mode = KQED_SORT_CELL;

@@ -1483,7 +1483,9 @@ __global__ void __launch_bounds__(256)
 k_algebra(size_t n, double *__restrict__ rec, const unsigned *__restrict__ kinds)
 {
   const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n || (kinds[p] & 7u) != (unsigned)KIND_STD) return;
+  /* only the records whose stage-1 values came from elsewhere: the interpolation kernels ran the
+   * algebra of every other record themselves */
+  if (p >= n || (kinds[p] & 7u) != (unsigned)KIND_STD || !(kinds[p] & KQED_SURR_BIT)) return;
   double *r = rec + p * KQED_REC_LEN;
   algebra_main(r + KQED_REC_MAIN, r + KQED_REC_X, r + KQED_REC_Y);
 }
@@ -1833,7 +1835,8 @@ k_interp(DT T, size_t n, const double *__restrict__ x, int xs, const double *__r
       for (int k = 0; k < 16; k++) r[k] = h[k];
     kinds[ri] = kd;
     /* fuse: the chain-rule algebra of a standard evaluation right here (no separate pass) */
-    if (fuse && !alg_done && (kd & 7u) == (unsigned)KIND_STD) algebra_main(r + KQED_REC_MAIN, xv, yv);
+    if (fuse && !alg_done && (kd & 7u) == (unsigned)KIND_STD && !(kd & KQED_SURR_BIT))
+      algebra_main(r + KQED_REC_MAIN, xv, yv);
   }
   /* the whole wavefront stores every active lane's header */
   if constexpr (CLS != 0) wave_store16(tw, r - (threadIdx.x & 63) * KQED_REC_LEN, h, __ballot(act));
