@@ -76,8 +76,8 @@ template <int AW, typename MT>
 __global__ void k_assemble2t(kqed_dev_tables T, size_t n, const unsigned *split, size_t c0, const double *rec,
                              const unsigned *kinds, void *out, int layout, int nslots, size_t ntot, const void *Tc);
 __global__ void k_ragged_expand(size_t n, int ng, const unsigned long long *off, const double *y, double *ypair);
-__global__ void k_contract_weights(size_t nx, const float *Tc, const int *cp, const unsigned *ent, const int *sched,
-                                   double *W, double *colpart);
+__global__ void k_contract_weights(size_t nx, const float *Tc, const unsigned *ent, const int *colx, double *W,
+                                   double *colpart);
 __global__ void k_block_sum(size_t n, size_t per, const double *in, double *out);
 __global__ void k_colsum_final(int nb, const double *part, double *sum);
 __global__ void k_combine(int ny, const double *sigma, const double *La, double ca, const double *sumT,
@@ -178,12 +178,11 @@ struct kqed_device {
   void *allocs[32];
   int nalloc = 0;
   kqed_map *maps = nullptr;
-  /* the standard assembly map by column (for the contraction pull-back) */
-  int *csc_cp = nullptr;
-  /* its entries as u32: upper 16 bits of the FP64 coefficient << 16 | row; and the pull-back's
-   * schedule: lane l takes columns csc_sched[65 + j] for j in [csc_sched[l], csc_sched[l + 1]) */
+  /* the standard assembly map by column for the contraction pull-back: entry k of slot m of lane l
+   * at csc_ent[64 (off_m + k) + l] as u32 (upper 16 bits of the FP64 coefficient << 16 | row), the
+   * compact weight index of that column at csc_colx[64 m + l] (-1: no column) */
   unsigned *csc_ent = nullptr;
-  int *csc_sched = nullptr;
+  int *csc_colx = nullptr;
   /* trace tables of the perturbative lepton loop (built on first use) */
   signed char *pert_tab = nullptr;
   /* every surrogate bin set: routes all packed-path pairs through the substitution step */
@@ -467,7 +466,6 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
       /* standard map by column: entries of column c in row order */
       const kqed_map &m0 = d->maps[0];
       std::vector<int> cp(201, 0);
-      if (m0.nnz > 2560) { set_err("standard map has %d entries: the pull-back kernel holds at most 2560", m0.nnz); goto fail; }
       std::vector<unsigned> ent(m0.nnz);
       for (int e = 0; e < m0.nnz; e++) cp[m0.col[e] + 1]++;
       for (int c = 0; c < 200; c++) cp[c + 1] += cp[c];
@@ -480,33 +478,36 @@ extern "C" int kqed_initialise_dir(struct QED_kernel_temps *t, const char *dir)
           if (bits & 0xFFFFFFFFFFFFull) { set_err("standard map coefficient %g not encodable", m0.val[e]); goto fail; }
           ent[k] = (unsigned)(bits >> 48) << 16 | (unsigned)r;
         }
-      /* columns dealt to the 64 lanes of a wavefront longest first, each to the least loaded lane
-       * (load: its entries plus one write), so the lanes of a site finish together */
-      std::vector<int> sched(65 + 200);
-      {
-        std::vector<int> order(200), load(64, 0);
-        std::vector<std::vector<int>> cols(64);
-        for (int c = 0; c < 200; c++) order[c] = c;
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cp[a + 1] - cp[a] > cp[b + 1] - cp[b]; });
-        for (int c : order) {
-          const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-          cols[l].push_back(c);
-          load[l] += cp[c + 1] - cp[c] + 1;
-        }
-        int k = 65;
-        for (int l = 0; l < 64; l++) {
-          sched[l] = k;
-          for (int c : cols[l]) sched[k++] = c;
-        }
-        sched[64] = k;
+      /* columns (those a row reads: exactly the ones with a compact weight index) sorted by length,
+       * longest first, and dealt to lane l of slot m in the order 64 m + l; each slot is unrolled to
+       * the length of its longest column (padding entries have coefficient 0) */
+      std::vector<int> order;
+      for (int c = 0; c < 200; c++) {
+        if ((cp[c + 1] > cp[c]) != (kqed_wcol(c) >= 0)) { set_err("standard map columns differ from the weight layout"); goto fail; }
+        if (cp[c + 1] > cp[c]) order.push_back(c);
       }
-      d->csc_cp = (int *)dev_alloc(d, 4 * 201);
-      d->csc_ent = (unsigned *)dev_alloc(d, 4 * (size_t)m0.nnz);
-      d->csc_sched = (int *)dev_alloc(d, 4 * sched.size());
-      if (!d->csc_cp || !d->csc_ent || !d->csc_sched) { set_err("device allocation failed"); goto fail; }
-      HIPCHK(hipMemcpy(d->csc_cp, cp.data(), 4 * 201, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d->csc_ent, ent.data(), 4 * (size_t)m0.nnz, hipMemcpyHostToDevice));
-      HIPCHK(hipMemcpy(d->csc_sched, sched.data(), 4 * sched.size(), hipMemcpyHostToDevice));
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cp[a + 1] - cp[a] > cp[b + 1] - cp[b]; });
+      {
+        const int K[3] = KQED_PB_K;
+        std::vector<unsigned> ent2(64 * KQED_PB_TOTAL, 0u);
+        std::vector<int> colx(3 * 64, -1);
+        int off = 0;
+        for (int m = 0; m < 3; m++) {
+          const int len0 = 64 * m < (int)order.size() ? cp[order[64 * m] + 1] - cp[order[64 * m]] : 0;
+          if (len0 != K[m] || (m == 2 && (int)order.size() > 192)) { set_err("pull-back slot lengths differ from the kernel's"); goto fail; }
+          for (int l = 0; l < 64 && 64 * m + l < (int)order.size(); l++) {
+            const int c = order[64 * m + l];
+            colx[64 * m + l] = kqed_wcol(c);
+            for (int k = 0; k < cp[c + 1] - cp[c]; k++) ent2[64 * (off + k) + l] = ent[cp[c] + k];
+          }
+          off += K[m];
+        }
+        d->csc_ent = (unsigned *)dev_alloc(d, 4 * ent2.size());
+        d->csc_colx = (int *)dev_alloc(d, 4 * colx.size());
+        if (!d->csc_ent || !d->csc_colx) { set_err("device allocation failed"); goto fail; }
+        HIPCHK(hipMemcpy(d->csc_ent, ent2.data(), 4 * ent2.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d->csc_colx, colx.data(), 4 * colx.size(), hipMemcpyHostToDevice));
+      }
     }
   }
   /* host-visible mirror of the grid (reference Grid_coeffs fields) */
@@ -697,18 +698,21 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
     if (two) {
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2_WAVES - 1) / KQED_ASM2_WAVES;
       const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
+      const size_t nb2d = ((c + 1) / 2 + KQED_ASM2D_WAVES - 1) / KQED_ASM2D_WAVES;
+      const unsigned grid2d = (unsigned)(nb2d < (size_t)tu.asm_grid ? nb2d : (size_t)tu.asm_grid);
       if (f32)
         hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, float, false>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
                            split, c0, w.rec, w.kinds, out_dev, layout, n, (const double *)nullptr);
       else if (Wc)
-        hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double, true>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
+        hipLaunchKernelGGL((k_assemble2<KQED_ASM2D_WAVES, double, true>), dim3(grid2d), dim3(64 * KQED_ASM2D_WAVES), 0, s, T, c,
                            split, c0, w.rec, w.kinds, out_dev, layout, n, Wc);
       else
         hipLaunchKernelGGL((k_assemble2<KQED_ASM2_WAVES, double, false>), dim3(grid2), dim3(64 * KQED_ASM2_WAVES), 0, s, T, c,
                            split, c0, w.rec, w.kinds, out_dev, layout, n, (const double *)nullptr);
     }
-    if (twom && which >= 1 && which <= 3) {
-      /* L1, L2, L3: one Tabd map application on the combined inputs */
+    if (twom && ((which >= 1 && which <= 3) || dot_all)) {
+      /* L1, L2, L3 and the one-slot dot passes over special points: one Tabd map application on the
+       * combined inputs */
       const size_t nb2 = ((c + 1) / 2 + KQED_ASM2T_WAVES - 1) / KQED_ASM2T_WAVES;
       const unsigned grid2 = (unsigned)(nb2 < (size_t)tu.asm_grid ? nb2 : (size_t)tu.asm_grid);
       if (f32)
@@ -1022,7 +1026,7 @@ static int ensure_contract_ws(Workspace &w, size_t nx, size_t ny, bool need_W)
     w.dots = w.W = w.red = nullptr;
     w.xcap = w.ycap = 0;
     if (hipMalloc(&w.dots, 8 * nx) != hipSuccess || hipMalloc(&w.red, 8 * red_len(ny)) != hipSuccess ||
-        (need_W && hipMalloc(&w.W, 8 * 200 * nx) != hipSuccess)) {
+        (need_W && hipMalloc(&w.W, 8 * (size_t)KQED_WCOLS * nx) != hipSuccess)) {
       set_err("contraction workspace allocation (%zu sites) failed", nx);
       return -2;
     }
@@ -1070,8 +1074,8 @@ extern "C" int kqed_contract(const struct QED_kernel_temps *t, int which, double
   /* pull-back of the site tensors; for L1..L3 the same pass sums them per column (sum_x T) */
   const size_t cw_blocks = std::min((nx + 3) / 4, RED_BLOCKS);
   if (shortcut)
-    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)cw_blocks), dim3(256), 0, s, nx, T_dev, d->csc_cp, d->csc_ent,
-                       d->csc_sched, w.W, which != KQED_L0 ? colpart : nullptr);
+    hipLaunchKernelGGL(k_contract_weights, dim3((unsigned)cw_blocks), dim3(256), 0, s, nx, T_dev, d->csc_ent, d->csc_colx,
+                       w.W, which != KQED_L0 ? colpart : nullptr);
   for (size_t j = 0; j < ny && !rc; j++) {
     rc = run_pairs(d, w, s, shortcut ? KQED_L0 : which, M, nx, x_dev, 4, yd + 4 * j, 0, w.dots, LAYOUT_DOT, T_dev,
                    shortcut ? w.W : nullptr);

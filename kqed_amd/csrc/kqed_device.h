@@ -58,9 +58,32 @@
 #define KQED_LDS_ARRAYS 8       /* node arrays per wavefront staged in LDS by the packed-path kernel */
 #define KQED_INTERP_THREADS 128 /* interpolation block: two wavefronts (256 measured 1.5-2% slower) */
 #define KQED_ASM2_WAVES 12  /* wavefronts per two-pair assembly block (LDS 100 KB; 16 measured 18 % slower: VGPR-bound) */
+#ifndef KQED_ASM2D_WAVES
+#define KQED_ASM2D_WAVES 12 /* the contraction shortcut's dot form of the two-pair kernel (no map in LDS; 16 measured 14 % slower: spills) */
+#endif
 #define KQED_ASM2M_WAVES 8 /* multi-slot two-pair assembly: LDS 139 KB (all three maps, record, O) */
 #define KQED_ASM2T_WAVES 12 /* L1-L3 two-pair assembly with one Tabd map application: LDS 112 KB */
 #define KQED_ASM_WAVES 16   /* wavefronts per assembly block: one block (LDS 136 KB) fills a CU */
+
+/* Pulled-back site weights of the contraction shortcut hold only the standard map's columns that
+ * some row reads: the map reads the symmetric T entries (Txv, Tyv: columns 64..191, index alpha beta
+ * delta) only at alpha <= delta, so 152 of the 200 columns.  kqed_wcol: compact index of column c,
+ * or -1 for a column no row reads. */
+#define KQED_WCOLS 152
+/* the pull-back deals these columns, longest first, to three slots of 64 lanes, each unrolled to
+ * its longest column: 26, 18 and 4 entries (checked against the map at init) */
+#define KQED_PB_K { 26, 18, 4 }
+#define KQED_PB_TOTAL 48
+__host__ __device__ inline int kqed_wcol(int c)
+{
+  if (c < 64) return c;
+  if (c >= 192) return c - 48;
+  const int t = (c - 64) >> 6, a = (c >> 4) & 3, b = (c >> 2) & 3, d = c & 3;
+  if (a > d) return -1;
+  /* (a, d) with a <= d in the order (0,0) (0,1) (0,2) (0,3) (1,1) (1,2) (1,3) (2,2) (2,3) (3,3) */
+  const int ad = a * 4 - a * (a - 1) / 2 + (d - a);
+  return 64 + 40 * t + 10 * b + ad;
+}
 
 enum {
   KIND_NONE = 0,

@@ -2270,9 +2270,12 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
       wave_sync();
       if (Wc && kd == KIND_STD) {
         /* contraction shortcut: only slot 0 exists (nslots == 1) */
-        const double *wq = Wc + q * 200;
+        const double *wq = Wc + q * KQED_WCOLS;
         double part = 0.0;
-        for (int k = 0, c = lane; c < 200; k++, c += 64) part = fma((double)Tv[std_slot(slw, k)], wq[c], part);
+        for (int k = 0, c = lane; c < 200; k++, c += 64) {
+          const int cc = kqed_wcol(c);
+          part = fma((double)Tv[std_slot(slw, k)], cc >= 0 ? wq[cc] : 0.0, part);
+        }
         dot = wave_sum(part);
         have_dot = true;
         continue;
@@ -2357,32 +2360,31 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
 /* W[i][c] = sum over the entries (r, v) of column c of the standard assembly map of v * T[i][r]:
  * the site tensor pulled back through the linear assembly (columns in a fixed order, so the sums
  * are deterministic).  One thread per (site, column). */
-/* Pulled-back site tensors of the contraction shortcut: W[i][c] = sum over column c of the standard
- * map of coefficient * T_i[row] (entries in the column's order).  One wavefront per site, the sites
- * of a wavefront strided by the grid; the site's 384 FP32 values are staged in LDS (the next site's
- * are already loaded), the map's columns (nnz <= CW_NNZ, u32 entries: coefficient's upper 16 bits,
- * row) are held in LDS by the block and dealt to the lanes by the host's balanced schedule.  With colpart, the block also sums its sites' tensors per column in
- * a fixed order (each wavefront in site order, then the four wavefronts in order) into
- * colpart[block][384]. */
-#define CW_NNZ 2560
-__global__ void __launch_bounds__(256)
-k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restrict__ cp,
-                   const unsigned *__restrict__ ent, const int *__restrict__ sched, double *__restrict__ W,
-                   double *__restrict__ colpart)
+/* Pulled-back site tensors of the contraction shortcut: W[i][wcol(c)] = sum over column c of the
+ * standard map of coefficient * T_i[row], entries in the column's (row) order, for the 152 columns a
+ * row reads.  One wavefront per site, the sites of a wavefront strided by the grid; the site's 384
+ * FP32 values are staged in LDS (the next site's are already loaded).  The columns are dealt to the
+ * lanes in three slots of fixed length (KQED_PB_K, longest columns first, padding entries with
+ * coefficient 0), the lane's 48 entries held in registers: every step of a slot is one LDS read and
+ * one fma in all lanes, with no per-column loop.  With colpart, the block also sums its sites'
+ * tensors per column in a fixed order (each wavefront in site order, then the four wavefronts in
+ * order) into colpart[block][384]. */
+#ifndef KQED_PB_WAVES
+#define KQED_PB_WAVES 4 /* waves per SIMD of the pull-back (its 48 entries stay in registers) */
+#endif
+__global__ void __launch_bounds__(256, KQED_PB_WAVES)
+k_contract_weights(size_t nx, const float *__restrict__ Tc, const unsigned *__restrict__ ent,
+                   const int *__restrict__ colx, double *__restrict__ W, double *__restrict__ colpart)
 {
-  __shared__ int scp[201];
-  __shared__ unsigned sent[CW_NNZ];
-  __shared__ int ssched[65 + 200];
   __shared__ float st[4][384];
   __shared__ double psum[4][384];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int nnz = cp[200];
-  for (int k = threadIdx.x; k < 201; k += blockDim.x) scp[k] = cp[k];
-  for (int k = threadIdx.x; k < 265; k += blockDim.x) ssched[k] = sched[k];
-  for (int k = threadIdx.x; k < nnz; k += blockDim.x) sent[k] = ent[k];
-  __syncthreads();
-  /* this lane's columns (dealt to balance the lanes' entry counts) */
-  const int j0 = ssched[lane], j1 = ssched[lane + 1];
+  constexpr int K0 = 26, K1 = 18, K2 = 4;
+  static_assert(K0 + K1 + K2 == KQED_PB_TOTAL, "pull-back slot lengths");
+  unsigned e[KQED_PB_TOTAL];
+#pragma unroll
+  for (int k = 0; k < KQED_PB_TOTAL; k++) e[k] = ent[64 * k + lane];
+  const int cx0 = colx[lane], cx1 = colx[64 + lane], cx2 = colx[128 + lane];
   double cs[6] = { 0, 0, 0, 0, 0, 0 };
   const size_t stride = (size_t)gridDim.x * 4;
   size_t i = (size_t)blockIdx.x * 4 + w;
@@ -2390,6 +2392,9 @@ k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restric
   if (i < nx)
 #pragma unroll
     for (int k = 0; k < 6; k++) nt[k] = Tc[i * 384 + lane + 64 * k];
+  auto step = [&](unsigned u, double s) {
+    return fma(__hiloint2double((int)(u & 0xFFFF0000u), 0), (double)st[w][u & 0xFFFFu], s);
+  };
   for (; i < nx; i += stride) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -2400,16 +2405,21 @@ k_contract_weights(size_t nx, const float *__restrict__ Tc, const int *__restric
 #pragma unroll
       for (int k = 0; k < 6; k++) nt[k] = Tc[(i + stride) * 384 + lane + 64 * k];
     wave_sync();
-    double *o = W + i * 200;
-    for (int j = j0; j < j1; j++) {
-      const int c = ssched[j];
-      double s = 0.0;
-      for (int e = scp[c]; e < scp[c + 1]; e++) {
-        const unsigned u = sent[e];
-        s = fma(__hiloint2double((int)(u & 0xFFFF0000u), 0), (double)st[w][u & 0xFFFFu], s);
-      }
-      o[c] = s;
-    }
+    /* the entries are decoded at each use: decoded once outside the loop they would take three
+     * registers each instead of one */
+#pragma unroll
+    for (int k = 0; k < KQED_PB_TOTAL; k++) asm volatile("" : "+v"(e[k]));
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < K0; k++) s0 = step(e[k], s0);
+#pragma unroll
+    for (int k = 0; k < K1; k++) s1 = step(e[K0 + k], s1);
+#pragma unroll
+    for (int k = 0; k < K2; k++) s2 = step(e[K0 + K1 + k], s2);
+    double *o = W + i * KQED_WCOLS;
+    o[cx0] = s0;
+    o[cx1] = s1;
+    if (cx2 >= 0) o[cx2] = s2;
     wave_sync();
   }
   if (!colpart) return;
@@ -2500,9 +2510,12 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   const size_t np = (n - lo + 1) / 2; /* record pairs */
   if ((size_t)blockIdx.x * AW >= np) return;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < 64 * KQED_WM_PER_LANE; i += blockDim.x) sE[i] = T.wmE[i];
-  for (int i = threadIdx.x; i < 384; i += blockDim.x) sRow[i] = T.wmRow[i];
-  __syncthreads();
+  if constexpr (!WDOT) {
+    /* the dot of the contraction shortcut never applies the map */
+    for (int i = threadIdx.x; i < 64 * KQED_WM_PER_LANE; i += blockDim.x) sE[i] = T.wmE[i];
+    for (int i = threadIdx.x; i < 384; i += blockDim.x) sRow[i] = T.wmRow[i];
+    __syncthreads();
+  }
   /* MT: arithmetic of the sparse map (the FP32 layouts run it in FP32, as the one-pair kernel) */
   typedef typename std::conditional<sizeof(MT) == 8, D2, F2>::type TV;
   D2 *R = sW[w], *H = R + KQED_REC_AUX, *O = H;
@@ -2517,7 +2530,7 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   /* the first row slot's 13 map entries stay in registers (a longer set spills) */
   unsigned er[13];
 #pragma unroll
-  for (int k = 0; k < 13; k++) er[k] = sE[64 * k + lane];
+  for (int k = 0; k < 13; k++) er[k] = WDOT ? 0u : sE[64 * k + lane];
   /* the next two records (header and main payload: 80 doubles) wait in registers */
   double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
   unsigned ka = KIND_NONE, kb = KIND_NONE;
@@ -2596,12 +2609,12 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
      * their latency */
     double wpa[4], wpb[4];
     if constexpr (WDOT) {
-      const double *wa = Wc + qa * 200, *wb = Wc + qb * 200;
+      const double *wa = Wc + qa * KQED_WCOLS, *wb = Wc + qb * KQED_WCOLS;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int c = lane + 64 * k;
-        wpa[k] = c < 200 ? wa[c] : 0.0;
-        wpb[k] = c < 200 ? wb[c] : 0.0;
+        const int cc = lane + 64 * k < 200 ? kqed_wcol(lane + 64 * k) : -1;
+        wpa[k] = cc >= 0 ? wa[cc] : 0.0;
+        wpb[k] = cc >= 0 ? wb[cc] : 0.0;
       }
     }
     H[lane] = std_helper(R, lane, hd0);
@@ -2664,7 +2677,7 @@ template __global__ void k_assemble2<KQED_ASM2_WAVES, double, false>(DT, size_t,
                                                                      const unsigned *, void *, int, size_t, const double *);
 template __global__ void k_assemble2<KQED_ASM2_WAVES, float, false>(DT, size_t, const unsigned *, size_t, const double *,
                                                                     const unsigned *, void *, int, size_t, const double *);
-template __global__ void k_assemble2<KQED_ASM2_WAVES, double, true>(DT, size_t, const unsigned *, size_t, const double *,
+template __global__ void k_assemble2<KQED_ASM2D_WAVES, double, true>(DT, size_t, const unsigned *, size_t, const double *,
                                                                     const unsigned *, void *, int, size_t, const double *);
 
 /* The Tabd slots 1..nslots-1 of two records (kinds packed 3 bits per slot) can be applied map by
@@ -3006,7 +3019,9 @@ DEV D2 tabd_value(unsigned k, const D2 *pl, const D2 *h, int c)
 }
 
 /* L1, L2, L3 in sorted order past the special pairs, two records per wavefront, with ONE application
- * of the Tabd map.  Both maps of kqed_assembly.c are the Dirac-trace contraction
+ * of the Tabd map (also the one-slot dot passes whose records are all special points: slot 0 of a
+ * Tabd kind enters as a Tabd slot with coefficient 1, which gives the one-pair kernel's XEQ0/YEQ0
+ * sums exactly).  Both maps of kqed_assembly.c are the Dirac-trace contraction
  * L = sum_A G^A T^(A); the standard map is it on the S/T/V entries folded into
  *   T^I = Vv, T^II_abd = Txv[b][a][d] + 1/4 d_bd Sxv[a], T^III_abd = Txv + Tyv + 1/4 d_ad (Sxv + Syv)[b],
  * and L(v, v) is it on permuted Tabd_yeq0 entries (tabd_value).  The kernel combinations are linear,
@@ -3133,11 +3148,15 @@ k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
       H[lane] = std_helper(R, lane, hd0);
       if (lane + 64 < WHG) H[lane + 64] = std_helper(R, lane + 64, hd1);
     }
-    /* the unit vectors of the Tabd slots' arguments, in the free helper slots H[WHG + 4 (s - 1) + i] */
+    /* the unit vectors of the Tabd slots' arguments, in the free helper slots H[WHG + 4 (s - 1) + i];
+     * a Tabd slot 0 (the one-slot passes of the contraction's x-only sums) in H[WHG + 12 + i] */
+    const unsigned k0a = kda & 7u, k0b = kdb & 7u;
+    const bool t0a = !sa && k0a != KIND_NONE, t0b = !sb && k0b != KIND_NONE;
     if (lane < 4 * (nslots - 1)) {
       const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (lane >> 2);
       H[WHG + lane] = pl[lane & 3] / pl[4];
     }
+    if ((t0a || t0b) && lane >= 12 && lane < 16) H[WHG + lane] = R[KQED_REC_MAIN + lane - 12] / R[KQED_REC_MAIN + 4];
     wave_sync();
     if (sa || sb) {
       for (int c = lane; c < 200; c += 64) Ts[c] = std_entry(R + KQED_REC_MAIN, R + KQED_REC_X, R + KQED_REC_Y, H, c);
@@ -3155,12 +3174,14 @@ k_assemble2t(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, cons
       if (!sa) { u0.x = 0.0; u1.x = 0.0; u2.x = 0.0; }
       if (!sb) { u0.y = 0.0; u1.y = 0.0; u2.y = 0.0; }
     }
-    for (int s2 = 1; s2 < nslots; s2++) {
-      const unsigned ks_a = (kda >> (3 * s2)) & 7u, ks_b = (kdb >> (3 * s2)) & 7u;
+    for (int s2 = t0a || t0b ? 0 : 1; s2 < nslots; s2++) {
+      /* slot 0 counts here only as a Tabd kind (payload in the main slot, coefficient 1) */
+      const unsigned ks_a = s2 ? (kda >> (3 * s2)) & 7u : t0a ? k0a : KIND_NONE;
+      const unsigned ks_b = s2 ? (kdb >> (3 * s2)) & 7u : t0b ? k0b : KIND_NONE;
       if (ks_a == KIND_NONE && ks_b == KIND_NONE) continue;
-      const D2 *pl = R + KQED_REC_AUX + KQED_AUX_LEN * (s2 - 1);
-      const D2 *h = H + WHG + 4 * (s2 - 1);
-      const D2 cf = R[KQED_REC_COEF + s2];
+      const D2 *pl = s2 ? R + KQED_REC_AUX + KQED_AUX_LEN * (s2 - 1) : R + KQED_REC_MAIN;
+      const D2 *h = H + WHG + (s2 ? 4 * (s2 - 1) : 12);
+      const D2 cf = s2 ? R[KQED_REC_COEF + s2] : D2{ 1.0, 1.0 };
       /* an empty slot's payload is not read: its component is zero */
       const D2 cc = D2{ ks_a != KIND_NONE ? cf.x : 0.0, ks_b != KIND_NONE ? cf.y : 0.0 };
       const unsigned k1 = ks_a != KIND_NONE ? ks_a : ks_b;
