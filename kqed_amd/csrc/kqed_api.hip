@@ -148,7 +148,7 @@ struct Workspace {
   int groups = KQED_SORT_GROUPS; /* sub-counters per bucket that count holds */
   unsigned *key = nullptr, *rank = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
   double *rec = nullptr;
-  /* contraction: per-pair dots [xcap], per-site pulled-back tensors W [xcap][200], and the small
+  /* contraction: per-pair dots [xcap], per-site pulled-back tensors W [xcap][KQED_WCOLS], and the small
    * reduction buffers (RED_LEN doubles + 4 * ycap * 384 + ...) */
   size_t xcap = 0, ycap = 0;
   double *dots = nullptr, *W = nullptr, *red = nullptr;
@@ -1013,7 +1013,7 @@ extern "C" int kqed_autotune(const struct QED_kernel_temps *t, int which, double
   return 0;
 }
 
-/* contraction buffers: dots [nx], W [nx][200] (shortcut only) and the reduction scratch `red`:
+/* contraction buffers: dots [nx], W [nx][KQED_WCOLS] (shortcut only) and the reduction scratch `red`:
  * block sums [RED_BLOCKS] | column partials [RED_BLOCKS][384] | sum_x T [384] | consts [2] |
  * zero 4-vector [4] | y [ny][4] | y-only kernels [ny][384] | sigma [ny] | result [ny] */
 static const size_t RED_BLOCKS = 1024;

@@ -2357,9 +2357,6 @@ k_assemble(DT T, size_t n, const double *__restrict__ rec, const unsigned *__res
 }
 
 /* ------------------------------------------------------------------ contraction helpers */
-/* W[i][c] = sum over the entries (r, v) of column c of the standard assembly map of v * T[i][r]:
- * the site tensor pulled back through the linear assembly (columns in a fixed order, so the sums
- * are deterministic).  One thread per (site, column). */
 /* Pulled-back site tensors of the contraction shortcut: W[i][wcol(c)] = sum over column c of the
  * standard map of coefficient * T_i[row], entries in the column's (row) order, for the 152 columns a
  * row reads.  One wavefront per site, the sites of a wavefront strided by the grid; the site's 384
