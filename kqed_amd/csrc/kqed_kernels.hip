@@ -2524,6 +2524,14 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
   };
   const uint2 hd0 = hdesc(lane >> 2), hd1 = hdesc((lane + 64) >> 2);
   const unsigned slw = std_slots(T, lane); /* LDS slots of this lane's standard entries */
+  /* compact weight indices of this lane's entries c = lane + 64 k, one byte each (0xFF: a column no
+   * row reads, weight 0) */
+  unsigned wcx = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int cc = lane + 64 * k < 200 ? kqed_wcol(lane + 64 * k) : -1;
+    wcx |= (cc < 0 ? 0xFFu : (unsigned)cc) << (8 * k);
+  }
   /* the first row slot's 13 map entries stay in registers (a longer set spills) */
   unsigned er[13];
 #pragma unroll
@@ -2601,7 +2609,10 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
     if (q + stride < np) load(q + stride);
     wave_sync();
     if (lane == 0) Tv[200] = TV{ 0, 0 };
-    const size_t qa = (size_t)R[KQED_REC_P].x, qb = (size_t)R[KQED_REC_P].y;
+    /* the pairs' output indices are wave-uniform (scalar registers: the weight loads below then
+     * take one address register each) */
+    const size_t qa = __builtin_amdgcn_readfirstlane((unsigned)R[KQED_REC_P].x),
+                 qb = __builtin_amdgcn_readfirstlane((unsigned)R[KQED_REC_P].y);
     /* the contraction shortcut's site weights: loaded now, so that the helper and tensor phases hide
      * their latency */
     double wpa[4], wpb[4];
@@ -2609,9 +2620,9 @@ k_assemble2(DT T, size_t n, const unsigned *__restrict__ split, size_t c0, const
       const double *wa = Wc + qa * KQED_WCOLS, *wb = Wc + qb * KQED_WCOLS;
 #pragma unroll
       for (int k = 0; k < 4; k++) {
-        const int cc = lane + 64 * k < 200 ? kqed_wcol(lane + 64 * k) : -1;
-        wpa[k] = cc >= 0 ? wa[cc] : 0.0;
-        wpb[k] = cc >= 0 ? wb[cc] : 0.0;
+        const unsigned cc = (wcx >> (8 * k)) & 0xFFu;
+        wpa[k] = cc != 0xFFu ? wa[cc] : 0.0;
+        wpb[k] = cc != 0xFFu ? wb[cc] : 0.0;
       }
     }
     H[lane] = std_helper(R, lane, hd0);
