@@ -364,6 +364,19 @@ def main():
     stages = t.profile(False)
     evals = a.n * a.steps / dt
 
+    # achievable HBM write rate on this GPU, measured live: torch's fill of 2 GiB of the output buffer
+    fill_gbps = None
+    if a.n * 384 >= 1 << 28:
+        buf = out.view(-1)[: 1 << 28]
+        buf.fill_(0.0)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(5):
+            buf.fill_(1.0)
+        ev1.record()
+        torch.cuda.synchronize()
+        fill_gbps = 5 * 8 * (1 << 28) / (ev0.elapsed_time(ev1) * 1e-3) / 1e9
+
     # uniform samples of the measured outputs (copied out before the buffer is reused), checked
     # against the oracle after the GPU legs: 2^18 pairs of the timed kernel, 2^16 of the others
     rng = np.random.default_rng(17)
@@ -443,6 +456,20 @@ def main():
                 "kernel": "one pipeline pass: k_interp<0,1,2> (interpolation and chain-rule algebra) + k_assemble2 / k_assemble",
                 "pairs_per_pass": pairs_per_pass, "pass_ms": avg * 1e3,
                 "flops_per_eval": per_eval, "flops_per_eval_with_node_arrays": flops_per_eval["total"]}
+    roof_asm = None
+    if a.kernel == 0 and cnt_a:
+        # the assembly stage (k_assemble2 + k_assemble of one pass) against HBM: it writes the 384
+        # FP64 outputs of every pair (3072 B) and reads each pair's 640-byte record
+        pairs_per_pass = a.n * a.steps / max(cnt_a, 1)
+        t_pass = ms_a / cnt_a / 1e3
+        alg = (64 + 3072) * pairs_per_pass / t_pass
+        rec = (640 + 3072) * pairs_per_pass / t_pass
+        roof_asm = {"bound": "hbm", "kernel": "assembly stage of one pass: k_assemble2 (two-pair) + k_assemble (special points)",
+                    "achieved": alg / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s", "frac": alg / PEAK_HBM,
+                    "bytes_per_pair": 3136, "bytes_per_pair_note": "algorithmic: x, y in (64 B) + 384 FP64 outputs (3072 B)",
+                    "with_record_reads_gbps": rec / 1e9, "measured_fill_gbps": fill_gbps,
+                    "with_record_reads_frac_of_fill": (rec / 1e9 / fill_gbps) if fill_gbps else None,
+                    "pass_ms": t_pass * 1e3, "pairs_per_pass": pairs_per_pass}
     res = {
         "metric": "L(x,y) evals/s, FP64 plain vs surrogate+FP32 at <=1e-5 max rel err",
         "value": evals, "unit": "evals/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -455,6 +482,7 @@ def main():
         "stages_ms_per_step": {"sort": ms_s / a.steps, "interp": ms_i / a.steps, "assemble": ms_a / a.steps},
         "dominant_kernel": dom,
         "roofline": roof,
+        "roofline_assembly": roof_asm,
         "cpu_baseline": cpu,
     }
     res.update(extra)
