@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import kqed_amd
-from pairs import log_uniform, normwise, regime_mix
+from pairs import iso, log_uniform, normwise, regime_mix
 
 TOL = 1e-12
 
@@ -194,6 +194,35 @@ def test_two_pair_multislot_assembly(gpu_full, oracle_full):
                 assert normwise(got, ref[w]).max() <= TOL, w
     finally:
         gpu_full.set_tune(old)
+
+
+def _on_nodes(rng, nodes, other_lo, other_hi, x_side):
+    """Pairs whose |x| (x_side) or |y| is each node, and the doubles on either side of it."""
+    r = np.concatenate([nodes, np.nextafter(nodes, 0.0), np.nextafter(nodes, np.inf)])
+    a = np.zeros((len(r), 4))
+    a[:, 0] = r  # |a| = sqrt(r*r) = r exactly
+    b = iso(rng, len(r)) * rng.uniform(other_lo, other_hi, (len(r), 1))
+    return (a, b) if x_side else (b, a)
+
+
+def test_grid_nodes_and_neighbours(gpu_full, oracle_full):
+    """Node searches at their boundaries (reference bisection getff-new.c:199-210): |x| on every x
+    node, |y| on every y node and on every node of the Taylor table's y row, and the doubles next to
+    each, through the sorted and the unsorted path."""
+    rng = np.random.default_rng(17)
+    XX, YY, TXY = gpu_full.XX, gpu_full.YY, oracle_full.TX[0]
+    sets = [_on_nodes(rng, XX, 0.05, 3.0, True),
+            _on_nodes(rng, YY, 0.05, 3.0, False),
+            _on_nodes(rng, YY, 1e-3, XX[0], False),
+            _on_nodes(rng, TXY, 1e-3, XX[0], False)]
+    x = np.concatenate([s[0] for s in sets])
+    y = np.concatenate([s[1] for s in sets])
+    ref = oracle_full.batch(0, x, y)
+    got = gpu_full.batch_host(0, x, y)
+    assert len(x) >= 2048  # above the sort threshold
+    assert normwise(got, ref).max() <= TOL
+    sub = rng.choice(len(x), 1500, replace=False)
+    assert normwise(gpu_full.batch_host(0, x[sub], y[sub]), ref[sub]).max() <= TOL
 
 
 def test_deterministic_and_order_independent(gpu_full):
