@@ -160,7 +160,7 @@ struct Workspace {
   double *cdots = nullptr;
   /* lepton loop: pihat1 evaluations [3 pcap][1280], iPihat [pcap][384], scaled pairs [2][pcap][4] */
   size_t pcap = 0;
-  double *pV = nullptr, *pP = nullptr, *pX = nullptr;
+  double *pP = nullptr, *pX = nullptr;
   /* ragged batches enqueued on this stream: group offsets and per-pair y vectors */
   RaggedBufs rb;
 };
@@ -184,7 +184,7 @@ struct kqed_device {
   unsigned *csc_ent = nullptr;
   int *csc_colx = nullptr;
   /* trace tables of the perturbative lepton loop (built on first use) */
-  signed char *pert_tab = nullptr;
+  unsigned *pert_tab = nullptr;
   /* every surrogate bin set: routes all packed-path pairs through the substitution step */
   unsigned char *allmask = nullptr;
   /* device address of this GPU's copy of the allocation-free entry's scratch array (null when
@@ -251,7 +251,7 @@ static void free_ws(Workspace &w)
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
   (void)hipFree(w.cargs); (void)hipFree(w.cL); (void)hipFree(w.cdots);
   w.rb.release();
-  (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
+  (void)hipFree(w.pP); (void)hipFree(w.pX);
   w = Workspace();
 }
 
@@ -1319,11 +1319,11 @@ static int ensure_pert_tab(kqed_device *d, const struct QED_kernel_temps *t, hip
 {
   if (d->pert_tab) return 0;
   if (!t->G8) { set_err("lepton loop: the G8 trace table is missing"); return -1; }
-  std::vector<signed char> h(kqed_pert_table_bytes());
-  kqed_pert_build_table(t->G8, h.data());
-  signed char *p = nullptr;
-  if (hipMalloc(&p, h.size()) != hipSuccess) { set_err("lepton loop: table allocation failed"); return -2; }
-  if (hipMemcpyAsync(p, h.data(), h.size(), hipMemcpyHostToDevice, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+  std::vector<unsigned> h(kqed_pert_table_bytes() / 4);
+  if (kqed_pert_build_table(t->G8, h.data())) { set_err("lepton loop: the G8 table is not a table of gamma traces"); return -1; }
+  unsigned *p = nullptr;
+  if (hipMalloc(&p, 4 * h.size()) != hipSuccess) { set_err("lepton loop: table allocation failed"); return -2; }
+  if (hipMemcpyAsync(p, h.data(), 4 * h.size(), hipMemcpyHostToDevice, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
     (void)hipFree(p);
     set_err("lepton loop: table upload failed");
     return -3;
@@ -1335,10 +1335,9 @@ static int ensure_pert_tab(kqed_device *d, const struct QED_kernel_temps *t, hip
 static int ensure_pert_ws(Workspace &w, size_t c)
 {
   if (w.pcap >= c) return 0;
-  (void)hipFree(w.pV); (void)hipFree(w.pP); (void)hipFree(w.pX);
-  w.pV = w.pP = w.pX = nullptr; w.pcap = 0;
-  if (hipMalloc(&w.pV, 8 * 3 * 1280 * c) != hipSuccess || hipMalloc(&w.pP, 8 * 384 * c) != hipSuccess ||
-      hipMalloc(&w.pX, 64 * c) != hipSuccess) {
+  (void)hipFree(w.pP); (void)hipFree(w.pX);
+  w.pP = w.pX = nullptr; w.pcap = 0;
+  if (hipMalloc(&w.pP, 8 * 384 * c) != hipSuccess || hipMalloc(&w.pX, 64 * c) != hipSuccess) {
     set_err("lepton loop workspace allocation of %zu pairs failed", c);
     return -2;
   }
@@ -1346,7 +1345,7 @@ static int ensure_pert_ws(Workspace &w, size_t c)
   return 0;
 }
 
-/* pairs per lepton-loop pass: 30 KiB of pihat1 evaluations and 3 KiB of iPihat each */
+/* pairs per lepton-loop pass: 3 KiB of iPihat and 64 B of scaled pairs each */
 static const size_t PERT_CHUNK = (size_t)1 << 16;
 
 extern "C" int kqed_pihat(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
@@ -1354,19 +1353,12 @@ extern "C" int kqed_pihat(const struct QED_kernel_temps *t, size_t n, const doub
 {
   if (!t || !t->dev) { set_err("kqed_pihat: tables not initialised"); return -1; }
   if (n == 0) return 0;
-  if (n > 0xFFFFFFFFull / 3) { set_err("kqed_pihat: too many pairs per call"); return -1; }
   kqed_device *d = t->dev;
   std::lock_guard<std::mutex> lk(d->mu);
   hipStream_t s = (hipStream_t)stream;
-  int rc = ensure_pert_tab(d, t, s);
+  const int rc = ensure_pert_tab(d, t, s);
   if (rc) return rc;
-  Workspace &w = d->ws[stream];
-  const size_t CH = n < PERT_CHUNK ? n : PERT_CHUNK;
-  if ((rc = ensure_pert_ws(w, CH))) return rc;
-  for (size_t c0 = 0; c0 < n; c0 += CH) {
-    const size_t c = n - c0 < CH ? n - c0 : CH;
-    kqed_pert_launch_ipihat(c, x_dev + 4 * c0, y_dev + 4 * c0, d->pert_tab, w.pV, out_dev + 384 * c0, s);
-  }
+  kqed_pert_launch_ipihat(n, x_dev, y_dev, d->pert_tab, out_dev, s);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) { set_err("kernel launch failed: %s", hipGetErrorString(e)); return -3; }
   return 0;
@@ -1391,7 +1383,7 @@ extern "C" int kqed_lepton_loop(const struct QED_kernel_temps *t, int which, dou
     const size_t c = n - c0 < CH ? n - c0 : CH;
     double *xs = w.pX, *ys = w.pX + 4 * c;
     kqed_pert_launch_scale(c, Mv, x_dev + 4 * c0, y_dev + 4 * c0, xs, ys, s);
-    kqed_pert_launch_ipihat(c, xs, ys, d->pert_tab, w.pV, w.pP, s);
+    kqed_pert_launch_ipihat(c, xs, ys, d->pert_tab, w.pP, s);
     rc = run_pairs(d, w, s, which, M, c, x_dev + 4 * c0, 4, y_dev + 4 * c0, 4, out_dev + c0, LAYOUT_DOT64, w.pP, nullptr);
     if (rc) return rc;
   }

@@ -7,14 +7,14 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
-/* bytes of the int8 trace tables, and their construction from the reference's G8 table */
+/* bytes of the lepton loop's gather lists, and their construction from the reference's G8 table
+ * (returns -1 when a trace is not 0 or +-4 or a thread has more nonzero terms than the lists hold) */
 size_t kqed_pert_table_bytes(void);
-void kqed_pert_build_table(const int *G8, signed char *tab);
+int kqed_pert_build_table(const int *G8, unsigned *tab);
 /* pihat1 at n pairs: out[n][1280] = v1[4][4][4][4][4], r1[4][4][4][4] per pair */
-void kqed_pert_launch_pihat1(size_t n, const double *x, const double *y, const signed char *tab, double *out, hipStream_t s);
-/* ipihatFermLoop_antisym at n pairs: out[n][384]; scratch holds 3 n * 1280 doubles */
-void kqed_pert_launch_ipihat(size_t n, const double *x, const double *y, const signed char *tab, double *scratch,
-                             double *out, hipStream_t s);
+void kqed_pert_launch_pihat1(size_t n, const double *x, const double *y, const unsigned *tab, double *out, hipStream_t s);
+/* ipihatFermLoop_antisym at n pairs: out[n][384] */
+void kqed_pert_launch_ipihat(size_t n, const double *x, const double *y, const unsigned *tab, double *out, hipStream_t s);
 /* xs = sc x, ys = sc y for n pairs */
 void kqed_pert_launch_scale(size_t n, double sc, const double *x, const double *y, double *xs, double *ys, hipStream_t s);
 
