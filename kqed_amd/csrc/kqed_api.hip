@@ -27,8 +27,8 @@
 __global__ void k_build_arrays(kqed_dev_tables T, const float *Fm, const float *Fp, double2 *P, double *PNC, double2 *S);
 __global__ void k_fill_u32(unsigned *a, size_t n, unsigned v);
 __global__ void k_classify(kqed_dev_tables T, size_t n, const double *x, int xs, const double *y, int ys, int mode,
-                           unsigned *key, unsigned *rank, unsigned *count, int G);
-__global__ void k_scan_tiles(unsigned *count, unsigned nb, unsigned *tsum);
+                           unsigned *key, unsigned *rank, unsigned *gcount, int G, size_t nb);
+__global__ void k_scan_tiles(unsigned *count, unsigned nb, unsigned *tsum, const unsigned *gcount, unsigned G, unsigned nkey);
 __global__ void k_scan_add(unsigned *count, unsigned nb, const unsigned *tsum);
 __global__ void k_scatter(size_t n, const unsigned *key, const unsigned *rank, const unsigned *start, unsigned *perm,
                           int G);
@@ -147,6 +147,9 @@ struct Workspace {
   size_t cap = 0, ncap = 0, nbuckets = 0;
   int groups = KQED_SORT_GROUPS; /* sub-counters per bucket that count holds */
   unsigned *key = nullptr, *rank = nullptr, *count = nullptr, *perm = nullptr, *kinds = nullptr;
+  /* the classification's counters, groups-major (KQED_SORT_GROUPS arrays of nbuckets); null when
+   * the sort runs with one group, which counts in count itself */
+  unsigned *gcount = nullptr;
   double *rec = nullptr;
   /* contraction: per-pair dots [xcap], per-site pulled-back tensors W [xcap][KQED_WCOLS], and the small
    * reduction buffers (RED_LEN doubles + 4 * ycap * 384 + ...) */
@@ -248,6 +251,7 @@ static void *dev_alloc(kqed_device *d, size_t bytes)
 static void free_ws(Workspace &w)
 {
   (void)hipFree(w.key); (void)hipFree(w.rank); (void)hipFree(w.count); (void)hipFree(w.perm); (void)hipFree(w.kinds);
+  (void)hipFree(w.gcount);
   (void)hipFree(w.rec); (void)hipFree(w.dots); (void)hipFree(w.W); (void)hipFree(w.red);
   (void)hipFree(w.cargs); (void)hipFree(w.cL); (void)hipFree(w.cdots);
   w.rb.release();
@@ -552,7 +556,7 @@ static size_t batch_workspace(size_t n, size_t chunk, size_t nbuckets)
 {
   const size_t c = n < chunk ? n : chunk;
   const size_t nc = nbuckets * KQED_SORT_GROUPS;
-  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (nc + (nc + 1023) / 1024);
+  return c * (KQED_REC_LEN * 8 + 4) + 12 * n + 4 * (2 * nc + (nc + 1023) / 1024);
 }
 
 extern "C" size_t kqed_batch_workspace(size_t n)
@@ -575,11 +579,11 @@ static int ensure_ws(kqed_device *d, Workspace &w, size_t c, size_t n)
 {
   const size_t nb = 1 + (size_t)d->dt.nstpx * d->dt.nstpy + d->dt.nstpy;
   if (w.cap < c || w.nbuckets != nb) {
-    (void)hipFree(w.kinds); (void)hipFree(w.rec); (void)hipFree(w.count);
-    w.kinds = w.count = nullptr; w.rec = nullptr; w.cap = 0;
+    (void)hipFree(w.kinds); (void)hipFree(w.rec); (void)hipFree(w.count); (void)hipFree(w.gcount);
+    w.kinds = w.count = w.gcount = nullptr; w.rec = nullptr; w.cap = 0;
     const size_t nc = nb * KQED_SORT_GROUPS;
     if (hipMalloc(&w.kinds, 4 * c) != hipSuccess || hipMalloc(&w.count, 4 * (nc + (nc + 1023) / 1024)) != hipSuccess ||
-        hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
+        hipMalloc(&w.gcount, 4 * nc) != hipSuccess || hipMalloc(&w.rec, 8 * KQED_REC_LEN * c) != hipSuccess) {
       set_err("workspace allocation of %zu pairs failed", c);
       return -2;
     }
@@ -621,13 +625,15 @@ static int run_pairs(kqed_device *d, Workspace &w, hipStream_t s, int which, dou
      * order, so every pass sees the densest possible cells */
     const unsigned blocks = (unsigned)((n + 255) / 256);
     prof_begin(d, s, ev, ST_SORT);
-    G = sort_groups(n, w.groups);
+    G = sort_groups(n, w.gcount ? w.groups : 1);
     const size_t nc = w.nbuckets * G;
-    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, w.count, nc, 0u);
+    unsigned *gc = G > 1 ? w.gcount : w.count;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, gc, nc, 0u);
     hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(256), 0, s, T, n, x_dev, xs, y_dev, ys, tu.sort_mode, w.key, w.rank,
-                       w.count, G);
+                       gc, G, w.nbuckets);
     const unsigned ntile = (unsigned)((nc + 1023) / 1024);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)nc, w.count + nc);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)nc, w.count + nc, (const unsigned *)gc,
+                       (unsigned)G, (unsigned)w.nbuckets);
     hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(1024), 0, s, w.count, (unsigned)nc, w.count + nc);
     hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, n, w.key, w.rank, w.count, w.perm, G);
     perm = w.perm;

@@ -1588,10 +1588,12 @@ DEV int yeq0_prims(const DT &T, const double *xv, double *o)
 /* Keys of 256 pairs per block, and each pair's slot inside its bucket.  The block sorts its keys in
  * LDS (bitonic, key then position) and takes one atomic per run of equal keys, so that buckets
  * holding a large share of the batch (small |y| cells, the Taylor region) see one atomic per block
- * instead of one per pair.  Launch with 256 threads. */
+ * instead of one per pair.  The counters are G arrays of nb buckets, gcount[g * nb + key], block b
+ * counting into array b mod G: concurrent blocks then add into different arrays (measured three
+ * times faster than the G counters of a bucket side by side).  Launch with 256 threads. */
 __global__ void __launch_bounds__(256)
 k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *__restrict__ y, int ys, int mode,
-           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ count, int G)
+           unsigned *__restrict__ key, unsigned *__restrict__ rank, unsigned *__restrict__ gcount, int G, size_t nb)
 {
   __shared__ unsigned sk[256], sp[256], sbase[256];
   __shared__ unsigned long long swb[4];
@@ -1643,9 +1645,9 @@ k_classify(DT T, size_t n, const double *__restrict__ x, int xs, const double *_
     for (int w2 = wv + 1; w2 < 4; w2++)
       if (swb[w2]) { e = w2 * 64 + __builtin_ctzll(swb[w2]); break; }
   }
-  /* bucket kt counts in G sub-counters count[kt * G + g], g = block mod G: a bucket that most
+  /* bucket kt counts in G sub-counters gcount[g * nb + kt], g = block mod G: a bucket that most
    * blocks touch takes 1/G of the atomics on each address */
-  if (kt != 0xFFFFFFFFu && bound) sbase[t] = atomicAdd(&count[(size_t)kt * G + blockIdx.x % G], (unsigned)(e - t));
+  if (kt != 0xFFFFFFFFu && bound) sbase[t] = atomicAdd(&gcount[(size_t)(blockIdx.x % G) * nb + kt], (unsigned)(e - t));
   __syncthreads();
   if (kt != 0xFFFFFFFFu) {
     int s0 = 0;
@@ -1665,16 +1667,18 @@ __global__ void k_fill_u32(unsigned *__restrict__ a, size_t n, unsigned v)
   if (i < n) a[i] = v;
 }
 
-/* exclusive scan of the bucket counts, one block */
 /* Exclusive prefix of the nb bucket counts, in two launches over tiles of 1024 counters: k_scan_tiles
  * turns each tile into its own exclusive prefix and writes the tile's total to tsum[tile];
- * k_scan_add adds the sum of the preceding tiles' totals.  Coalesced loads and stores. */
-__global__ void __launch_bounds__(1024) k_scan_tiles(unsigned *__restrict__ count, unsigned nb, unsigned *__restrict__ tsum)
+ * k_scan_add adds the sum of the preceding tiles' totals.  Counter i = G key + g (bucket-major, the
+ * order of the starts) is read from the classification's arrays at gcount[g nkey + key]; with G = 1
+ * gcount may be count itself.  Coalesced stores. */
+__global__ void __launch_bounds__(1024) k_scan_tiles(unsigned *count, unsigned nb, unsigned *__restrict__ tsum,
+                                                     const unsigned *gcount, unsigned G, unsigned nkey)
 {
   __shared__ unsigned wsum[16];
   const unsigned t = threadIdx.x, lane = t & 63, w = t >> 6;
   const unsigned i = blockIdx.x * 1024 + t;
-  const unsigned v = i < nb ? count[i] : 0u;
+  const unsigned v = i < nb ? gcount[(size_t)(i % G) * nkey + i / G] : 0u;
   unsigned inc = v; /* inclusive prefix within the wavefront */
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) {
