@@ -1352,7 +1352,7 @@ static int ensure_pert_ws(Workspace &w, size_t c)
 }
 
 /* pairs per lepton-loop pass: 3 KiB of iPihat and 64 B of scaled pairs each */
-static const size_t PERT_CHUNK = (size_t)1 << 16;
+static const size_t PERT_CHUNK = (size_t)1 << 18;
 
 extern "C" int kqed_pihat(const struct QED_kernel_temps *t, size_t n, const double *x_dev, const double *y_dev,
                           double *out_dev, void *stream)

@@ -202,3 +202,32 @@ def test_gpu_lepton_loop_integrand(gpu_small, oracle_small, olib):
         scale = np.abs(L * ref_p).sum(axis=1) + 1e-300
         err = (np.abs(out.cpu().numpy() - ref) / scale).max()
         assert err < 1e-12, (which, err)
+
+
+@pytest.mark.gpu
+def test_gpu_lepton_loop_passes_and_pieces(gpu_small):
+    """A batch longer than one lepton-loop pass (2^18 pairs) gives, pair by pair, what separate calls
+    on its first and last 3000 pairs give (each pair's iPihat and its dot with L0 are independent of
+    the batch around it; every call is above the sort threshold), and kqed_pihat agrees the same way."""
+    import torch
+    n = (1 << 18) + 3001
+    X, Y = points(21, n)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def loop(sel):
+        xd = torch.from_numpy(np.ascontiguousarray(X[sel])).to(dev)
+        yd = torch.from_numpy(np.ascontiguousarray(Y[sel])).to(dev)
+        out = torch.zeros(len(xd), dtype=torch.float64, device=dev)
+        gpu_small.lepton_loop_device(0, len(xd), xd.data_ptr(), yd.data_ptr(), out.data_ptr(), Mv=1.5, stream=st)
+        p = torch.zeros((len(xd), 384), dtype=torch.float64, device=dev)
+        gpu_small.pihat_device(len(xd), xd.data_ptr(), yd.data_ptr(), p.data_ptr(), stream=st)
+        torch.cuda.synchronize()
+        return out.cpu().numpy(), p.cpu().numpy()
+
+    print(f"lepton loop passes n={n}", flush=True)
+    whole, pw = loop(slice(0, n))
+    head, ph = loop(slice(0, 3000))
+    tail, pt = loop(slice(n - 3000, n))
+    assert np.array_equal(whole[:3000], head) and np.array_equal(whole[-3000:], tail)
+    assert np.array_equal(pw[:3000], ph) and np.array_equal(pw[-3000:], pt)
