@@ -161,7 +161,7 @@ struct Workspace {
   /* combination contraction: per-pair dots [NK][nx], y points, partial and final sums */
   size_t cdcap = 0;
   double *cdots = nullptr;
-  /* lepton loop: pihat1 evaluations [3 pcap][1280], iPihat [pcap][384], scaled pairs [2][pcap][4] */
+  /* lepton loop: iPihat [pcap][384], scaled pairs [2][pcap][4] */
   size_t pcap = 0;
   double *pP = nullptr, *pX = nullptr;
   /* ragged batches enqueued on this stream: group offsets and per-pair y vectors */
