@@ -205,10 +205,12 @@ def _on_nodes(rng, nodes, other_lo, other_hi, x_side):
     return (a, b) if x_side else (b, a)
 
 
-def test_grid_nodes_and_neighbours(gpu_full, oracle_full):
+@pytest.mark.parametrize("which,M", [(0, 0.0), (1, 0.0), (2, 0.0), (3, 0.0), (4, 0.7)])
+def test_grid_nodes_and_neighbours(gpu_full, oracle_full, which, M):
     """Node searches at their boundaries (reference bisection getff-new.c:199-210): |x| on every x
     node, |y| on every y node and on every node of the Taylor table's y row, and the doubles next to
-    each, through the sorted and the unsorted path."""
+    each, through the sorted and the unsorted path, for L0..L3 and M-L2 (whose subtraction terms put
+    the same nodes into the x = 0, y = 0 and x = y branches)."""
     rng = np.random.default_rng(17)
     XX, YY, TXY = gpu_full.XX, gpu_full.YY, oracle_full.TX[0]
     sets = [_on_nodes(rng, XX, 0.05, 3.0, True),
@@ -217,12 +219,12 @@ def test_grid_nodes_and_neighbours(gpu_full, oracle_full):
             _on_nodes(rng, TXY, 1e-3, XX[0], False)]
     x = np.concatenate([s[0] for s in sets])
     y = np.concatenate([s[1] for s in sets])
-    ref = oracle_full.batch(0, x, y)
-    got = gpu_full.batch_host(0, x, y)
+    ref = oracle_full.batch(which, x, y, M=M)
+    got = gpu_full.batch_host(which, x, y, M=M)
     assert len(x) >= 2048  # above the sort threshold
     assert normwise(got, ref).max() <= TOL
     sub = rng.choice(len(x), 1500, replace=False)
-    assert normwise(gpu_full.batch_host(0, x[sub], y[sub]), ref[sub]).max() <= TOL
+    assert normwise(gpu_full.batch_host(which, x[sub], y[sub], M=M), ref[sub]).max() <= TOL
 
 
 def test_deterministic_and_order_independent(gpu_full):
