@@ -95,15 +95,16 @@ __device__ void pert_args(int k, const double *x, const double *y, double *a, do
   }
 }
 
-/* Bessel functions of the nev evaluations of pair pr (pi_pert.c:239-242): lane 3 ev + w of the
- * first wavefront takes K0, K1 / |v|, K2 / |v|^2 at v = b, a, a - b of evaluation ev's arguments
- * (a, b), all in parallel, into bes[ev][w] */
-__device__ __forceinline__ void pihat1_bessel(size_t pr, int nev, const double *__restrict__ X, const double *__restrict__ Y,
-                                              double (*bes)[3][3])
+/* Bessel functions of the nev evaluations of the ng pairs p0 .. p0 + ng - 1 (pi_pert.c:239-242):
+ * thread 9 j + 3 ev + w takes K0, K1 / |v|, K2 / |v|^2 at v = b, a, a - b of evaluation ev's
+ * arguments (a, b) of pair p0 + j, all in parallel, into bes[j][ev][w] */
+__device__ __forceinline__ void pihat1_bessel(size_t p0, int ng, int nev, const double *__restrict__ X,
+                                              const double *__restrict__ Y, double (*bes)[3][3][3])
 {
   const int t = threadIdx.x;
-  if (t < 3 * nev) {
-    const int ev = t / 3, w = t % 3;
+  if (t < 9 * ng && t % 9 < 3 * nev) {
+    const int j = t / 9, ev = (t % 9) / 3, w = t % 3;
+    const size_t pr = p0 + j;
     double px[4], py[4], a[4], b[4], v[4];
     for (int i = 0; i < 4; i++) { px[i] = X[4 * pr + i]; py[i] = Y[4 * pr + i]; }
     pert_args(ev, px, py, a, b);
@@ -111,9 +112,9 @@ __device__ __forceinline__ void pihat1_bessel(size_t pr, int nev, const double *
     const double vn = norm4(v);
     double k[3];
     bessk2(vn, k);
-    bes[ev][w][0] = k[0];
-    bes[ev][w][1] = k[1] / vn;
-    bes[ev][w][2] = k[2] / (vn * vn);
+    bes[j][ev][w][0] = k[0];
+    bes[j][ev][w][1] = k[1] / vn;
+    bes[j][ev][w][2] = k[2] / (vn * vn);
   }
 }
 
@@ -257,54 +258,60 @@ __global__ void __launch_bounds__(256) k_pihat1(size_t n, const double *__restri
                                                 const unsigned *__restrict__ tab, double *__restrict__ out)
 {
   __shared__ PiConst C;
-  __shared__ double bes[1][3][3];
+  __shared__ double bes[1][3][3][3];
   unsigned e[PT_ENT / 4];
   load_lists(tab, e);
   for (size_t pr = blockIdx.x; pr < n; pr += gridDim.x) {
-    pihat1_bessel(pr, 1, X, Y, bes);
+    pihat1_bessel(pr, 1, 1, X, Y, bes);
     __syncthreads();
-    pihat1_item(pr, 0, X, Y, e, bes[0], C, out + pr * 1280);
+    pihat1_item(pr, 0, X, Y, e, bes[0][0], C, out + pr * 1280);
   }
 }
 
-/* ipihatFermLoop_antisym (pi_pert.c:180-220) at pairs blockIdx.x, + gridDim.x, ...: the three
- * pihat1 evaluations of a pair into LDS, then its (rho, sigma)-antisymmetrised combination
- * (pi_pert.c:196-218) from there into out[pair][384], rows (rho, sigma) = 01 02 03 12 13 23 */
+/* ipihatFermLoop_antisym (pi_pert.c:180-220) over groups of PT_GROUP consecutive pairs, group
+ * blockIdx.x, + gridDim.x, ...: the Bessel values of a group's pairs at once (one latency per
+ * group), then per pair the three pihat1 evaluations into LDS and the (rho, sigma)-antisymmetrised
+ * combination (pi_pert.c:196-218) from there into out[pair][384], rows (rho, sigma) = 01 02 03 12
+ * 13 23.  A group's Bessel values are written after every thread has passed the last evaluation
+ * of the previous group (whose tensor phase read them); P is read by the combination before the
+ * next evaluation's first barrier. */
+constexpr int PT_GROUP = 16;
 __global__ void __launch_bounds__(256, 3) k_ipihat(size_t n, const double *__restrict__ X, const double *__restrict__ Y,
-                                                const unsigned *__restrict__ tab, double *__restrict__ out)
+                                                   const unsigned *__restrict__ tab, double *__restrict__ out)
 {
   __shared__ PiConst C;
-  __shared__ double bes[3][3][3];
+  __shared__ double bes[PT_GROUP][3][3][3];
   __shared__ double P[3 * 1280];
   const int t = threadIdx.x;
   unsigned e[PT_ENT / 4];
   load_lists(tab, e);
-  for (size_t pr = blockIdx.x; pr < n; pr += gridDim.x) {
-    /* the Bessel values are read by the evaluations' tensor phases, all past the barrier that
-     * ends the previous pair's last evaluation; P is read by the combination below, before the
-     * next pair's first barrier */
-    pihat1_bessel(pr, 3, X, Y, bes);
+  for (size_t p0 = (size_t)blockIdx.x * PT_GROUP; p0 < n; p0 += (size_t)gridDim.x * PT_GROUP) {
+    const int ng = n - p0 < (size_t)PT_GROUP ? (int)(n - p0) : PT_GROUP;
+    pihat1_bessel(p0, ng, 3, X, Y, bes);
     __syncthreads();
+    for (int j = 0; j < ng; j++) {
+      const size_t pr = p0 + j;
 #pragma unroll 1
-    for (int ev = 0; ev < 3; ev++) pihat1_item(pr, ev, X, Y, e, bes[ev], C, P + 1280 * ev);
-    double x[4];
-    for (int i = 0; i < 4; i++) x[i] = X[4 * pr + i];
-    const double *v1 = P, *v2 = P + 1280, *v3 = P + 2560;
-    const double *r2 = v2 + 1024, *r3 = v3 + 1024;
+      for (int ev = 0; ev < 3; ev++) pihat1_item(pr, ev, X, Y, e, bes[j][ev], C, P + 1280 * ev);
+      double x[4];
+      for (int i = 0; i < 4; i++) x[i] = X[4 * pr + i];
+      const double *v1 = P, *v2 = P + 1280, *v3 = P + 2560;
+      const double *r2 = v2 + 1024, *r3 = v3 + 1024;
 #define V5(A, a, b, c, d, e) A[256 * (a) + 64 * (b) + 16 * (c) + 4 * (d) + (e)]
 #define R4(A, b, c, d, e) A[64 * (b) + 16 * (c) + 4 * (d) + (e)]
-    for (int oi = t; oi < 384; oi += 256) {
-      const int rs = oi >> 6, mu = (oi >> 4) & 3, nu = (oi >> 2) & 3, ld = oi & 3;
-      const int rho = rs < 3 ? 0 : rs < 5 ? 1 : 2, sig = rs < 3 ? rs + 1 : rs < 5 ? rs - 1 : 3;
-      const double xr = comp(x, rho), xs = comp(x, sig);
-      out[pr * 384 + oi] =
-          0.5 * (+V5(v1, rho, mu, nu, ld, sig) + V5(v2, rho, nu, ld, mu, sig) + xr * R4(r2, nu, ld, mu, sig) +
-                 V5(v3, rho, ld, nu, mu, sig) + xr * R4(r3, ld, nu, mu, sig) -
-                 (+V5(v1, sig, mu, nu, ld, rho) + V5(v2, sig, nu, ld, mu, rho) + xs * R4(r2, nu, ld, mu, rho) +
-                  V5(v3, sig, ld, nu, mu, rho) + xs * R4(r3, ld, nu, mu, rho)));
-    }
+      for (int oi = t; oi < 384; oi += 256) {
+        const int rs = oi >> 6, mu = (oi >> 4) & 3, nu = (oi >> 2) & 3, ld = oi & 3;
+        const int rho = rs < 3 ? 0 : rs < 5 ? 1 : 2, sig = rs < 3 ? rs + 1 : rs < 5 ? rs - 1 : 3;
+        const double xr = comp(x, rho), xs = comp(x, sig);
+        out[pr * 384 + oi] =
+            0.5 * (+V5(v1, rho, mu, nu, ld, sig) + V5(v2, rho, nu, ld, mu, sig) + xr * R4(r2, nu, ld, mu, sig) +
+                   V5(v3, rho, ld, nu, mu, sig) + xr * R4(r3, ld, nu, mu, sig) -
+                   (+V5(v1, sig, mu, nu, ld, rho) + V5(v2, sig, nu, ld, mu, rho) + xs * R4(r2, nu, ld, mu, rho) +
+                    V5(v3, sig, ld, nu, mu, rho) + xs * R4(r3, ld, nu, mu, rho)));
+      }
 #undef V5
 #undef R4
+    }
   }
 }
 
@@ -380,6 +387,11 @@ int kqed_pert_build_table(const int *G8, unsigned *tab)
 
 /* blocks of the pair loops: at most eight per CU of the 256 */
 static unsigned pert_grid(size_t n) { return (unsigned)(n < 2048 ? n : 2048); }
+static unsigned pert_groups(size_t n)
+{
+  const size_t g = (n + PT_GROUP - 1) / PT_GROUP;
+  return (unsigned)(g < 2048 ? g : 2048);
+}
 
 void kqed_pert_launch_pihat1(size_t n, const double *x, const double *y, const unsigned *tab, double *out, hipStream_t s)
 {
@@ -388,7 +400,7 @@ void kqed_pert_launch_pihat1(size_t n, const double *x, const double *y, const u
 
 void kqed_pert_launch_ipihat(size_t n, const double *x, const double *y, const unsigned *tab, double *out, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_ipihat, dim3(pert_grid(n)), dim3(256), 0, s, n, x, y, tab, out);
+  hipLaunchKernelGGL(k_ipihat, dim3(pert_groups(n)), dim3(256), 0, s, n, x, y, tab, out);
 }
 
 void kqed_pert_launch_scale(size_t n, double sc, const double *x, const double *y, double *xs, double *ys, hipStream_t s)
